@@ -1,0 +1,119 @@
+/*
+ * seisfat2d_eik.h — C ABI of the MI355X fast-sweeping eikonal solver.
+ *
+ * Drop-in boundary for SeisFAT2D's eikonal modeling path.  Every entry point
+ * names the reference interface it replaces (paths relative to the SeisFAT2D
+ * repository).  Conventions:
+ *   - status: 0 on success, a negative SEISFAT_E* code on failure; the message
+ *     of the last failure on the calling thread is seisfat_last_error().
+ *   - grids are float32, z-fastest (index = i + j*n_rows), as the reference
+ *     stores them (src/modeling/modeling.cu:209-212).  "Padded" grids carry the
+ *     reference's nb = 3 replicated border: (nz+6) x (nx+6) nodes.
+ *   - positions are in metres from the model origin, x along columns and z
+ *     along rows, as in the reference's SPS/RPS files.
+ *   - a handle is used by one host thread at a time; buffers are caller owned.
+ *   - "stream" arguments are hipStream_t values passed as void* (NULL = the
+ *     handle's own stream).
+ */
+#ifndef SEISFAT2D_EIK_H
+#define SEISFAT2D_EIK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEISFAT_OK 0
+#define SEISFAT_EINVAL (-1) /* invalid argument (sizes, positions, slowness) */
+#define SEISFAT_EHIP (-2)   /* HIP runtime error */
+#define SEISFAT_ENOMEM (-3) /* device or host allocation failed */
+#define SEISFAT_ESTATE (-4) /* call out of order (e.g. solve before set_slowness) */
+#define SEISFAT_EIO (-5)    /* file I/O error */
+
+/* Model type: the reference's plugin index (src/modeling_main.cpp:6-15). */
+#define SEISFAT_ISO 0 /* Eikonal_ISO (src/modeling/eikonal_iso.cu) */
+#define SEISFAT_ANI 1 /* Eikonal_ANI (src/modeling/eikonal_ani.cu) */
+
+/* Sweep schedule of a solve.  Both compute the reference's single pass of four
+ * sweeps (src/modeling/modeling.cu:123-159) and give bit-identical traveltimes. */
+#define SEISFAT_SCHED_PLAIN 0 /* one launch per anti-diagonal: the reference's schedule */
+#define SEISFAT_SCHED_TILED 1 /* 64x64 skewed LDS tiles, one launch per tile diagonal */
+
+typedef struct seisfat_eik_s* seisfat_eik_t;
+typedef struct seisfat_batch_s* seisfat_batch_t;
+
+/* One shot of a batch.  The shot is solved on the model columns
+ * [x0, x0+width) as if that window were the whole model file (nb = 3
+ * replicated border around the window); sx is still measured from the model
+ * origin.  Receivers rec_offset .. rec_offset+nrec-1 of the batch's receiver
+ * arrays are picked from the shot's traveltimes. */
+typedef struct {
+  float sx, sz;
+  int32_t x0, width;
+  int32_t rec_offset, nrec;
+} seisfat_shot;
+
+/* Last error message of the calling thread ("" if none). */
+const char* seisfat_last_error(void);
+
+/* Model geometry: Modeling::set_parameters (src/modeling/modeling.cu:3-36)
+ * reading x_samples, z_samples, x_spacing, z_spacing.  Allocates the model on
+ * the current HIP device. */
+int seisfat_eik_create(int nz, int nx, float dz, float dx, seisfat_eik_t* out);
+int seisfat_eik_destroy(seisfat_eik_t h);
+
+/* Slowness model, nz*nx unpadded, strictly positive and finite.
+ * Replaces Modeling::set_properties + expand_boundary + copy_slowness_to_device
+ * (src/modeling/modeling.cu:47-60,204-232,281-284).  The host variant copies
+ * from host memory; the device variant from a device pointer on `stream`. */
+int seisfat_eik_set_slowness(seisfat_eik_t h, const float* s);
+int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* stream);
+
+/* Anisotropic stiffness C11, C13, C15, C33, C35, C55 (each nz*nx unpadded, in
+ * the reference's units).  Replaces Eikonal_ANI::set_stiffness_element
+ * (src/modeling/eikonal_ani.cu:3-16): pad, then uint16 compression with exact
+ * min/max (src/modeling/modeling.cu:264-279). */
+int seisfat_eik_set_stiffness(seisfat_eik_t h, const float* const c[6]);
+
+/* One shot on the whole model: Eikonal_ISO/ANI::time_propagation
+ * (src/modeling/eikonal_iso.cu:9-13, src/modeling/eikonal_ani.cu:33-45)
+ * followed by Modeling::compute_seismogram (src/modeling/modeling.cu:161-194).
+ * T_padded (nullable, host) receives the padded (nz+6)*(nx+6) traveltimes;
+ * picks (nullable when nrec = 0, host) receive the Catmull-Rom receiver
+ * times.  Synchronous. */
+int seisfat_eik_solve(seisfat_eik_t h, int type, int schedule, float sx, float sz,
+                      float* T_padded, const float* rx, const float* rz, int nrec,
+                      float* picks);
+
+/* Device pointer of the padded traveltimes of the last seisfat_eik_solve (the
+ * reference's public Modeling::d_T, read by migration,
+ * src/migration/migration.cu:130,167).  Valid until the next solve/destroy. */
+const float* seisfat_eik_device_T(seisfat_eik_t h);
+
+/* Batched solve of many shots (no reference counterpart: the reference loops
+ * shots one at a time, src/modeling_main.cpp:21-29).  create() validates the
+ * shots, allocates every shot's traveltime grid and, for the tiled schedule,
+ * records the whole solve (initialisation, all sweeps, picks) as one HIP
+ * graph.  rx/rz are host arrays of nrec_total receivers. */
+int seisfat_batch_create(seisfat_eik_t h, int type, int schedule, int nshots,
+                         const seisfat_shot* shots, int nrec_total, const float* rx,
+                         const float* rz, seisfat_batch_t* out);
+/* Run the batch asynchronously on `stream` (graph replay for the tiled schedule). */
+int seisfat_batch_run(seisfat_batch_t b, void* stream);
+/* Wait for the batch and copy all nrec_total picks to host memory. */
+int seisfat_batch_picks(seisfat_batch_t b, float* picks);
+/* Device pointer of the picks (nrec_total floats). */
+const float* seisfat_batch_device_picks(seisfat_batch_t b);
+/* Device pointer of shot k's padded traveltimes; *nzz, *nxx receive its shape. */
+const float* seisfat_batch_device_T(seisfat_batch_t b, int shot, int* nzz, int* nxx);
+/* Copy shot k's padded traveltimes to host memory (nzz*nxx floats). */
+int seisfat_batch_copy_T(seisfat_batch_t b, int shot, float* T_padded);
+/* Number of HIP kernel launches of one run (diagnostics). */
+int seisfat_batch_num_launches(seisfat_batch_t b);
+int seisfat_batch_destroy(seisfat_batch_t b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEISFAT2D_EIK_H */

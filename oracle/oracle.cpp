@@ -1,0 +1,684 @@
+// ORACLE — test infrastructure only.
+//
+// A CPU restatement of SeisFAT2D's eikonal path, used as the checker for the
+// HIP implementation in seisfat2d_amd/.  Only tests/, __graft_entry__.smoke()
+// and bench.py's cpu_baseline leg may load this library; the product path never
+// calls it.
+//
+// Every routine is templated on the working real type R (float = the
+// reference's FP32 arithmetic, double = the FP64 oracle).  The FP32 build uses
+// -ffp-contract=off so every a*b+c is two correctly rounded operations, exactly
+// as the C++ expressions in the reference read; fminf semantics (NaN-ignoring)
+// stand in for CUDA's device min(float,float).
+//
+// Parity pins (what checks each function; see tests/test_oracle.py):
+//   expand/reduce boundary ...... hand-checked 4x5 fixture, round trip
+//   init ......................... closed form S*dist on the 3x3 stencil
+//   fsm_pass (raster) ............ literal diagonal schedule (fsm_pass_diag)
+//                                  bit-exact; plane-wave exactness; t=r/v for
+//                                  constant velocity; head-wave formula for a
+//                                  two-layer model; FP32 vs FP64; monotone
+//                                  under extra passes; x-mirror symmetry
+//   compress/decode .............. closed-form fixture, exact min/max
+//   quasi_slowness ............... isotropic stiffness -> S unchanged; VTI
+//                                  qP phase velocity (Thomsen exact formula)
+//   picks (Catmull-Rom) .......... on-node receivers exact; quadratic fields
+//                                  reproduced exactly
+//   ray_paths .................... constant-velocity straight rays: row sums
+//                                  equal the source-receiver distance
+//   adjoint ...................... FP64 central finite differences of the
+//                                  FP64 single-pass forward
+//
+// Reference files restated (paths relative to the SeisFAT2D repository):
+//   src/modeling/modeling.cu, src/modeling/eikonal_iso.cu,
+//   src/modeling/eikonal_ani.cu, src/admin/admin.cpp,
+//   src/inversion/inversion.cpp.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int NB = 3;          // modeling.cu:113
+constexpr int COMPRESS = 65535; // modeling.cuh:13
+
+template <class R> inline R rmin(R a, R b) { return std::fmin(a, b); }
+template <class R> inline R rsqrt_(R x) { return std::sqrt(x); }
+
+// ---------------------------------------------------------------------------
+// A1: Modeling::expand_boundary  (src/modeling/modeling.cu:204-232)
+// Interior copy, then edge rows (z) replicated for interior columns, then edge
+// columns (x) replicated for every row.  Storage is z-fastest: idx = i + j*n.
+template <class R>
+void expand_boundary(const R* in, R* out, int nz, int nx) {
+  const int nzz = nz + 2 * NB, nxx = nx + 2 * NB;
+  for (int j = 0; j < nx; ++j)
+    for (int i = 0; i < nz; ++i) out[(i + NB) + (j + NB) * nzz] = in[i + j * nz];
+  for (int i = 0; i < NB; ++i)
+    for (int j = NB; j < nxx - NB; ++j) {
+      out[i + j * nzz] = out[NB + j * nzz];
+      out[(nzz - i - 1) + j * nzz] = out[(nzz - NB - 1) + j * nzz];
+    }
+  for (int i = 0; i < nzz; ++i)
+    for (int j = 0; j < NB; ++j) {
+      out[i + j * nzz] = out[i + NB * nzz];
+      out[i + (nxx - j - 1) * nzz] = out[i + (nxx - NB - 1) * nzz];
+    }
+}
+
+// Modeling::reduce_boundary  (src/modeling/modeling.cu:234-244)
+template <class R>
+void reduce_boundary(const R* in, R* out, int nz, int nx) {
+  const int nzz = nz + 2 * NB;
+  for (int j = 0; j < nx; ++j)
+    for (int i = 0; i < nz; ++i) out[i + j * nz] = in[(i + NB) + (j + NB) * nzz];
+}
+
+// ---------------------------------------------------------------------------
+// A3: Modeling::initialization + kernels time_set / time_init
+// (src/modeling/modeling.cu:110-121, 286-306).  The source snaps to the nearest
+// node; the 3x3 nodes around it get S[node] * |node - (sx,sz)| using the node's
+// own slowness and the exact source position.  powf(d, 2.0f) is evaluated as
+// d*d (the correctly rounded square).
+template <class R>
+void init_time(R* T, const R* S, int nzz, int nxx, R dx, R dz, R sx, R sz,
+               int* out_sIdx, int* out_sIdz) {
+  const R half = R(0.5);
+  const int sIdx = (int)((sx + half * dx) / dx) + NB;
+  const int sIdz = (int)((sz + half * dz) / dz) + NB;
+  const size_t matsize = (size_t)nzz * nxx;
+  for (size_t k = 0; k < matsize; ++k) T[k] = R(1e6);
+  for (int jj = 0; jj < 3; ++jj)
+    for (int ii = 0; ii < 3; ++ii) {
+      const int zi = sIdz + (ii - 1);
+      const int xi = sIdx + (jj - 1);
+      const size_t index = (size_t)zi + (size_t)xi * nzz;
+      const R ddx = (R)(xi - NB) * dx - sx;
+      const R ddz = (R)(zi - NB) * dz - sz;
+      T[index] = S[index] * rsqrt_(ddx * ddx + ddz * ddz);
+    }
+  if (out_sIdx) *out_sIdx = sIdx;
+  if (out_sIdz) *out_sIdz = sIdz;
+}
+
+// ---------------------------------------------------------------------------
+// A5: the local solver of kernel inner_sweep (src/modeling/modeling.cu:308-357)
+// for node (i,j) with sweep signs (a,b) = sgnt; sgnv = (sgnt+1)/2
+// (src/modeling/modeling.cu:62-102).
+template <class R>
+struct Consts {
+  R dx, dz, dx2i, dz2i;
+};
+
+template <class R>
+inline R local_candidate(const R* T, const R* S, int i, int j, int nzz, int nxx,
+                         int a, int b, const Consts<R>& c) {
+  const int i1 = i - (a + 1) / 2;
+  const int j1 = j - (b + 1) / 2;
+  const R tv = T[(i - a) + (size_t)j * nzz];
+  const R te = T[i + (size_t)(j - b) * nzz];
+  const R tev = T[(i - a) + (size_t)(j - b) * nzz];
+  const R t1d1 = tv + c.dz * rmin(S[i1 + (size_t)std::max(j - 1, 1) * nzz],
+                                  S[i1 + (size_t)std::min(j, nxx - 1) * nzz]);
+  const R t1d2 = te + c.dx * rmin(S[std::max(i - 1, 1) + (size_t)j1 * nzz],
+                                  S[std::min(i, nzz - 1) + (size_t)j1 * nzz]);
+  const R t1D = rmin(t1d1, t1d2);
+  R t1 = R(1e6), t2 = R(1e6), t3 = R(1e6);
+  const R Sref = S[i1 + (size_t)j1 * nzz];
+  if ((tv <= te + c.dx * Sref) && (te <= tv + c.dz * Sref) && (te - tev >= R(0)) &&
+      (tv - tev >= R(0))) {
+    const R ta = tev + te - tv;
+    const R tb = tev - te + tv;
+    t1 = ((tb * c.dz2i + ta * c.dx2i) +
+          rsqrt_(R(4) * Sref * Sref * (c.dz2i + c.dx2i) -
+                 c.dz2i * c.dx2i * (ta - tb) * (ta - tb))) /
+         (c.dz2i + c.dx2i);
+  } else if ((te - tev <= Sref * c.dz * c.dz / rsqrt_(c.dx * c.dx + c.dz * c.dz)) &&
+             (te - tev > R(0))) {
+    t2 = te + c.dx * rsqrt_(Sref * Sref - ((te - tev) / c.dz) * ((te - tev) / c.dz));
+  } else if ((tv - tev <= Sref * c.dx * c.dx / rsqrt_(c.dx * c.dx + c.dz * c.dz)) &&
+             (tv - tev > R(0))) {
+    t3 = tv + c.dz * rsqrt_(Sref * Sref - ((tv - tev) / c.dx) * ((tv - tev) / c.dx));
+  }
+  const R t2D = rmin(t1, rmin(t2, t3));
+  return rmin(t1D, t2D);
+}
+
+inline void sweep_signs(int sweep, int* a, int* b) {
+  // sgnt = {{1,1},{-1,1},{1,-1},{-1,-1}}  (src/modeling/modeling.cu:70)
+  *a = (sweep == 0 || sweep == 2) ? 1 : -1;
+  *b = (sweep == 0 || sweep == 1) ? 1 : -1;
+}
+
+template <class R>
+Consts<R> make_consts(R dx, R dz) {
+  Consts<R> c;
+  c.dx = dx;
+  c.dz = dz;
+  c.dz2i = R(1) / (dz * dz);  // modeling.cu:64-65
+  c.dx2i = R(1) / (dx * dx);
+  return c;
+}
+
+// A4, raster form: one directional sweep visiting nodes in any order that
+// respects the sweep's dependencies (each node reads only its three upwind
+// neighbours, which lie on the two previous anti-diagonals).
+template <class R>
+void sweep_raster(R* T, const R* S, int nzz, int nxx, int sweep, const Consts<R>& c) {
+  int a, b;
+  sweep_signs(sweep, &a, &b);
+  for (int jj = 1; jj < nxx - 1; ++jj) {
+    const int j = (b > 0) ? jj : nxx - 1 - jj;
+    for (int ii = 1; ii < nzz - 1; ++ii) {
+      const int i = (a > 0) ? ii : nzz - 1 - ii;
+      const size_t k = i + (size_t)j * nzz;
+      T[k] = rmin(T[k], local_candidate(T, S, i, j, nzz, nxx, a, b, c));
+    }
+  }
+}
+
+// A4, literal form: the per-level launch schedule of Modeling::eikonal_solver
+// (src/modeling/modeling.cu:123-159), element by element.
+template <class R>
+void sweep_diag(R* T, const R* S, int nzz, int nxx, int sweep, const Consts<R>& c) {
+  int a, b;
+  sweep_signs(sweep, &a, &b);
+  const int total_levels = (nxx - 1) + (nzz - 1);
+  const int min_level = std::min(nxx, nzz), max_level = std::max(nxx, nzz);
+  const int zd = (sweep == 2 || sweep == 3) ? -1 : 1;
+  const int xd = (sweep == 0 || sweep == 2) ? -1 : 1;
+  for (int level = 0; level < total_levels; ++level) {
+    const int z_offset = (sweep == 0) ? ((level < nxx) ? 0 : level - nxx + 1)
+                         : (sweep == 1) ? ((level < nzz) ? nzz - level - 1 : 0)
+                         : (sweep == 2) ? ((level < nzz) ? level : nzz - 1)
+                                        : ((level < nxx) ? nzz - 1 : nzz - 1 - (level - nxx + 1));
+    const int x_offset = (sweep == 0) ? ((level < nxx) ? level : nxx - 1)
+                         : (sweep == 1) ? ((level < nzz) ? 0 : level - nzz + 1)
+                         : (sweep == 2) ? ((level < nzz) ? nxx - 1 : nxx - 1 - (level - nzz + 1))
+                                        : ((level < nxx) ? nxx - level - 1 : 0);
+    const int n_elements = (level < min_level)   ? level + 1
+                           : (level >= max_level) ? total_levels - level
+                                                  : total_levels - min_level - max_level + level;
+    const int nthreads = ((n_elements + 255) / 256) * 256;  // whole blocks launched
+    // Threads of one level are independent: compute every candidate from the
+    // state before the level, then write (what a launch guarantees).
+    std::vector<std::pair<size_t, R>> upd;
+    for (int e = 0; e < nthreads; ++e) {
+      const int i = z_offset + zd * e;
+      const int j = x_offset + xd * e;
+      if ((i > 0) && (i < nzz - 1) && (j > 0) && (j < nxx - 1)) {
+        const size_t k = i + (size_t)j * nzz;
+        upd.emplace_back(k, rmin(T[k], local_candidate(T, S, i, j, nzz, nxx, a, b, c)));
+      }
+    }
+    for (auto& u : upd) T[u.first] = u.second;
+  }
+}
+
+// A4/A6: one pass of the four sweeps (ref mode), or passes repeated until no
+// value changes (conv mode).  Returns the number of passes executed.
+template <class R>
+int fsm(R* T, const R* S, int nzz, int nxx, R dx, R dz, int mode, int max_passes,
+        int schedule) {
+  const Consts<R> c = make_consts(dx, dz);
+  const size_t matsize = (size_t)nzz * nxx;
+  std::vector<R> prev;
+  int passes = 0;
+  for (;;) {
+    if (mode == 1) prev.assign(T, T + matsize);
+    for (int s = 0; s < 4; ++s) {
+      if (schedule == 1)
+        sweep_diag(T, S, nzz, nxx, s, c);
+      else
+        sweep_raster(T, S, nzz, nxx, s, c);
+    }
+    ++passes;
+    if (mode == 0 || passes >= max_passes) break;
+    if (std::memcmp(prev.data(), T, matsize * sizeof(R)) == 0) break;
+  }
+  return passes;
+}
+
+// ---------------------------------------------------------------------------
+// A7: Modeling::compression (src/modeling/modeling.cu:264-279) with exact
+// (serial) min/max; the reference's OpenMP min/max race is not reproduced.
+void compress(const float* in, uint16_t* out, size_t n, float* maxv, float* minv) {
+  float mx = -1e20f, mn = 1e20f;
+  for (size_t k = 0; k < n; ++k) {
+    mn = std::min(in[k], mn);
+    mx = std::max(in[k], mx);
+  }
+  for (size_t k = 0; k < n; ++k)
+    out[k] = static_cast<uint16_t>(1.0f + (COMPRESS - 1) * (in[k] - mn) / (mx - mn));
+  *maxv = mx;
+  *minv = mn;
+}
+
+// Decode of one stiffness element (src/modeling/eikonal_ani.cu:76-83).
+inline float decode(uint16_t u, float mn, float mx) {
+  return (mn + (static_cast<float>(u) - 1.0f) * (mx - mn) / (COMPRESS - 1));
+}
+
+// A8: kernel get_quasi_slowness (src/modeling/eikonal_ani.cu:47-111), FP32 as
+// in the reference including the double-precision eigenvalue sum at :103-104.
+void quasi_slowness(const float* T, float* S, int nzz, int nxx, float dx, float dz, int sIdx,
+                    int sIdz, const uint16_t* const* C, const float* mn, const float* mx) {
+  for (int j = NB; j < nxx - NB; ++j)
+    for (int i = NB; i < nzz - NB; ++i) {
+      if ((i == sIdz) && (j == sIdx)) continue;
+      const size_t index = i + (size_t)j * nzz;
+      const float dTz = 0.5f * (T[(i + 1) + (size_t)j * nzz] - T[(i - 1) + (size_t)j * nzz]) / dz;
+      const float dTx = 0.5f * (T[i + (size_t)(j + 1) * nzz] - T[i + (size_t)(j - 1) * nzz]) / dx;
+      const float norm = std::sqrt(dTx * dTx + dTz * dTz);
+      const float p0 = dTx / norm, p1 = dTz / norm;
+      const float c11 = decode(C[0][index], mn[0], mx[0]);
+      const float c13 = decode(C[1][index], mn[1], mx[1]);
+      const float c15 = decode(C[2][index], mn[2], mx[2]);
+      const float c33 = decode(C[3][index], mn[3], mx[3]);
+      const float c35 = decode(C[4][index], mn[4], mx[4]);
+      const float c55 = decode(C[5][index], mn[5], mx[5]);
+      float Cm[9] = {c11, c13, c15, c13, c33, c35, c15, c35, c55};  // C[r + c*3]
+      const float Ro = c33 * S[index] * S[index];
+      for (int k = 0; k < 9; ++k) Cm[k] = Cm[k] / Ro / Ro;
+      const float Gxx = Cm[0] * p0 * p0 + Cm[8] * p1 * p1 + 2.0f * Cm[6] * p0 * p1;
+      const float Gzz = Cm[8] * p0 * p0 + Cm[4] * p1 * p1 + 2.0f * Cm[7] * p0 * p1;
+      const float Gxz = Cm[6] * p0 * p0 + Cm[7] * p1 * p1 + (Cm[3] + Cm[8]) * p0 * p1;
+      const float coeff1 = Gxx + Gzz;
+      const float coeff2 = Gxx - Gzz;
+      const float det = std::sqrt((coeff2 * coeff2) / 4.0f + Gxz * Gxz);
+      float g0 = (float)((double)coeff1 / 2.0 + (double)det);
+      float g1 = (float)((double)coeff1 / 2.0 - (double)det);
+      if (g0 < g1) std::swap(g0, g1);
+      S[index] = 1.0f / std::sqrt(g0 * Ro);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// A10: cubic1d / cubic2d (src/admin/admin.cpp:121-134) and the receiver stencil
+// of Modeling::compute_seismogram (src/modeling/modeling.cu:161-194).
+template <class R>
+inline R cubic1d(const R P[4], R d) {
+  return P[1] + R(0.5) * d *
+                    (P[2] - P[0] +
+                     d * (R(2) * P[0] - R(5) * P[1] + R(4) * P[2] - P[3] +
+                          d * (R(3) * (P[1] - P[2]) + P[3] - P[0])));
+}
+
+template <class R>
+R pick_one(const R* T, int nzz, R dx, R dz, R x, R z) {
+  const R half = R(0.5);
+  const R x0 = std::floor((x + half * dx) / dx) * dx;
+  const R z0 = std::floor((z + half * dz) / dz) * dz;
+  const R x1 = std::floor((x + half * dx) / dx) * dx + dx;
+  const R z1 = std::floor((z + half * dz) / dz) * dz + dz;
+  const R xd = (x - x0) / (x1 - x0);
+  const R zd = (z - z0) / (z1 - z0);
+  const int i = (int)((z + half * dz) / dz) + NB;
+  const int j = (int)((x + half * dx) / dx) + NB;
+  R P[4][4];
+  for (int pIdx = 0; pIdx < 4; ++pIdx)
+    for (int pIdz = 0; pIdz < 4; ++pIdz)
+      P[pIdx][pIdz] = T[(i + pIdz - 1) + (size_t)(j + pIdx - 1) * nzz];
+  R p[4];
+  for (int k = 0; k < 4; ++k) p[k] = cubic1d(P[k], zd);
+  return cubic1d(p, xd);
+}
+
+// Catmull-Rom weights of cubic1d (the pick is linear in the 16 stencil values).
+inline void cubic_weights(double d, double w[4]) {
+  w[0] = 0.5 * d * (-1.0 + d * (2.0 - d));
+  w[1] = 1.0 + 0.5 * d * d * (-5.0 + 3.0 * d);
+  w[2] = 0.5 * d * (1.0 + d * (4.0 - 3.0 * d));
+  w[3] = 0.5 * d * d * (-1.0 + d);
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of one single-pass solve (init + 4 sweeps in raster order).
+// J = 1/2 sum_r (pick_r - obs_r)^2; returns J and accumulates dJ/dS (padded
+// grid, double) into g.  Tie rule: T_old wins when T_old <= candidate; within a
+// min of two slownesses the first argument wins on ties.
+template <class R>
+struct NodeDeriv {
+  int kind;  // 0 old, 1 d1z, 2 d1x, 3 branch A, 4 branch B, 5 branch C, 6 constant
+  double dtv, dte, dtev;  // d(new)/d(neighbour)
+  int scell;              // slowness cell index receiving dS
+  double ds;              // d(new)/dS[scell]
+};
+
+template <class R>
+NodeDeriv<R> node_deriv(const R* T, const R* S, int i, int j, int nzz, int nxx, int a, int b,
+                        const Consts<R>& c, R told) {
+  NodeDeriv<R> d{};
+  const int i1 = i - (a + 1) / 2;
+  const int j1 = j - (b + 1) / 2;
+  const R tv = T[(i - a) + (size_t)j * nzz];
+  const R te = T[i + (size_t)(j - b) * nzz];
+  const R tev = T[(i - a) + (size_t)(j - b) * nzz];
+  const size_t cz_a = i1 + (size_t)std::max(j - 1, 1) * nzz, cz_b = i1 + (size_t)std::min(j, nxx - 1) * nzz;
+  const size_t cx_a = std::max(i - 1, 1) + (size_t)j1 * nzz, cx_b = std::min(i, nzz - 1) + (size_t)j1 * nzz;
+  const R t1d1 = tv + c.dz * rmin(S[cz_a], S[cz_b]);
+  const R t1d2 = te + c.dx * rmin(S[cx_a], S[cx_b]);
+  const R t1D = rmin(t1d1, t1d2);
+  R t1 = R(1e6), t2 = R(1e6), t3 = R(1e6);
+  int br = 0;
+  const size_t cref = i1 + (size_t)j1 * nzz;
+  const R Sref = S[cref];
+  double dd_tv = 0, dd_te = 0, dd_tev = 0, dd_s = 0;
+  if ((tv <= te + c.dx * Sref) && (te <= tv + c.dz * Sref) && (te - tev >= R(0)) &&
+      (tv - tev >= R(0))) {
+    const R ta = tev + te - tv;
+    const R tb = tev - te + tv;
+    const R rad = R(4) * Sref * Sref * (c.dz2i + c.dx2i) - c.dz2i * c.dx2i * (ta - tb) * (ta - tb);
+    t1 = ((tb * c.dz2i + ta * c.dx2i) + rsqrt_(rad)) / (c.dz2i + c.dx2i);
+    br = 3;
+    const double D = (double)c.dz2i + (double)c.dx2i;
+    const double sq = std::sqrt((double)rad);
+    const double k = 2.0 * (double)c.dz2i * (double)c.dx2i * ((double)ta - (double)tb) / sq;
+    dd_te = ((double)c.dx2i - (double)c.dz2i - k) / D;
+    dd_tv = ((double)c.dz2i - (double)c.dx2i + k) / D;
+    dd_tev = 1.0;
+    dd_s = 4.0 * (double)Sref / sq;
+  } else if ((te - tev <= Sref * c.dz * c.dz / rsqrt_(c.dx * c.dx + c.dz * c.dz)) && (te - tev > R(0))) {
+    const R q = (te - tev) / c.dz;
+    const R Q = rsqrt_(Sref * Sref - q * q);
+    t2 = te + c.dx * Q;
+    br = 4;
+    const double r = (double)c.dx * (double)q / ((double)c.dz * (double)Q);
+    dd_te = 1.0 - r;
+    dd_tev = r;
+    dd_s = (double)c.dx * (double)Sref / (double)Q;
+  } else if ((tv - tev <= Sref * c.dx * c.dx / rsqrt_(c.dx * c.dx + c.dz * c.dz)) && (tv - tev > R(0))) {
+    const R p = (tv - tev) / c.dx;
+    const R P = rsqrt_(Sref * Sref - p * p);
+    t3 = tv + c.dz * P;
+    br = 5;
+    const double r = (double)c.dz * (double)p / ((double)c.dx * (double)P);
+    dd_tv = 1.0 - r;
+    dd_tev = r;
+    dd_s = (double)c.dz * (double)Sref / (double)P;
+  }
+  const R t2D = rmin(t1, rmin(t2, t3));
+  const R cand = rmin(t1D, t2D);
+  if (!(cand < told)) {
+    d.kind = 0;
+    return d;
+  }
+  if (t1D <= t2D) {
+    if (t1d1 <= t1d2) {
+      d.kind = 1;
+      d.dtv = 1.0;
+      d.scell = (int)((S[cz_a] <= S[cz_b]) ? cz_a : cz_b);
+      d.ds = (double)c.dz;
+    } else {
+      d.kind = 2;
+      d.dte = 1.0;
+      d.scell = (int)((S[cx_a] <= S[cx_b]) ? cx_a : cx_b);
+      d.ds = (double)c.dx;
+    }
+    return d;
+  }
+  // t2D = min(t1, t2, t3) with the untaken ones at 1e6: a taken branch whose
+  // value is not below 1e6 loses to the constant.
+  const R tbr = (br == 3) ? t1 : (br == 4) ? t2 : (br == 5) ? t3 : R(1e6);
+  if (br == 0 || !(tbr < R(1e6))) {
+    d.kind = 6;
+    return d;
+  }
+  d.kind = br;
+  d.dtv = dd_tv;
+  d.dte = dd_te;
+  d.dtev = dd_tev;
+  d.scell = (int)cref;
+  d.ds = dd_s;
+  return d;
+}
+
+template <class R>
+double adjoint(const R* S, int nzz, int nxx, R dx, R dz, R sx, R sz, const R* rx, const R* rz,
+               int nrec, const R* obs, double* g, R* picks_out) {
+  const Consts<R> c = make_consts(dx, dz);
+  const size_t matsize = (size_t)nzz * nxx;
+  std::vector<R> snap(5 * matsize);  // T after init, after sweep 0..3
+  int sIdx, sIdz;
+  init_time(snap.data(), S, nzz, nxx, dx, dz, sx, sz, &sIdx, &sIdz);
+  for (int s = 0; s < 4; ++s) {
+    std::copy(snap.begin() + s * matsize, snap.begin() + (s + 1) * matsize,
+              snap.begin() + (s + 1) * matsize);
+    sweep_raster(snap.data() + (s + 1) * matsize, S, nzz, nxx, s, c);
+  }
+  const R* Tf = snap.data() + 4 * matsize;
+  std::vector<double> L(matsize, 0.0);
+  double J = 0.0;
+  for (int r = 0; r < nrec; ++r) {
+    const R pk = pick_one(Tf, nzz, dx, dz, rx[r], rz[r]);
+    if (picks_out) picks_out[r] = pk;
+    const double res = (double)pk - (double)obs[r];
+    J += 0.5 * res * res;
+    // Same stencil position and fractional offsets as pick_one.
+    const R half = R(0.5);
+    const R x0 = std::floor((rx[r] + half * dx) / dx) * dx;
+    const R z0 = std::floor((rz[r] + half * dz) / dz) * dz;
+    const R x1 = std::floor((rx[r] + half * dx) / dx) * dx + dx;
+    const R z1 = std::floor((rz[r] + half * dz) / dz) * dz + dz;
+    const double xd = (double)((rx[r] - x0) / (x1 - x0));
+    const double zd = (double)((rz[r] - z0) / (z1 - z0));
+    const int i = (int)((rz[r] + half * dz) / dz) + NB;
+    const int j = (int)((rx[r] + half * dx) / dx) + NB;
+    double wx[4], wz[4];
+    cubic_weights(xd, wx);
+    cubic_weights(zd, wz);
+    for (int pIdx = 0; pIdx < 4; ++pIdx)
+      for (int pIdz = 0; pIdz < 4; ++pIdz)
+        L[(i + pIdz - 1) + (size_t)(j + pIdx - 1) * nzz] += res * wx[pIdx] * wz[pIdz];
+  }
+  for (int s = 3; s >= 0; --s) {
+    int a, b;
+    sweep_signs(s, &a, &b);
+    const R* Tnew = snap.data() + (s + 1) * matsize;  // state after sweep s
+    const R* Told = snap.data() + s * matsize;        // state before sweep s
+    for (int jj = nxx - 2; jj >= 1; --jj) {
+      const int j = (b > 0) ? jj : nxx - 1 - jj;
+      for (int ii = nzz - 2; ii >= 1; --ii) {
+        const int i = (a > 0) ? ii : nzz - 1 - ii;
+        const size_t k = i + (size_t)j * nzz;
+        const double lam = L[k];
+        if (lam == 0.0) continue;
+        // Neighbour values seen by the forward update are the sweep-s values.
+        const NodeDeriv<R> d = node_deriv(Tnew, S, i, j, nzz, nxx, a, b, c, Told[k]);
+        if (d.kind == 0) continue;  // adjoint passes to T_old unchanged
+        L[k] = 0.0;
+        if (d.kind == 6) continue;
+        if (d.dtv != 0.0) L[(i - a) + (size_t)j * nzz] += lam * d.dtv;
+        if (d.dte != 0.0) L[i + (size_t)(j - b) * nzz] += lam * d.dte;
+        if (d.dtev != 0.0) L[(i - a) + (size_t)(j - b) * nzz] += lam * d.dtev;
+        g[d.scell] += lam * d.ds;
+      }
+    }
+  }
+  // Source initialisation: T = S[node] * dist on the 3x3 stencil.
+  for (int jj = 0; jj < 3; ++jj)
+    for (int ii = 0; ii < 3; ++ii) {
+      const int zi = sIdz + (ii - 1), xi = sIdx + (jj - 1);
+      const size_t index = (size_t)zi + (size_t)xi * nzz;
+      const R ddx = (R)(xi - NB) * dx - sx;
+      const R ddz = (R)(zi - NB) * dz - sz;
+      g[index] += L[index] * (double)rsqrt_(ddx * ddx + ddz * ddz);
+    }
+  return J;
+}
+
+// ---------------------------------------------------------------------------
+// A11: Inversion::gradient_ray_tracing (src/inversion/inversion.cpp:95-177) for
+// one shot: steepest descent on T from each receiver, visited cells sorted and
+// run-length counted; the first run carries one extra step and the source cell
+// gets the final residual distance.  Emits COO triplets (receiver, cell, len).
+// max_steps guards against a ray that never reaches the source cell.
+int ray_paths(const float* T, int nz, int nx, float dx, float dz, float sx, float sz,
+              const float* rx, const float* rz, int nrec, int max_steps, int* out_rec,
+              int* out_cell, float* out_len, int cap) {
+  const int nzz = nz + 2 * NB;
+  const int sIdx = (int)((sx + 0.5f * dx) / dx);
+  const int sIdz = (int)((sz + 0.5f * dz) / dz);
+  const int sId = sIdz + sIdx * nz;
+  const float rayStep = 0.2f * dz;
+  int n = 0;
+  std::vector<int> ray_index;
+  for (int r = 0; r < nrec; ++r) {
+    float xi = rx[r], zi = rz[r];
+    if ((sz == zi) && (sx == xi)) continue;
+    ray_index.clear();
+    int steps = 0;
+    for (;;) {
+      const int j = (int)((xi + 0.5f * dx) / dx) + NB;
+      const int i = (int)((zi + 0.5f * dz) / dz) + NB;
+      const float dTx = (T[i + (size_t)(j + 1) * nzz] - T[i + (size_t)(j - 1) * nzz]) / (2.0f * dx);
+      const float dTz = (T[(i + 1) + (size_t)j * nzz] - T[(i - 1) + (size_t)j * nzz]) / (2.0f * dz);
+      const float norm = std::sqrt(dTx * dTx + dTz * dTz);
+      xi -= rayStep * dTx / norm;
+      zi -= rayStep * dTz / norm;
+      const int jm = (int)((xi + 0.5f * dx) / dx);
+      const int im = (int)((zi + 0.5f * dz) / dz);
+      ray_index.push_back(im + jm * nz);
+      if (ray_index.back() == sId) break;
+      if (++steps >= max_steps || im < 0 || im >= nz || jm < 0 || jm >= nx) return -1 - r;
+    }
+    const float dzs = zi - sz, dxs = xi - sx;
+    const float final_distance = std::sqrt(dzs * dzs + dxs * dxs);
+    std::sort(ray_index.begin(), ray_index.end());
+    int current = ray_index[0];
+    float dist = rayStep;
+    auto emit = [&](int cell, float len) {
+      if (n < cap) {
+        out_rec[n] = r;
+        out_cell[n] = cell;
+        out_len[n] = len;
+      }
+      ++n;
+    };
+    for (size_t k = 0; k < ray_index.size(); ++k) {
+      if (ray_index[k] == current) {
+        dist += rayStep;
+      } else {
+        emit(current, (current == sId) ? final_distance : dist);
+        dist = rayStep;
+        current = ray_index[k];
+      }
+    }
+    emit(current, (current == sId) ? final_distance : dist);
+  }
+  return n;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C entry points (ctypes).  All grids are z-fastest; "padded" grids are
+// (nz+6) x (nx+6).
+extern "C" {
+
+void orc_expand_boundary_f32(const float* in, float* out, int nz, int nx) { expand_boundary(in, out, nz, nx); }
+void orc_expand_boundary_f64(const double* in, double* out, int nz, int nx) { expand_boundary(in, out, nz, nx); }
+void orc_reduce_boundary_f32(const float* in, float* out, int nz, int nx) { reduce_boundary(in, out, nz, nx); }
+
+void orc_init_f32(float* T, const float* S, int nzz, int nxx, float dx, float dz, float sx, float sz, int* sIdx, int* sIdz) {
+  init_time(T, S, nzz, nxx, dx, dz, sx, sz, sIdx, sIdz);
+}
+void orc_init_f64(double* T, const double* S, int nzz, int nxx, double dx, double dz, double sx, double sz, int* sIdx, int* sIdz) {
+  init_time(T, S, nzz, nxx, dx, dz, sx, sz, sIdx, sIdz);
+}
+
+// mode 0 = ref (one pass), 1 = conv; schedule 0 = raster, 1 = literal diagonal
+int orc_fsm_f32(float* T, const float* S, int nzz, int nxx, float dx, float dz, int mode, int max_passes, int schedule) {
+  return fsm(T, S, nzz, nxx, dx, dz, mode, max_passes, schedule);
+}
+int orc_fsm_f64(double* T, const double* S, int nzz, int nxx, double dx, double dz, int mode, int max_passes, int schedule) {
+  return fsm(T, S, nzz, nxx, dx, dz, mode, max_passes, schedule);
+}
+void orc_sweep_f32(float* T, const float* S, int nzz, int nxx, float dx, float dz, int sweep) {
+  sweep_raster(T, S, nzz, nxx, sweep, make_consts(dx, dz));
+}
+
+// Full iso solve on a padded slowness grid: init + fsm (Eikonal_ISO::time_propagation,
+// src/modeling/eikonal_iso.cu:9-13).
+int orc_solve_iso_f32(float* T, const float* S, int nzz, int nxx, float dx, float dz, float sx, float sz, int mode, int max_passes) {
+  init_time(T, S, nzz, nxx, dx, dz, sx, sz, nullptr, nullptr);
+  return fsm(T, S, nzz, nxx, dx, dz, mode, max_passes, 0);
+}
+int orc_solve_iso_f64(double* T, const double* S, int nzz, int nxx, double dx, double dz, double sx, double sz, int mode, int max_passes) {
+  init_time(T, S, nzz, nxx, dx, dz, sx, sz, nullptr, nullptr);
+  return fsm(T, S, nzz, nxx, dx, dz, mode, max_passes, 0);
+}
+
+// Shot-parallel batch of iso solves on windows of one model (the CPU timing
+// path).  Shot k uses columns [x0[k], x0[k]+w[k]) of the unpadded model s
+// (nz x nx), padded by replication like a standalone model; T for shot k is
+// written at T + toff[k] with shape (nz+6) x (w[k]+6).  nthreads<=0 uses the
+// OpenMP default.
+void orc_solve_windows_f32(const float* s, int nz, int nx, float dx, float dz, int nshots,
+                           const float* sx, const float* sz, const int* x0, const int* w,
+                           const long long* toff, float* T, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads != 1)
+  for (int k = 0; k < nshots; ++k) {
+    const int nzz = nz + 2 * NB, nxx = w[k] + 2 * NB;
+    std::vector<float> crop((size_t)nz * w[k]), S((size_t)nzz * nxx);
+    std::memcpy(crop.data(), s + (size_t)x0[k] * nz, crop.size() * sizeof(float));
+    expand_boundary(crop.data(), S.data(), nz, w[k]);
+    float* Tk = T + toff[k];
+    init_time(Tk, S.data(), nzz, nxx, dx, dz, sx[k] - x0[k] * dx, sz[k], nullptr, nullptr);
+    fsm(Tk, S.data(), nzz, nxx, dx, dz, 0, 1, 0);
+  }
+}
+
+void orc_compress(const float* in, uint16_t* out, long long n, float* maxv, float* minv) {
+  compress(in, out, (size_t)n, maxv, minv);
+}
+
+void orc_quasi_slowness(const float* T, float* S, int nzz, int nxx, float dx, float dz, int sIdx, int sIdz,
+                        const uint16_t* c11, const uint16_t* c13, const uint16_t* c15, const uint16_t* c33,
+                        const uint16_t* c35, const uint16_t* c55, const float* mn, const float* mx) {
+  const uint16_t* C[6] = {c11, c13, c15, c33, c35, c55};
+  quasi_slowness(T, S, nzz, nxx, dx, dz, sIdx, sIdz, C, mn, mx);
+}
+
+// Eikonal_ANI::time_propagation (src/modeling/eikonal_ani.cu:33-45) on padded
+// grids; S_work receives the qP slowness used by the second solve.
+void orc_solve_ani_f32(float* T, const float* S_iso, float* S_work, int nzz, int nxx, float dx, float dz,
+                       float sx, float sz, const uint16_t* c11, const uint16_t* c13, const uint16_t* c15,
+                       const uint16_t* c33, const uint16_t* c35, const uint16_t* c55, const float* mn,
+                       const float* mx) {
+  const size_t matsize = (size_t)nzz * nxx;
+  std::memcpy(S_work, S_iso, matsize * sizeof(float));
+  int sIdx, sIdz;
+  init_time(T, S_work, nzz, nxx, dx, dz, sx, sz, &sIdx, &sIdz);
+  fsm(T, S_work, nzz, nxx, dx, dz, 0, 1, 0);
+  const uint16_t* C[6] = {c11, c13, c15, c33, c35, c55};
+  quasi_slowness(T, S_work, nzz, nxx, dx, dz, sIdx, sIdz, C, mn, mx);
+  init_time(T, S_work, nzz, nxx, dx, dz, sx, sz, nullptr, nullptr);
+  fsm(T, S_work, nzz, nxx, dx, dz, 0, 1, 0);
+}
+
+void orc_picks_f32(const float* T, int nzz, float dx, float dz, const float* rx, const float* rz, int nrec, float* out) {
+  for (int r = 0; r < nrec; ++r) out[r] = pick_one(T, nzz, dx, dz, rx[r], rz[r]);
+}
+void orc_picks_f64(const double* T, int nzz, double dx, double dz, const double* rx, const double* rz, int nrec, double* out) {
+  for (int r = 0; r < nrec; ++r) out[r] = pick_one(T, nzz, dx, dz, rx[r], rz[r]);
+}
+
+double orc_adjoint_f32(const float* S, int nzz, int nxx, float dx, float dz, float sx, float sz, const float* rx,
+                       const float* rz, int nrec, const float* obs, double* g, float* picks) {
+  return adjoint(S, nzz, nxx, dx, dz, sx, sz, rx, rz, nrec, obs, g, picks);
+}
+double orc_adjoint_f64(const double* S, int nzz, int nxx, double dx, double dz, double sx, double sz, const double* rx,
+                       const double* rz, int nrec, const double* obs, double* g, double* picks) {
+  return adjoint(S, nzz, nxx, dx, dz, sx, sz, rx, rz, nrec, obs, g, picks);
+}
+
+int orc_ray_paths(const float* T, int nz, int nx, float dx, float dz, float sx, float sz, const float* rx,
+                  const float* rz, int nrec, int max_steps, int* out_rec, int* out_cell, float* out_len, int cap) {
+  return ray_paths(T, nz, nx, dx, dz, sx, sz, rx, rz, nrec, max_steps, out_rec, out_cell, out_len, cap);
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker for the HIP path; the product package never imports it.  The
+library is built by ``make -C oracle`` (or ``__graft_entry__.build()``).
+
+All grids are numpy arrays in the reference's z-fastest layout: a padded grid
+of nzz x nxx nodes is held as an array of shape (nxx, nzz) in C order, so that
+``A[j, i]`` is node (z=i, x=j) and ``A.ravel()[i + j*nzz]`` is the reference's
+``idx = i + j*nzz`` (src/modeling/modeling.cu:209-212,324).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+NB = 3
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+ip = C.POINTER(C.c_int)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(path)
+        i, f, d = C.c_int, C.c_float, C.c_double
+        sig = {
+            "orc_expand_boundary_f32": (None, [f32p, f32p, i, i]),
+            "orc_expand_boundary_f64": (None, [f64p, f64p, i, i]),
+            "orc_reduce_boundary_f32": (None, [f32p, f32p, i, i]),
+            "orc_init_f32": (None, [f32p, f32p, i, i, f, f, f, f, ip, ip]),
+            "orc_init_f64": (None, [f64p, f64p, i, i, d, d, d, d, ip, ip]),
+            "orc_fsm_f32": (i, [f32p, f32p, i, i, f, f, i, i, i]),
+            "orc_fsm_f64": (i, [f64p, f64p, i, i, d, d, i, i, i]),
+            "orc_sweep_f32": (None, [f32p, f32p, i, i, f, f, i]),
+            "orc_solve_iso_f32": (i, [f32p, f32p, i, i, f, f, f, f, i, i]),
+            "orc_solve_iso_f64": (i, [f64p, f64p, i, i, d, d, d, d, i, i]),
+            "orc_solve_windows_f32": (None, [f32p, i, i, f, f, i, f32p, f32p, i32p, i32p, i64p, f32p, i]),
+            "orc_compress": (None, [f32p, u16p, C.c_longlong, C.POINTER(f), C.POINTER(f)]),
+            "orc_quasi_slowness": (None, [f32p, f32p, i, i, f, f, i, i, u16p, u16p, u16p, u16p, u16p, u16p, f32p, f32p]),
+            "orc_solve_ani_f32": (None, [f32p, f32p, f32p, i, i, f, f, f, f, u16p, u16p, u16p, u16p, u16p, u16p, f32p, f32p]),
+            "orc_picks_f32": (None, [f32p, i, f, f, f32p, f32p, i, f32p]),
+            "orc_picks_f64": (None, [f64p, i, d, d, f64p, f64p, i, f64p]),
+            "orc_adjoint_f32": (d, [f32p, i, i, f, f, f, f, f32p, f32p, i, f32p, f64p, f32p]),
+            "orc_adjoint_f64": (d, [f64p, i, i, d, d, d, d, f64p, f64p, i, f64p, f64p, f64p]),
+            "orc_ray_paths": (i, [f32p, i, i, f, f, f, f, f32p, f32p, i, i, i32p, i32p, f32p, i]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _suffix(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+def expand_boundary(s):
+    """Pad an (nx, nz) model by nb=3 with edge replication (modeling.cu:204-232)."""
+    s = np.ascontiguousarray(s)
+    nx, nz = s.shape
+    out = np.zeros((nx + 2 * NB, nz + 2 * NB), s.dtype)
+    getattr(lib(), "orc_expand_boundary_" + _suffix(s.dtype))(s, out, nz, nx)
+    return out
+
+
+def reduce_boundary(S):
+    S = np.ascontiguousarray(S, np.float32)
+    nxx, nzz = S.shape
+    out = np.zeros((nxx - 2 * NB, nzz - 2 * NB), np.float32)
+    lib().orc_reduce_boundary_f32(S, out, nzz - 2 * NB, nxx - 2 * NB)
+    return out
+
+
+def reduce_boundary_adjoint(G):
+    """Adjoint of expand_boundary: sum each padded cell into the model cell it copies."""
+    nxx, nzz = G.shape
+    nx, nz = nxx - 2 * NB, nzz - 2 * NB
+    jj = np.clip(np.arange(nxx) - NB, 0, nx - 1)
+    ii = np.clip(np.arange(nzz) - NB, 0, nz - 1)
+    out = np.zeros((nx, nz), np.float64)
+    np.add.at(out, (jj[:, None], ii[None, :]), G)
+    return out
+
+
+def init_time(S, dx, dz, sx, sz):
+    S = np.ascontiguousarray(S)
+    nxx, nzz = S.shape
+    T = np.empty_like(S)
+    a, b = C.c_int(), C.c_int()
+    getattr(lib(), "orc_init_" + _suffix(S.dtype))(T, S, nzz, nxx, dx, dz, sx, sz, C.byref(a), C.byref(b))
+    return T, a.value, b.value
+
+
+def fsm(T, S, dx, dz, mode=0, max_passes=1, schedule=0):
+    """In-place sweeps on T.  mode 0 = one pass (reference), 1 = until unchanged."""
+    assert T.dtype == S.dtype and T.flags.c_contiguous and S.flags.c_contiguous
+    nxx, nzz = S.shape
+    return getattr(lib(), "orc_fsm_" + _suffix(S.dtype))(T, S, nzz, nxx, dx, dz, mode, max_passes, schedule)
+
+
+def sweep(T, S, dx, dz, s):
+    nxx, nzz = S.shape
+    lib().orc_sweep_f32(T, S, nzz, nxx, dx, dz, s)
+
+
+def solve_iso(S, dx, dz, sx, sz, mode=0, max_passes=1):
+    """Eikonal_ISO::time_propagation on a padded slowness grid; returns padded T."""
+    S = np.ascontiguousarray(S)
+    nxx, nzz = S.shape
+    T = np.empty_like(S)
+    passes = getattr(lib(), "orc_solve_iso_" + _suffix(S.dtype))(T, S, nzz, nxx, dx, dz, sx, sz, mode, max_passes)
+    return T if mode == 0 else (T, passes)
+
+
+def solve_windows(s, dx, dz, sx, sz, x0, w, nthreads=1):
+    """Shot-parallel iso solves on column windows of an unpadded (nx, nz) model."""
+    s = np.ascontiguousarray(s, np.float32)
+    nx, nz = s.shape
+    sx = np.ascontiguousarray(sx, np.float32)
+    sz = np.ascontiguousarray(sz, np.float32)
+    x0 = np.ascontiguousarray(x0, np.int32)
+    w = np.ascontiguousarray(w, np.int32)
+    sizes = (nz + 2 * NB) * (w.astype(np.int64) + 2 * NB)
+    toff = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64))
+    T = np.empty(int(sizes.sum()), np.float32)
+    lib().orc_solve_windows_f32(s, nz, nx, dx, dz, len(w), sx, sz, x0, w, toff, T, nthreads)
+    return [T[o:o + n].reshape(int(wk) + 2 * NB, nz + 2 * NB) for o, n, wk in zip(toff, sizes, w)]
+
+
+def compress(c):
+    c = np.ascontiguousarray(c, np.float32)
+    out = np.empty(c.shape, np.uint16)
+    mx, mn = C.c_float(), C.c_float()
+    lib().orc_compress(c.ravel(), out.ravel(), c.size, C.byref(mx), C.byref(mn))
+    return out, mn.value, mx.value
+
+
+def quasi_slowness(T, S, dx, dz, sIdx, sIdz, Cq, mn, mx):
+    """get_quasi_slowness on padded T/S (S updated in place); Cq = 6 uint16 grids."""
+    nxx, nzz = S.shape
+    mn = np.ascontiguousarray(mn, np.float32)
+    mx = np.ascontiguousarray(mx, np.float32)
+    lib().orc_quasi_slowness(T, S, nzz, nxx, dx, dz, sIdx, sIdz, *[np.ascontiguousarray(c) for c in Cq], mn, mx)
+
+
+def solve_ani(S_iso, dx, dz, sx, sz, Cq, mn, mx):
+    S_iso = np.ascontiguousarray(S_iso, np.float32)
+    nxx, nzz = S_iso.shape
+    T = np.empty_like(S_iso)
+    S_work = np.empty_like(S_iso)
+    lib().orc_solve_ani_f32(T, S_iso, S_work, nzz, nxx, dx, dz, sx, sz,
+                            *[np.ascontiguousarray(c) for c in Cq],
+                            np.ascontiguousarray(mn, np.float32), np.ascontiguousarray(mx, np.float32))
+    return T, S_work
+
+
+def picks(T, dx, dz, rx, rz):
+    T = np.ascontiguousarray(T)
+    nxx, nzz = T.shape
+    dt = T.dtype
+    rx = np.ascontiguousarray(rx, dt)
+    rz = np.ascontiguousarray(rz, dt)
+    out = np.empty(len(rx), dt)
+    getattr(lib(), "orc_picks_" + _suffix(dt))(T, nzz, dx, dz, rx, rz, len(rx), out)
+    return out
+
+
+def adjoint(S, dx, dz, sx, sz, rx, rz, obs):
+    """Discrete adjoint of one single-pass solve.  Returns (J, dJ/dS padded, picks)."""
+    S = np.ascontiguousarray(S)
+    dt = S.dtype
+    nxx, nzz = S.shape
+    g = np.zeros(S.shape, np.float64)
+    pk = np.empty(len(rx), dt)
+    J = getattr(lib(), "orc_adjoint_" + _suffix(dt))(
+        S, nzz, nxx, dx, dz, sx, sz, np.ascontiguousarray(rx, dt), np.ascontiguousarray(rz, dt),
+        len(rx), np.ascontiguousarray(obs, dt), g, pk)
+    return J, g, pk
+
+
+def ray_paths(T, nz, nx, dx, dz, sx, sz, rx, rz, max_steps=200000, cap=None):
+    """Ray-traced sensitivity rows for one shot (inversion.cpp:95-177) as COO."""
+    T = np.ascontiguousarray(T, np.float32)
+    rx = np.ascontiguousarray(rx, np.float32)
+    rz = np.ascontiguousarray(rz, np.float32)
+    cap = cap or 64 * (nx + nz) * max(1, len(rx))
+    rec = np.empty(cap, np.int32)
+    cell = np.empty(cap, np.int32)
+    ln = np.empty(cap, np.float32)
+    n = lib().orc_ray_paths(T, nz, nx, dx, dz, sx, sz, rx, rz, len(rx), max_steps, rec, cell, ln, cap)
+    if n < 0:
+        raise RuntimeError(f"ray from receiver {-1 - n} did not reach the source cell")
+    if n > cap:
+        raise RuntimeError("COO capacity exceeded")
+    return rec[:n], cell[:n], ln[:n]

@@ -1,0 +1,90 @@
+"""Native build of seisfat2d_amd: hipcc for gfx950, g++ for the host library.
+
+Everything is built in-tree so the shared objects travel with the source tree:
+  seisfat2d_amd/libseisfat2d_eik.so    HIP kernels + C ABI (include/seisfat2d_eik.h)
+  seisfat2d_amd/libseisfat2d_host.so   C++ Modeling / Eikonal_ISO / Eikonal_ANI classes
+  seisfat2d_amd/seisfat2d_modeling     the per-shot modeling executable
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+INCLUDE = os.path.join(ROOT, "include")
+
+EIK_LIB = os.path.join(PKG, "libseisfat2d_eik.so")
+HOST_LIB = os.path.join(PKG, "libseisfat2d_host.so")
+MODELING_EXE = os.path.join(PKG, "seisfat2d_modeling")
+
+# Device arithmetic contract: no FMA contraction, correctly rounded fp32
+# division and square root (bit parity with the oracle); no packed fp32 math.
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
+]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def _hipcc():
+    for p in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if p and os.path.exists(p):
+            return p
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + out)
+    return out
+
+
+def build_eik(verbose=False):
+    srcs = [os.path.join(CSRC, f) for f in ("eik_kernels.hip", "eik_capi.hip", "eik_adjoint.hip")]
+    srcs = [s for s in srcs if os.path.exists(s)]
+    tmp = EIK_LIB + ".tmp"
+    out = _run([_hipcc(), *HIP_FLAGS, "-shared", "-I", INCLUDE, "-o", tmp, *srcs])
+    os.replace(tmp, EIK_LIB)
+    if verbose and out.strip():
+        print(out)
+    return EIK_LIB
+
+
+def build_host(verbose=False):
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp") and f != "modeling_main.cpp")
+    if not srcs:
+        return None
+    tmp = HOST_LIB + ".tmp"
+    _run(["g++", *HOST_FLAGS, "-shared", "-I", INCLUDE, "-o", tmp, *srcs,
+          "-L", PKG, "-lseisfat2d_eik", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, HOST_LIB)
+    main = os.path.join(HOST, "modeling_main.cpp")
+    if os.path.exists(main):
+        tmp = MODELING_EXE + ".tmp"
+        _run(["g++", *HOST_FLAGS, "-I", INCLUDE, "-o", tmp, main, "-L", PKG,
+              "-lseisfat2d_host", "-lseisfat2d_eik", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, MODELING_EXE)
+    return HOST_LIB
+
+
+def build_oracle():
+    _run(["make", "-B", "-C", os.path.join(ROOT, "oracle")])
+
+
+def build_all(verbose=False):
+    build_eik(verbose)
+    build_host(verbose)
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all(verbose=True)
+    print("built", EIK_LIB)

@@ -1,0 +1,91 @@
+"""ctypes binding of the C ABI in include/seisfat2d_eik.h.
+
+Loads seisfat2d_amd/libseisfat2d_eik.so (built in-tree by seisfat2d_amd.build);
+raises if it is missing — there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .build import EIK_LIB
+
+OK, EINVAL, EHIP, ENOMEM, ESTATE, EIO = 0, -1, -2, -3, -4, -5
+ISO, ANI = 0, 1
+SCHED_PLAIN, SCHED_TILED = 0, 1
+
+# Every function the header declares (checked by the CPU test suite).
+EXPORTS = [
+    "seisfat_last_error", "seisfat_eik_create", "seisfat_eik_destroy", "seisfat_eik_set_slowness",
+    "seisfat_eik_set_slowness_device", "seisfat_eik_set_stiffness", "seisfat_eik_solve",
+    "seisfat_eik_device_T", "seisfat_batch_create", "seisfat_batch_run", "seisfat_batch_picks",
+    "seisfat_batch_device_picks", "seisfat_batch_device_T", "seisfat_batch_copy_T",
+    "seisfat_batch_num_launches", "seisfat_batch_destroy",
+]
+
+
+class SeisfatError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"seisfat error {code}: {msg}")
+        self.code = code
+
+
+class Shot(C.Structure):
+    _fields_ = [("sx", C.c_float), ("sz", C.c_float), ("x0", C.c_int32), ("width", C.c_int32),
+                ("rec_offset", C.c_int32), ("nrec", C.c_int32)]
+
+
+_lib = None
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(EIK_LIB):
+            raise RuntimeError(f"{EIK_LIB} is not built; run __graft_entry__.build()")
+        L = C.CDLL(EIK_LIB)
+        vp, i, f = C.c_void_p, C.c_int, C.c_float
+        L.seisfat_last_error.restype = C.c_char_p
+        L.seisfat_last_error.argtypes = []
+        L.seisfat_eik_create.argtypes = [i, i, f, f, C.POINTER(vp)]
+        L.seisfat_eik_destroy.argtypes = [vp]
+        L.seisfat_eik_set_slowness.argtypes = [vp, f32p]
+        L.seisfat_eik_set_slowness_device.argtypes = [vp, vp, vp]
+        L.seisfat_eik_set_stiffness.argtypes = [vp, C.POINTER(vp)]
+        L.seisfat_eik_solve.argtypes = [vp, i, i, f, f, vp, vp, vp, i, vp]
+        L.seisfat_eik_device_T.restype = vp
+        L.seisfat_eik_device_T.argtypes = [vp]
+        L.seisfat_batch_create.argtypes = [vp, i, i, i, C.POINTER(Shot), i, vp, vp, C.POINTER(vp)]
+        L.seisfat_batch_run.argtypes = [vp, vp]
+        L.seisfat_batch_picks.argtypes = [vp, vp]
+        L.seisfat_batch_device_picks.restype = vp
+        L.seisfat_batch_device_picks.argtypes = [vp]
+        L.seisfat_batch_device_T.restype = vp
+        L.seisfat_batch_device_T.argtypes = [vp, i, C.POINTER(i), C.POINTER(i)]
+        L.seisfat_batch_copy_T.argtypes = [vp, i, vp]
+        L.seisfat_batch_num_launches.argtypes = [vp]
+        L.seisfat_batch_destroy.argtypes = [vp]
+        ptr_ret = {"seisfat_last_error", "seisfat_eik_device_T", "seisfat_batch_device_picks",
+                   "seisfat_batch_device_T"}
+        for name in EXPORTS:
+            if name not in ptr_ret:
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        raise SeisfatError(rc, lib().seisfat_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, np.float32)

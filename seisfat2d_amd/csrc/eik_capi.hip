@@ -1,0 +1,554 @@
+// C ABI of the eikonal solver: model state, validation, batch plans, HIP graphs.
+//
+// Drop-in for the device side of SeisFAT2D's Modeling classes
+// (src/modeling/modeling.cuh:17-79): the reference keeps d_S/d_T per Modeling
+// object and solves one shot per time_propagation(); here a handle owns the
+// row-padded model (and compressed stiffness), and a batch owns the per-shot
+// traveltime grids plus, for the tiled schedule, a HIP graph of the whole
+// solve.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/seisfat2d_eik.h"
+#include "eik_kernels.h"
+
+using namespace seisfat;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(e_ == hipErrorOutOfMemory ? SEISFAT_ENOMEM : SEISFAT_EHIP,              \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+  } while (0)
+
+template <class T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return SEISFAT_OK;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(SEISFAT_ENOMEM, std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) +
+                                    " bytes: " + hipGetErrorString(e));
+  }
+  return SEISFAT_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+}  // namespace
+
+struct seisfat_eik_s {
+  int nz = 0, nx = 0, nzz = 0, nxx = 0;
+  float dz = 0.f, dx = 0.f;
+  EikConsts kc{};
+  bool eqh = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float* d_szp = nullptr;  // row-padded slowness, nzz x nx
+  bool has_s = false;
+  uint16_t* d_c[6] = {};
+  float cmin[6] = {}, cmax[6] = {};
+  bool has_c = false;
+  int* d_flag = nullptr;
+  seisfat_batch_s* single = nullptr;  // batch of the last seisfat_eik_solve
+};
+
+struct seisfat_batch_s {
+  seisfat_eik_s* h = nullptr;
+  int type = SEISFAT_ISO, schedule = SEISFAT_SCHED_TILED;
+  int nshots = 0, nrec = 0;
+  std::vector<seisfat_shot> shots;
+  std::vector<ShotDev> iso, qp;  // host copies of the shot tables
+  std::vector<size_t> toff;      // offset of each shot's T grid (floats)
+  size_t tsize = 0;
+  ShotDev* d_iso = nullptr;
+  ShotDev* d_qp = nullptr;
+  float* d_T = nullptr;
+  float* d_swork = nullptr;
+  float* d_rx = nullptr;
+  float* d_rz = nullptr;
+  int* d_recshot = nullptr;
+  float* d_picks = nullptr;
+  int4* d_tiles = nullptr;
+  std::vector<int> diag_off;
+  int launches = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+int check_handle(seisfat_eik_t h) {
+  if (!h) return fail(SEISFAT_EINVAL, "null handle");
+  return SEISFAT_OK;
+}
+
+EikConsts make_consts(float dx, float dz) {
+  EikConsts k;
+  k.dx = dx;
+  k.dz = dz;
+  k.dz2i = 1.0f / (dz * dz);
+  k.dx2i = 1.0f / (dx * dx);
+  k.hyp = std::sqrt(dx * dx + dz * dz);
+  k.sum = k.dz2i + k.dx2i;
+  k.prod = k.dz2i * k.dx2i;
+  return k;
+}
+
+void free_batch(seisfat_batch_s* b) {
+  if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  dfree(b->d_iso);
+  dfree(b->d_qp);
+  dfree(b->d_T);
+  dfree(b->d_swork);
+  dfree(b->d_rx);
+  dfree(b->d_rz);
+  dfree(b->d_recshot);
+  dfree(b->d_picks);
+  dfree(b->d_tiles);
+  delete b;
+}
+
+// Enqueue one full solve of the batch (all shots) on stream st.
+int enqueue_solve(seisfat_batch_s* b, hipStream_t st) {
+  seisfat_eik_s* h = b->h;
+  const int nzz = h->nzz;
+  int launches = 0;
+  auto one_solve = [&](ShotDev* d_tab, const std::vector<ShotDev>& tab) -> int {
+    launch_fill(b->d_T, b->tsize, TINF, st);
+    launch_init(d_tab, b->nshots, nzz, h->dx, h->dz, st);
+    launches += 2;
+    if (b->schedule == SEISFAT_SCHED_TILED) {
+      launches += launch_sweeps_tiled(d_tab, b->d_tiles, b->diag_off, nzz, h->kc, h->eqh, st);
+    } else {
+      for (int k = 0; k < b->nshots; ++k) launches += launch_sweeps_plain(tab[k], nzz, h->kc, h->eqh, st);
+    }
+    HIPCHK(hipGetLastError());
+    return SEISFAT_OK;
+  };
+  int rc = one_solve(b->d_iso, b->iso);
+  if (rc) return rc;
+  if (b->type == SEISFAT_ANI) {
+    StiffDev sf;
+    for (int c = 0; c < 6; ++c) {
+      sf.c[c] = h->d_c[c];
+      sf.mn[c] = h->cmin[c];
+      sf.mx[c] = h->cmax[c];
+    }
+    launch_quasi_slowness(b->d_iso, b->d_qp, b->nshots, nzz, h->nxx, h->dx, h->dz, sf, st);
+    ++launches;
+    rc = one_solve(b->d_qp, b->qp);
+    if (rc) return rc;
+  }
+  // Picks read the final traveltimes (the T pointers are shared by both tables).
+  launch_picks(b->d_iso, b->d_recshot, b->d_rx, b->d_rz, b->d_picks, b->nrec, nzz, h->dx, h->dz, st);
+  if (b->nrec > 0) ++launches;
+  HIPCHK(hipGetLastError());
+  b->launches = launches;
+  return SEISFAT_OK;
+}
+
+// Tiles of every shot, grouped by tile diagonal d = 2r + bb (one launch each).
+void build_tiles(seisfat_batch_s* b, std::vector<int4>& tiles) {
+  const int NR = b->h->nzz - 2;
+  const int NRS = (NR + 63) / 64;
+  std::vector<int> NBs(b->nshots);
+  int Dmax = 0;
+  for (int k = 0; k < b->nshots; ++k) {
+    const int NC = b->iso[k].nxx - 2;
+    NBs[k] = (NC + 63 + 63) / 64;
+    Dmax = std::max(Dmax, 2 * (NRS - 1) + NBs[k]);
+  }
+  b->diag_off.assign(1, 0);
+  tiles.clear();
+  for (int d = 0; d < Dmax; ++d) {
+    for (int r = 0; r < NRS; ++r) {
+      const int bb = d - 2 * r;
+      if (bb < 0) break;
+      for (int k = 0; k < b->nshots; ++k)
+        if (bb < NBs[k]) tiles.push_back(make_int4(k, r, bb, 0));
+    }
+    b->diag_off.push_back((int)tiles.size());
+  }
+}
+
+int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seisfat_shot* shots,
+                 int nrec_total, const float* rx, const float* rz, bool use_graph, seisfat_batch_s** out) {
+  *out = nullptr;
+  if (type != SEISFAT_ISO && type != SEISFAT_ANI) return fail(SEISFAT_EINVAL, "type must be ISO (0) or ANI (1)");
+  if (schedule != SEISFAT_SCHED_PLAIN && schedule != SEISFAT_SCHED_TILED)
+    return fail(SEISFAT_EINVAL, "schedule must be PLAIN (0) or TILED (1)");
+  if (!h->has_s) return fail(SEISFAT_ESTATE, "slowness not set (seisfat_eik_set_slowness)");
+  if (type == SEISFAT_ANI && !h->has_c) return fail(SEISFAT_ESTATE, "stiffness not set (seisfat_eik_set_stiffness)");
+  if (nshots <= 0 || !shots) return fail(SEISFAT_EINVAL, "nshots must be positive");
+  if (nrec_total < 0 || (nrec_total > 0 && (!rx || !rz))) return fail(SEISFAT_EINVAL, "bad receiver arrays");
+  const float dx = h->dx, dz = h->dz;
+  std::vector<int> recshot(nrec_total, -1);
+  std::vector<float> rxr(nrec_total), rzr(nrec_total);
+  for (int k = 0; k < nshots; ++k) {
+    const seisfat_shot& s = shots[k];
+    const std::string tag = "shot " + std::to_string(k) + ": ";
+    if (s.width < 1 || s.x0 < 0 || (long long)s.x0 + s.width > h->nx)
+      return fail(SEISFAT_EINVAL, tag + "window [x0, x0+width) outside the model");
+    if (type == SEISFAT_ANI && (s.x0 != 0 || s.width != h->nx))
+      return fail(SEISFAT_EINVAL, tag + "anisotropic shots must cover the whole model");
+    const float sxr = s.sx - (float)s.x0 * dx;
+    if (!(sxr >= 0.0f && sxr <= (float)(s.width - 1) * dx && s.sz >= 0.0f && s.sz <= (float)(h->nz - 1) * dz))
+      return fail(SEISFAT_EINVAL, tag + "source outside its window");
+    if (s.nrec < 0 || s.rec_offset < 0 || (long long)s.rec_offset + s.nrec > nrec_total)
+      return fail(SEISFAT_EINVAL, tag + "receiver range outside the receiver arrays");
+    for (int r = s.rec_offset; r < s.rec_offset + s.nrec; ++r) {
+      if (recshot[r] != -1) return fail(SEISFAT_EINVAL, "receiver " + std::to_string(r) + " assigned to two shots");
+      const float xr = rx[r] - (float)s.x0 * dx;
+      if (!(xr >= 0.0f && xr <= (float)(s.width - 1) * dx && rz[r] >= 0.0f && rz[r] <= (float)(h->nz - 1) * dz))
+        return fail(SEISFAT_EINVAL, tag + "receiver " + std::to_string(r) + " outside the shot window");
+      recshot[r] = k;
+      rxr[r] = xr;
+      rzr[r] = rz[r];
+    }
+  }
+  for (int r = 0; r < nrec_total; ++r)
+    if (recshot[r] < 0) return fail(SEISFAT_EINVAL, "receiver " + std::to_string(r) + " belongs to no shot");
+
+  seisfat_batch_s* b = new (std::nothrow) seisfat_batch_s();
+  if (!b) return fail(SEISFAT_ENOMEM, "host allocation");
+  b->h = h;
+  b->type = type;
+  b->schedule = schedule;
+  b->nshots = nshots;
+  b->nrec = nrec_total;
+  b->shots.assign(shots, shots + nshots);
+  b->iso.resize(nshots);
+  b->toff.resize(nshots);
+  const int nzz = h->nzz;
+  size_t off = 0;
+  for (int k = 0; k < nshots; ++k) {
+    const seisfat_shot& s = shots[k];
+    b->toff[k] = off;
+    ShotDev& sd = b->iso[k];
+    sd.nxx = s.width + 2 * NB;
+    sd.sbase = h->d_szp + (size_t)s.x0 * nzz;
+    sd.sofs = NB;
+    sd.smax = s.width - 1;
+    sd.sx = s.sx - (float)s.x0 * dx;
+    sd.sz = s.sz;
+    off += (size_t)nzz * sd.nxx;
+  }
+  b->tsize = off;
+  int rc;
+#define TRY(x)        \
+  do {                \
+    rc = (x);         \
+    if (rc) {         \
+      free_batch(b);  \
+      return rc;      \
+    }                 \
+  } while (0)
+  TRY(dalloc(&b->d_T, b->tsize));
+  for (int k = 0; k < nshots; ++k) b->iso[k].T = b->d_T + b->toff[k];
+  if (type == SEISFAT_ANI) {
+    TRY(dalloc(&b->d_swork, b->tsize));
+    b->qp = b->iso;
+    for (int k = 0; k < nshots; ++k) {
+      b->qp[k].sbase = b->d_swork + b->toff[k];
+      b->qp[k].sofs = 0;
+      b->qp[k].smax = b->qp[k].nxx - 1;
+    }
+    TRY(dalloc(&b->d_qp, nshots));
+  }
+  TRY(dalloc(&b->d_iso, nshots));
+  TRY(dalloc(&b->d_rx, nrec_total));
+  TRY(dalloc(&b->d_rz, nrec_total));
+  TRY(dalloc(&b->d_recshot, nrec_total));
+  TRY(dalloc(&b->d_picks, nrec_total));
+  std::vector<int4> tiles;
+  if (schedule == SEISFAT_SCHED_TILED) {
+    build_tiles(b, tiles);
+    TRY(dalloc(&b->d_tiles, tiles.size()));
+  }
+  auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (bytes == 0) return SEISFAT_OK;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+    return SEISFAT_OK;
+  };
+  TRY(up(b->d_iso, b->iso.data(), nshots * sizeof(ShotDev)));
+  if (type == SEISFAT_ANI) TRY(up(b->d_qp, b->qp.data(), nshots * sizeof(ShotDev)));
+  TRY(up(b->d_rx, rxr.data(), nrec_total * sizeof(float)));
+  TRY(up(b->d_rz, rzr.data(), nrec_total * sizeof(float)));
+  TRY(up(b->d_recshot, recshot.data(), nrec_total * sizeof(int)));
+  TRY(up(b->d_tiles, tiles.data(), tiles.size() * sizeof(int4)));
+  {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+      free_batch(b);
+      return fail(SEISFAT_EHIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+  }
+  if (use_graph && schedule == SEISFAT_SCHED_TILED) {
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      free_batch(b);
+      return fail(SEISFAT_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    rc = enqueue_solve(b, h->stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(h->stream, &g);
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      free_batch(b);
+      return rc ? rc : fail(SEISFAT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    b->graph = g;
+    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      free_batch(b);
+      return fail(SEISFAT_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  *out = b;
+  return SEISFAT_OK;
+#undef TRY
+}
+
+int run_batch(seisfat_batch_s* b, hipStream_t st) {
+  if (!st) st = b->h->stream;
+  if (b->gexec) {
+    HIPCHK(hipGraphLaunch(b->gexec, st));
+    return SEISFAT_OK;
+  }
+  return enqueue_solve(b, st);
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* seisfat_last_error(void) { return g_err.c_str(); }
+
+int seisfat_eik_create(int nz, int nx, float dz, float dx, seisfat_eik_t* out) {
+  if (!out) return fail(SEISFAT_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (nz < 2 || nx < 2) return fail(SEISFAT_EINVAL, "model needs at least 2x2 samples");
+  if ((long long)(nz + 6) * (nx + 6) > (1LL << 30)) return fail(SEISFAT_EINVAL, "model too large (> 2^30 padded nodes)");
+  if (!(dz > 0.f) || !(dx > 0.f) || !std::isfinite(dz) || !std::isfinite(dx))
+    return fail(SEISFAT_EINVAL, "spacings must be positive and finite");
+  seisfat_eik_s* h = new (std::nothrow) seisfat_eik_s();
+  if (!h) return fail(SEISFAT_ENOMEM, "host allocation");
+  h->nz = nz;
+  h->nx = nx;
+  h->nzz = nz + 2 * NB;
+  h->nxx = nx + 2 * NB;
+  h->dz = dz;
+  h->dx = dx;
+  h->kc = make_consts(dx, dz);
+  h->eqh = (dx == dz);
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(SEISFAT_EHIP, std::string("device/stream: ") + hipGetErrorString(e));
+  }
+  int rc = dalloc(&h->d_szp, (size_t)h->nzz * nx);
+  if (!rc) rc = dalloc(&h->d_flag, 1);
+  if (rc) {
+    dfree(h->d_szp);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return SEISFAT_OK;
+}
+
+int seisfat_eik_destroy(seisfat_eik_t h) {
+  if (!h) return SEISFAT_OK;
+  (void)hipStreamSynchronize(h->stream);
+  free_batch(h->single);
+  dfree(h->d_szp);
+  dfree(h->d_flag);
+  for (auto& c : h->d_c) dfree(c);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return SEISFAT_OK;
+}
+
+int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* stream) {
+  if (int rc = check_handle(h)) return rc;
+  if (!d_s) return fail(SEISFAT_EINVAL, "null slowness pointer");
+  hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+  HIPCHK(hipMemsetAsync(h->d_flag, 0, sizeof(int), st));
+  launch_check_positive(d_s, (size_t)h->nz * h->nx, h->d_flag, st);
+  launch_zpad(d_s, h->d_szp, h->nz, h->nx, st);
+  HIPCHK(hipGetLastError());
+  int bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (bad) {
+    h->has_s = false;
+    return fail(SEISFAT_EINVAL, "slowness must be positive and finite everywhere");
+  }
+  h->has_s = true;
+  return SEISFAT_OK;
+}
+
+int seisfat_eik_set_slowness(seisfat_eik_t h, const float* s) {
+  if (int rc = check_handle(h)) return rc;
+  if (!s) return fail(SEISFAT_EINVAL, "null slowness pointer");
+  const size_t n = (size_t)h->nz * h->nx;
+  for (size_t k = 0; k < n; ++k)
+    if (!(s[k] > 0.0f) || !std::isfinite(s[k])) {
+      h->has_s = false;
+      return fail(SEISFAT_EINVAL, "slowness must be positive and finite everywhere (index " + std::to_string(k) + ")");
+    }
+  float* d_s = nullptr;
+  if (int rc = dalloc(&d_s, n)) return rc;
+  hipError_t e = hipMemcpyAsync(d_s, s, n * sizeof(float), hipMemcpyHostToDevice, h->stream);
+  int rc = SEISFAT_OK;
+  if (e != hipSuccess)
+    rc = fail(SEISFAT_EHIP, std::string("upload slowness: ") + hipGetErrorString(e));
+  else
+    rc = seisfat_eik_set_slowness_device(h, d_s, nullptr);
+  (void)hipStreamSynchronize(h->stream);
+  dfree(d_s);
+  return rc;
+}
+
+int seisfat_eik_set_stiffness(seisfat_eik_t h, const float* const c[6]) {
+  if (int rc = check_handle(h)) return rc;
+  if (!c) return fail(SEISFAT_EINVAL, "null stiffness array");
+  const int nz = h->nz, nx = h->nx, nzz = h->nzz, nxx = h->nxx;
+  const size_t n = (size_t)nzz * nxx;
+  std::vector<float> pad(n);
+  std::vector<uint16_t> q(n);
+  for (int e = 0; e < 6; ++e) {
+    if (!c[e]) return fail(SEISFAT_EINVAL, "null stiffness component " + std::to_string(e));
+    for (size_t k = 0; k < (size_t)nz * nx; ++k)
+      if (!std::isfinite(c[e][k])) return fail(SEISFAT_EINVAL, "non-finite stiffness component " + std::to_string(e));
+    // expand_boundary (src/modeling/modeling.cu:204-232)
+    for (int j = 0; j < nxx; ++j) {
+      const int jm = std::min(std::max(j - NB, 0), nx - 1);
+      for (int i = 0; i < nzz; ++i) {
+        const int im = std::min(std::max(i - NB, 0), nz - 1);
+        pad[i + (size_t)j * nzz] = c[e][im + (size_t)jm * nz];
+      }
+    }
+    // compression with exact min/max (src/modeling/modeling.cu:264-279)
+    float mx = -1e20f, mn = 1e20f;
+    for (size_t k = 0; k < n; ++k) {
+      mn = std::min(pad[k], mn);
+      mx = std::max(pad[k], mx);
+    }
+    for (size_t k = 0; k < n; ++k)
+      q[k] = static_cast<uint16_t>(1.0f + (65535 - 1) * (pad[k] - mn) / (mx - mn));
+    h->cmin[e] = mn;
+    h->cmax[e] = mx;
+    if (!h->d_c[e])
+      if (int rc = dalloc(&h->d_c[e], n)) return rc;
+    HIPCHK(hipMemcpyAsync(h->d_c[e], q.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  h->has_c = true;
+  return SEISFAT_OK;
+}
+
+int seisfat_eik_solve(seisfat_eik_t h, int type, int schedule, float sx, float sz, float* T_padded,
+                      const float* rx, const float* rz, int nrec, float* picks) {
+  if (int rc = check_handle(h)) return rc;
+  if (nrec < 0 || (nrec > 0 && !picks)) return fail(SEISFAT_EINVAL, "picks buffer required when nrec > 0");
+  free_batch(h->single);
+  h->single = nullptr;
+  seisfat_shot s{sx, sz, 0, h->nx, 0, nrec};
+  seisfat_batch_s* b = nullptr;
+  int rc = create_batch(h, type, schedule, 1, &s, nrec, rx, rz, /*use_graph=*/false, &b);
+  if (rc) return rc;
+  h->single = b;
+  if ((rc = run_batch(b, h->stream))) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (T_padded) HIPCHK(hipMemcpy(T_padded, b->d_T, b->tsize * sizeof(float), hipMemcpyDeviceToHost));
+  if (nrec > 0) HIPCHK(hipMemcpy(picks, b->d_picks, nrec * sizeof(float), hipMemcpyDeviceToHost));
+  return SEISFAT_OK;
+}
+
+const float* seisfat_eik_device_T(seisfat_eik_t h) {
+  if (!h || !h->single) {
+    g_err = "no solve yet";
+    return nullptr;
+  }
+  return h->single->d_T;
+}
+
+int seisfat_batch_create(seisfat_eik_t h, int type, int schedule, int nshots, const seisfat_shot* shots,
+                         int nrec_total, const float* rx, const float* rz, seisfat_batch_t* out) {
+  if (!out) return fail(SEISFAT_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (int rc = check_handle(h)) return rc;
+  return create_batch(h, type, schedule, nshots, shots, nrec_total, rx, rz, /*use_graph=*/true, out);
+}
+
+int seisfat_batch_run(seisfat_batch_t b, void* stream) {
+  if (!b) return fail(SEISFAT_EINVAL, "null batch");
+  return run_batch(b, (hipStream_t)stream);
+}
+
+int seisfat_batch_picks(seisfat_batch_t b, float* picks) {
+  if (!b) return fail(SEISFAT_EINVAL, "null batch");
+  if (b->nrec > 0 && !picks) return fail(SEISFAT_EINVAL, "null picks buffer");
+  HIPCHK(hipDeviceSynchronize());
+  if (b->nrec > 0) HIPCHK(hipMemcpy(picks, b->d_picks, b->nrec * sizeof(float), hipMemcpyDeviceToHost));
+  return SEISFAT_OK;
+}
+
+const float* seisfat_batch_device_picks(seisfat_batch_t b) { return b ? b->d_picks : nullptr; }
+
+const float* seisfat_batch_device_T(seisfat_batch_t b, int shot, int* nzz, int* nxx) {
+  if (!b || shot < 0 || shot >= b->nshots) {
+    g_err = "bad batch or shot index";
+    return nullptr;
+  }
+  if (nzz) *nzz = b->h->nzz;
+  if (nxx) *nxx = b->iso[shot].nxx;
+  return b->d_T + b->toff[shot];
+}
+
+int seisfat_batch_copy_T(seisfat_batch_t b, int shot, float* T_padded) {
+  if (!b || shot < 0 || shot >= b->nshots || !T_padded) return fail(SEISFAT_EINVAL, "bad batch, shot or buffer");
+  HIPCHK(hipDeviceSynchronize());
+  const size_t n = (size_t)b->h->nzz * b->iso[shot].nxx;
+  HIPCHK(hipMemcpy(T_padded, b->d_T + b->toff[shot], n * sizeof(float), hipMemcpyDeviceToHost));
+  return SEISFAT_OK;
+}
+
+int seisfat_batch_num_launches(seisfat_batch_t b) { return b ? b->launches : -1; }
+
+int seisfat_batch_destroy(seisfat_batch_t b) {
+  if (!b) return SEISFAT_OK;
+  (void)hipStreamSynchronize(b->h->stream);
+  free_batch(b);
+  return SEISFAT_OK;
+}
+
+}  // extern "C"

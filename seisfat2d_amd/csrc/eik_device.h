@@ -1,0 +1,100 @@
+// Device-side types and the eikonal local solver shared by every sweep kernel.
+//
+// Arithmetic contract: all kernels are compiled with -ffp-contract=off and
+// HIP's correctly rounded fp32 division and square root, so each expression
+// below rounds exactly as the reference's C++ reads (no FMA contraction).
+// The result is bit-identical to the CPU oracle for every node.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace seisfat {
+
+constexpr int NB = 3;  // replicated border width (src/modeling/modeling.cu:113)
+constexpr float TINF = 1e6f;  // "not reached" (src/modeling/modeling.cu:286-291)
+
+// Uniform constants of one model geometry.  hyp, sum and prod are the
+// sub-expressions the reference evaluates per thread from dx, dz
+// (src/modeling/modeling.cu:337-351); precomputing them with the same float
+// operations gives the same values.
+struct EikConsts {
+  float dx, dz;
+  float dx2i, dz2i;  // 1/dx^2, 1/dz^2   (src/modeling/modeling.cu:64-65)
+  float hyp;         // sqrtf(dx*dx + dz*dz)
+  float sum;         // dz2i + dx2i
+  float prod;        // dz2i * dx2i
+};
+
+// A shot as the kernels see it.  T is the shot's padded traveltime grid
+// (nzz x nxx, column stride nzz).  Slowness of padded node/cell (i, j) is
+//   sbase[i + clamp(j - sofs, 0, smax) * nzz]
+// which covers both a window of the row-padded model (sbase = model column
+// x0, sofs = 3, smax = width-1: the window's own replicated x border) and a
+// fully padded per-shot grid (sofs = 0, smax = nxx-1).
+struct ShotDev {
+  float* T;
+  const float* sbase;
+  int sofs, smax;
+  int nxx;
+  float sx, sz;  // source position relative to the shot's grid origin
+};
+
+// Pointers held in ShotDev are read from memory, so the compiler cannot see
+// that they address global memory; these casts give global (not flat) loads
+// and stores.
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ __forceinline__ gfloat* gptr(float* p) { return (gfloat*)p; }
+__device__ __forceinline__ const gfloat* gptr(const float* p) { return (const gfloat*)p; }
+
+__device__ __forceinline__ float s_at(const ShotDev& sd, int nzz, int i, int j) {
+  const int jm = min(max(j - sd.sofs, 0), sd.smax);
+  return gptr(sd.sbase)[i + (size_t)jm * nzz];
+}
+
+// Local solver of the reference's inner_sweep (src/modeling/modeling.cu:
+// 308-357) for one node, given its upwind neighbours in the current sweep
+// (tv: along z, te: along x, tev: diagonal) and the three slowness cells it
+// reads: sref = the upwind cell S[i1][j1], szo = the other cell along the
+// z-edge (S[i1][j1+b]), sxo = the other cell along the x-edge (S[i1+a][j1]).
+// The reference's max(j-1,1)/max(i-1,1) clamps select cell column/row 1 in
+// place of 0; replicated padding makes those cells equal, so they are not
+// repeated here.  Returns min(t1D, t2D) — the caller takes min with T_old.
+template <bool EQH>
+__device__ __forceinline__ float eik_local(float tv, float te, float tev, float sref, float szo,
+                                           float sxo, const EikConsts& k) {
+  const float t1d1 = tv + k.dz * fminf(sref, szo);
+  const float t1d2 = te + k.dx * fminf(sref, sxo);
+  const float t1D = fminf(t1d1, t1d2);
+  const float de = te - tev;
+  const float dv = tv - tev;
+  float t2D = TINF;
+  if ((tv <= te + k.dx * sref) && (te <= tv + k.dz * sref) && (de >= 0.0f) && (dv >= 0.0f)) {
+    // branch A: plane-wave update from both upwind neighbours
+    const float ta = tev + te - tv;
+    const float tb = tev - te + tv;
+    const float dab = ta - tb;
+    t2D = ((tb * k.dz2i + ta * k.dx2i) + sqrtf(4.0f * sref * sref * k.sum - k.prod * dab * dab)) / k.sum;
+  } else {
+    // branches B (from te) and C (from tv) are exclusive else-ifs in the
+    // reference; one evaluation serves whichever a lane takes.
+    const float thrB = sref * k.dz * k.dz / k.hyp;
+    const bool cB = (de <= thrB) && (de > 0.0f);
+    bool cC = false;
+    if (!cB) {
+      const float thrC = EQH ? thrB : sref * k.dx * k.dx / k.hyp;
+      cC = (dv <= thrC) && (dv > 0.0f);
+    }
+    if (cB || cC) {
+      const float dd = cB ? de : dv;
+      const float h1 = cB ? k.dz : k.dx;
+      const float h2 = cB ? k.dx : k.dz;
+      const float t0 = cB ? te : tv;
+      const float q = dd / h1;
+      t2D = t0 + h2 * sqrtf(sref * sref - q * q);
+    }
+  }
+  return fminf(t1D, fminf(t2D, TINF));
+}
+
+}  // namespace seisfat

@@ -1,0 +1,33 @@
+// Launch interface of the eikonal kernels (host side of eik_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "eik_device.h"
+
+namespace seisfat {
+
+// Compressed stiffness of the padded model (src/modeling/eikonal_ani.cuh:10-17).
+struct StiffDev {
+  const uint16_t* c[6];  // C11, C13, C15, C33, C35, C55
+  float mn[6], mx[6];
+};
+
+void launch_fill(float* p, size_t n, float v, hipStream_t st);
+void launch_zpad(const float* s, float* szp, int nz, int nx, hipStream_t st);
+void launch_check_positive(const float* s, size_t n, int* bad, hipStream_t st);
+void launch_init(const ShotDev* shots, int nshots, int nzz, float dx, float dz, hipStream_t st);
+// Returns the number of kernel launches issued.
+int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh, hipStream_t st);
+int launch_sweeps_tiled(const ShotDev* shots, const int4* tiles, const std::vector<int>& diag_off, int nzz,
+                        const EikConsts& k, bool eqh, hipStream_t st);
+void launch_picks(const ShotDev* shots, const int* rec_shot, const float* rx, const float* rz, float* out,
+                  int nrec, int nzz, float dx, float dz, hipStream_t st);
+void launch_quasi_slowness(const ShotDev* iso, ShotDev* qp, int nshots, int nzz, int nxx, float dx, float dz,
+                           const StiffDev& stf, hipStream_t st);
+
+}  // namespace seisfat

@@ -1,0 +1,147 @@
+"""Python front end of the C ABI (used by the tests, the benchmark and the
+autograd wrapper; C++ callers use seisfat2d_amd/host/ instead).
+
+Arrays follow the reference's z-fastest layout: a model of nz x nx samples is
+a numpy array of shape (nx, nz) in C order (``a.ravel()[i + j*nz]`` is sample
+z=i, x=j); a padded traveltime grid has shape (nx+6, nz+6).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import ANI, ISO, SCHED_PLAIN, SCHED_TILED, Shot, check, lib
+
+__all__ = ["Eikonal", "Batch", "ISO", "ANI", "SCHED_PLAIN", "SCHED_TILED", "NB"]
+
+NB = 3
+
+
+class Eikonal:
+    """One model on the GPU (Modeling::set_parameters + set_properties)."""
+
+    def __init__(self, nz: int, nx: int, dz: float, dx: float):
+        self.nz, self.nx, self.dz, self.dx = int(nz), int(nx), float(dz), float(dx)
+        self.nzz, self.nxx = self.nz + 2 * NB, self.nx + 2 * NB
+        h = C.c_void_p()
+        check(lib().seisfat_eik_create(self.nz, self.nx, self.dz, self.dx, C.byref(h)))
+        self._h = h
+
+    # -- model -------------------------------------------------------------
+    def _model(self, a):
+        a = np.ascontiguousarray(a, np.float32)
+        if a.shape != (self.nx, self.nz):
+            raise ValueError(f"model must have shape (nx, nz) = {(self.nx, self.nz)}, got {a.shape}")
+        return a
+
+    def set_slowness(self, s):
+        """s: (nx, nz) numpy array, or a CUDA tensor of nx*nz float32 (device copy)."""
+        if hasattr(s, "data_ptr") and getattr(s, "is_cuda", False):
+            import torch
+            t = s.detach().contiguous().to(torch.float32)
+            if t.numel() != self.nx * self.nz:
+                raise ValueError("slowness tensor has the wrong number of elements")
+            stream = torch.cuda.current_stream(t.device).cuda_stream
+            check(lib().seisfat_eik_set_slowness_device(self._h, C.c_void_p(t.data_ptr()), C.c_void_p(stream)))
+        else:
+            check(lib().seisfat_eik_set_slowness(self._h, self._model(s)))
+
+    def set_stiffness(self, cs):
+        """cs: six (nx, nz) arrays C11, C13, C15, C33, C35, C55."""
+        if len(cs) != 6:
+            raise ValueError("need six stiffness components")
+        arrs = [self._model(c) for c in cs]
+        ptrs = (C.c_void_p * 6)(*[a.ctypes.data_as(C.c_void_p) for a in arrs])
+        check(lib().seisfat_eik_set_stiffness(self._h, ptrs))
+
+    # -- solves -------------------------------------------------------------
+    def solve(self, sx, sz, type=ISO, schedule=SCHED_TILED, rx=None, rz=None, return_T=True):
+        """One shot on the whole model; returns (padded T (nxx, nzz) or None, picks or None)."""
+        T = np.empty((self.nxx, self.nzz), np.float32) if return_T else None
+        rx = None if rx is None else np.ascontiguousarray(rx, np.float32)
+        rz = None if rz is None else np.ascontiguousarray(rz, np.float32)
+        nrec = 0 if rx is None else len(rx)
+        if nrec and (rz is None or len(rz) != nrec):
+            raise ValueError("rx and rz must have the same length")
+        picks = np.empty(nrec, np.float32) if nrec else None
+        check(lib().seisfat_eik_solve(self._h, type, schedule, float(sx), float(sz), capi._ptr(T),
+                                      capi._ptr(rx), capi._ptr(rz), nrec, capi._ptr(picks)))
+        return T, picks
+
+    def device_T_ptr(self):
+        return lib().seisfat_eik_device_T(self._h)
+
+    def batch(self, shots, rx=None, rz=None, type=ISO, schedule=SCHED_TILED):
+        return Batch(self, shots, rx, rz, type, schedule)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().seisfat_eik_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many shots solved together (tiled schedule: one HIP graph replay per run).
+
+    shots: sequence of (sx, sz, x0, width, rec_offset, nrec) tuples.
+    """
+
+    def __init__(self, eik: Eikonal, shots, rx, rz, type=ISO, schedule=SCHED_TILED):
+        self.eik = eik
+        n = len(shots)
+        arr = (Shot * n)(*[Shot(float(s[0]), float(s[1]), int(s[2]), int(s[3]), int(s[4]), int(s[5]))
+                           for s in shots])
+        self.shots = [tuple(s) for s in shots]
+        self.rx = np.ascontiguousarray(rx if rx is not None else np.zeros(0), np.float32)
+        self.rz = np.ascontiguousarray(rz if rz is not None else np.zeros(0), np.float32)
+        self.nrec = len(self.rx)
+        b = C.c_void_p()
+        check(lib().seisfat_batch_create(eik._h, type, schedule, n, arr, self.nrec,
+                                         capi._ptr(self.rx), capi._ptr(self.rz), C.byref(b)))
+        self._b = b
+
+    def run(self, stream=None):
+        check(lib().seisfat_batch_run(self._b, None if stream is None else C.c_void_p(stream)))
+
+    def picks(self):
+        out = np.empty(self.nrec, np.float32)
+        check(lib().seisfat_batch_picks(self._b, capi._ptr(out)))
+        return out
+
+    def device_picks_ptr(self):
+        return lib().seisfat_batch_device_picks(self._b)
+
+    def T(self, shot):
+        nzz, nxx = C.c_int(), C.c_int()
+        lib().seisfat_batch_device_T(self._b, shot, C.byref(nzz), C.byref(nxx))
+        out = np.empty((nxx.value, nzz.value), np.float32)
+        check(lib().seisfat_batch_copy_T(self._b, shot, capi._ptr(out)))
+        return out
+
+    def device_T_ptr(self, shot):
+        nzz, nxx = C.c_int(), C.c_int()
+        p = lib().seisfat_batch_device_T(self._b, shot, C.byref(nzz), C.byref(nxx))
+        return p, nzz.value, nxx.value
+
+    @property
+    def num_launches(self):
+        return lib().seisfat_batch_num_launches(self._b)
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().seisfat_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
