@@ -109,8 +109,13 @@ const float* seisfat_batch_device_picks(seisfat_batch_t b);
 const float* seisfat_batch_device_T(seisfat_batch_t b, int shot, int* nzz, int* nxx);
 /* Copy shot k's padded traveltimes to host memory (nzz*nxx floats). */
 int seisfat_batch_copy_T(seisfat_batch_t b, int shot, float* T_padded);
-/* Number of HIP kernel launches of one run (diagnostics). */
+/* Number of HIP kernel launches of one run, and of them the sweep-kernel
+ * launches (diagnostics). */
 int seisfat_batch_num_launches(seisfat_batch_t b);
+int seisfat_batch_num_sweep_launches(seisfat_batch_t b);
+/* Device time in ms of the sweep phase of the last completed run (from HIP
+ * events recorded on the run's stream around the sweep launches). */
+int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms);
 int seisfat_batch_destroy(seisfat_batch_t b);
 
 #ifdef __cplusplus

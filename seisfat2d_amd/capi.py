@@ -22,7 +22,8 @@ EXPORTS = [
     "seisfat_eik_set_slowness_device", "seisfat_eik_set_stiffness", "seisfat_eik_solve",
     "seisfat_eik_device_T", "seisfat_batch_create", "seisfat_batch_run", "seisfat_batch_picks",
     "seisfat_batch_device_picks", "seisfat_batch_device_T", "seisfat_batch_copy_T",
-    "seisfat_batch_num_launches", "seisfat_batch_destroy",
+    "seisfat_batch_num_launches", "seisfat_batch_num_sweep_launches", "seisfat_batch_sweep_ms",
+    "seisfat_batch_destroy",
 ]
 
 
@@ -67,6 +68,8 @@ def lib():
         L.seisfat_batch_device_T.argtypes = [vp, i, C.POINTER(i), C.POINTER(i)]
         L.seisfat_batch_copy_T.argtypes = [vp, i, vp]
         L.seisfat_batch_num_launches.argtypes = [vp]
+        L.seisfat_batch_num_sweep_launches.argtypes = [vp]
+        L.seisfat_batch_sweep_ms.argtypes = [vp, C.POINTER(f)]
         L.seisfat_batch_destroy.argtypes = [vp]
         ptr_ret = {"seisfat_last_error", "seisfat_eik_device_T", "seisfat_batch_device_picks",
                    "seisfat_batch_device_T"}

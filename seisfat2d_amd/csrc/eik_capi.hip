@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -92,9 +93,13 @@ struct seisfat_batch_s {
   float* d_picks = nullptr;
   int4* d_tiles = nullptr;
   std::vector<int> diag_off;
-  int launches = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  int launches = 0, sweep_launches = 0;
+  hipEvent_t ev_sweep[2] = {nullptr, nullptr};
+  // The tiled solve is recorded as three graphs: initialisation, sweeps (for
+  // anisotropic batches: both solves and the qP slowness between them), and
+  // picks; the timing events are recorded on the stream between them.
+  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -118,8 +123,12 @@ EikConsts make_consts(float dx, float dz) {
 
 void free_batch(seisfat_batch_s* b) {
   if (!b) return;
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto& g : b->gexec)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& g : b->graph)
+    if (g) (void)hipGraphDestroy(g);
+  for (auto& ev : b->ev_sweep)
+    if (ev) (void)hipEventDestroy(ev);
   dfree(b->d_iso);
   dfree(b->d_qp);
   dfree(b->d_T);
@@ -132,20 +141,27 @@ void free_batch(seisfat_batch_s* b) {
   delete b;
 }
 
-// Enqueue one full solve of the batch (all shots) on stream st.
-int enqueue_solve(seisfat_batch_s* b, hipStream_t st) {
+// Enqueue one full solve of the batch (all shots) on stream st.  mark(0) is
+// called before the first sweep launch and mark(1) after the last one.
+using Mark = std::function<int(int)>;
+int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
   seisfat_eik_s* h = b->h;
   const int nzz = h->nzz;
-  int launches = 0;
+  int launches = 0, sweeps = 0;
   auto one_solve = [&](ShotDev* d_tab, const std::vector<ShotDev>& tab) -> int {
     launch_fill(b->d_T, b->tsize, TINF, st);
     launch_init(d_tab, b->nshots, nzz, h->dx, h->dz, st);
     launches += 2;
+    if (sweeps == 0)
+      if (int rc = mark(0)) return rc;
+    int n = 0;
     if (b->schedule == SEISFAT_SCHED_TILED) {
-      launches += launch_sweeps_tiled(d_tab, b->d_tiles, b->diag_off, nzz, h->kc, h->eqh, st);
+      n = launch_sweeps_tiled(d_tab, b->d_tiles, b->diag_off, nzz, h->kc, h->eqh, st);
     } else {
-      for (int k = 0; k < b->nshots; ++k) launches += launch_sweeps_plain(tab[k], nzz, h->kc, h->eqh, st);
+      for (int k = 0; k < b->nshots; ++k) n += launch_sweeps_plain(tab[k], nzz, h->kc, h->eqh, st);
     }
+    launches += n;
+    sweeps += n;
     HIPCHK(hipGetLastError());
     return SEISFAT_OK;
   };
@@ -163,11 +179,13 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st) {
     rc = one_solve(b->d_qp, b->qp);
     if (rc) return rc;
   }
+  if ((rc = mark(1))) return rc;
   // Picks read the final traveltimes (the T pointers are shared by both tables).
   launch_picks(b->d_iso, b->d_recshot, b->d_rx, b->d_rz, b->d_picks, b->nrec, nzz, h->dx, h->dz, st);
   if (b->nrec > 0) ++launches;
   HIPCHK(hipGetLastError());
   b->launches = launches;
+  b->sweep_launches = sweeps;
   return SEISFAT_OK;
 }
 
@@ -267,6 +285,11 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
       return rc;      \
     }                 \
   } while (0)
+  for (auto& ev : b->ev_sweep)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      free_batch(b);
+      return fail(SEISFAT_EHIP, "hipEventCreate");
+    }
   TRY(dalloc(&b->d_T, b->tsize));
   for (int k = 0; k < nshots; ++k) b->iso[k].T = b->d_T + b->toff[k];
   if (type == SEISFAT_ANI) {
@@ -308,24 +331,37 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
     }
   }
   if (use_graph && schedule == SEISFAT_SCHED_TILED) {
-    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) {
-      free_batch(b);
-      return fail(SEISFAT_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    hipStream_t st = h->stream;
+    int part = 0;
+    auto end_part = [&]() -> int {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamEndCapture(st, &g));
+      b->graph[part] = g;
+      HIPCHK(hipGraphInstantiate(&b->gexec[part], g, nullptr, nullptr, 0));
+      ++part;
+      return SEISFAT_OK;
+    };
+    auto begin_part = [&]() -> int {
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      return SEISFAT_OK;
+    };
+    rc = begin_part();
+    if (!rc) {
+      rc = enqueue_solve(b, st, [&](int) -> int {
+        int r = end_part();
+        return r ? r : begin_part();
+      });
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      (void)hipStreamIsCapturing(st, &cs);
+      if (cs == hipStreamCaptureStatusActive) {
+        const int r = end_part();
+        if (!rc) rc = r;
+      }
     }
-    rc = enqueue_solve(b, h->stream);
-    hipGraph_t g = nullptr;
-    e = hipStreamEndCapture(h->stream, &g);
-    if (rc || e != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
+    if (!rc && part != 3) rc = fail(SEISFAT_EHIP, "graph capture produced an unexpected number of parts");
+    if (rc) {
       free_batch(b);
-      return rc ? rc : fail(SEISFAT_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    }
-    b->graph = g;
-    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      free_batch(b);
-      return fail(SEISFAT_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      return rc;
     }
   }
   *out = b;
@@ -335,11 +371,18 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
 
 int run_batch(seisfat_batch_s* b, hipStream_t st) {
   if (!st) st = b->h->stream;
-  if (b->gexec) {
-    HIPCHK(hipGraphLaunch(b->gexec, st));
+  if (b->gexec[0]) {
+    HIPCHK(hipGraphLaunch(b->gexec[0], st));
+    HIPCHK(hipEventRecord(b->ev_sweep[0], st));
+    HIPCHK(hipGraphLaunch(b->gexec[1], st));
+    HIPCHK(hipEventRecord(b->ev_sweep[1], st));
+    HIPCHK(hipGraphLaunch(b->gexec[2], st));
     return SEISFAT_OK;
   }
-  return enqueue_solve(b, st);
+  return enqueue_solve(b, st, [&](int i) -> int {
+    HIPCHK(hipEventRecord(b->ev_sweep[i], st));
+    return SEISFAT_OK;
+  });
 }
 
 }  // namespace
@@ -543,6 +586,15 @@ int seisfat_batch_copy_T(seisfat_batch_t b, int shot, float* T_padded) {
 }
 
 int seisfat_batch_num_launches(seisfat_batch_t b) { return b ? b->launches : -1; }
+
+int seisfat_batch_num_sweep_launches(seisfat_batch_t b) { return b ? b->sweep_launches : -1; }
+
+int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms) {
+  if (!b || !ms) return fail(SEISFAT_EINVAL, "null batch or output");
+  HIPCHK(hipEventSynchronize(b->ev_sweep[1]));
+  HIPCHK(hipEventElapsedTime(ms, b->ev_sweep[0], b->ev_sweep[1]));
+  return SEISFAT_OK;
+}
 
 int seisfat_batch_destroy(seisfat_batch_t b) {
   if (!b) return SEISFAT_OK;

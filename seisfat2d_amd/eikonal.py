@@ -135,6 +135,16 @@ class Batch:
     def num_launches(self):
         return lib().seisfat_batch_num_launches(self._b)
 
+    @property
+    def num_sweep_launches(self):
+        return lib().seisfat_batch_num_sweep_launches(self._b)
+
+    def sweep_ms(self):
+        """Device time of the sweep phase of the last completed run (HIP events)."""
+        ms = C.c_float()
+        check(lib().seisfat_batch_sweep_ms(self._b, C.byref(ms)))
+        return ms.value
+
     def close(self):
         if getattr(self, "_b", None):
             lib().seisfat_batch_destroy(self._b)

@@ -1,0 +1,242 @@
+"""GPU parity of the eikonal path against the CPU oracle.
+
+The contract is bit-exactness: both schedules evaluate the reference's local
+solver with the same correctly rounded FP32 operations in a dependency-
+equivalent order, so traveltimes and picks must equal the oracle's bit for
+bit (tolerance 0).  Full-size workloads are compared node by node where the
+oracle finishes in seconds (C2 single shot, every C3 window with the
+shot-parallel oracle) and through properties otherwise.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy.ndimage import gaussian_filter
+
+from seisfat2d_amd import ANI, ISO, SCHED_PLAIN, SCHED_TILED, Eikonal, capi, models
+
+pytestmark = pytest.mark.gpu
+NB = 3
+
+
+def smooth_slowness(rng, nz, nx, dh, sigma=3.0):
+    v = 2000 + 500 * gaussian_filter(rng.standard_normal((nx, nz)), sigma) + 1.0 * np.arange(nz)[None, :] * dh
+    return (1.0 / v).astype(np.float32)
+
+
+def solve_both(oracle, s, dx, dz, sx, sz, rx=None, rz=None):
+    nx, nz = s.shape
+    e = Eikonal(nz, nx, dz, dx)
+    e.set_slowness(s)
+    Tref = oracle.solve_iso(oracle.expand_boundary(s), dx, dz, sx, sz)
+    out = {}
+    for sched in (SCHED_TILED, SCHED_PLAIN):
+        out[sched] = e.solve(sx, sz, schedule=sched, rx=rx, rz=rz)
+    e.close()
+    return Tref, out
+
+
+def test_c1_bit_exact_both_schedules(oracle):
+    w = models.c1()
+    Tref, out = solve_both(oracle, w.s, w.dx, w.dz, 5000.0, 0.0, w.rx, w.rz)
+    pref = oracle.picks(Tref, w.dx, w.dz, w.rx, w.rz)
+    for sched, (T, pk) in out.items():
+        assert np.array_equal(T, Tref), sched
+        assert np.array_equal(pk, pref), sched
+
+
+def test_c1_constant_velocity_vs_r_over_v(oracle):
+    w = models.c1(constant=True)
+    Tref, out = solve_both(oracle, w.s, w.dx, w.dz, 5000.0, 0.0)
+    T = out[SCHED_TILED][0]
+    assert np.array_equal(T, Tref)
+    x = (np.arange(w.nx + 6) - NB) * w.dx
+    z = (np.arange(w.nz + 6) - NB) * w.dz
+    r = np.hypot(x[:, None] - 5000.0, z[None, :])[NB:-NB, NB:-NB]
+    m = r > 0
+    rel = np.abs(T[NB:-NB, NB:-NB][m] - r[m] / 2000.0) / (r[m] / 2000.0)
+    print(f"C1 constant velocity: max rel err vs r/v = {rel.max():.4f}, mean {rel.mean():.5f}")
+    assert rel.max() < 0.03
+
+
+@pytest.mark.parametrize("nz,nx,dz,dx,sxf,szf", [
+    (2, 2, 10.0, 10.0, 0.0, 0.0),
+    (3, 97, 5.0, 5.0, 1.0, 1.0),
+    (59, 3, 5.0, 5.0, 0.5, 0.0),
+    (64, 64, 10.0, 10.0, 0.3, 0.7),
+    (65, 130, 10.0, 8.0, 0.0, 1.0),
+    (123, 66, 7.0, 10.0, 1.0, 0.0),
+    (130, 200, 4.0, 4.0, 0.37, 0.61),
+    (200, 129, 10.0, 10.0, 0.999, 0.5),
+])
+def test_shapes_sources_and_spacings(oracle, rng, nz, nx, dz, dx, sxf, szf):
+    s = smooth_slowness(rng, nz, nx, min(dx, dz))
+    sx, sz = sxf * (nx - 1) * dx, szf * (nz - 1) * dz
+    rx = rng.uniform(0, (nx - 1) * dx, 17).astype(np.float32)
+    rz = rng.uniform(0, (nz - 1) * dz, 17).astype(np.float32)
+    Tref, out = solve_both(oracle, s, dx, dz, sx, sz, rx, rz)
+    pref = oracle.picks(Tref, dx, dz, rx, rz)
+    for sched, (T, pk) in out.items():
+        assert np.array_equal(T, Tref), (sched, nz, nx)
+        assert np.array_equal(pk, pref), (sched, nz, nx)
+
+
+def test_c2_full_size_bit_exact(oracle):
+    w = models.c2()
+    Tref, out = solve_both(oracle, w.s, w.dx, w.dz, 10000.0, 0.0)
+    for sched, (T, _) in out.items():
+        assert np.array_equal(T, Tref), sched
+
+
+def test_c3_all_256_windows_bit_exact(oracle):
+    w = models.c3()
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    b = e.batch(w.shots, w.rx, w.rz)
+    b.run()
+    picks = b.picks()
+    b.run()  # graph replay must reproduce the same bits
+    picks2 = b.picks()
+    assert np.array_equal(picks, picks2)
+    sx = [s[0] for s in w.shots]
+    x0 = [s[2] for s in w.shots]
+    wd = [s[3] for s in w.shots]
+    chunk = 32
+    for k0 in range(0, len(w.shots), chunk):
+        ks = range(k0, min(k0 + chunk, len(w.shots)))
+        Trefs = oracle.solve_windows(w.s, w.dx, w.dz, [sx[k] for k in ks], [0.0] * len(ks),
+                                     [x0[k] for k in ks], [wd[k] for k in ks], nthreads=16)
+        for k, Tr in zip(ks, Trefs):
+            assert np.array_equal(b.T(k), Tr), k
+            _, _, xk, _, ro, nr = w.shots[k]
+            pref = oracle.picks(Tr, w.dx, w.dz, w.rx[ro:ro + nr] - np.float32(xk * w.dx), w.rz[ro:ro + nr])
+            assert np.array_equal(picks[ro:ro + nr], pref), k
+    b.close()
+    e.close()
+
+
+def ani_setup(nz, nx, dh, nshots, seed=7):
+    from seisfat2d_amd.stiffness import compute_stiffness
+    rng = np.random.default_rng(seed)
+    vp = models.layered_model(nz, nx, dh, seed=2, nlayers=6)
+
+    def field(lo, hi):
+        g = gaussian_filter(rng.standard_normal((nx, nz)), 10.0)
+        return lo + (hi - lo) * (g - g.min()) / (g.max() - g.min())
+
+    cs = compute_stiffness(vp, vp / np.sqrt(3), 1000.0, field(0.0, 0.3), field(-0.1, 0.2), field(-30.0, 30.0))
+    s = np.sqrt(1000.0 / cs[3]).astype(np.float32)
+    cs = [c.astype(np.float32) for c in cs]
+    sxs = (np.arange(nshots) + 0.5) * (nx - 1) * dh / nshots
+    return s, cs, sxs
+
+
+def oracle_ani(oracle, s, cs, dh, sx, sz):
+    S = oracle.expand_boundary(s)
+    Cq, mn, mx = [], [], []
+    for c in cs:
+        u, a, b = oracle.compress(oracle.expand_boundary(c))
+        Cq.append(u)
+        mn.append(a)
+        mx.append(b)
+    return oracle.solve_ani(S, dh, dh, sx, sz, Cq, mn, mx)[0]
+
+
+def test_ani_batch_bit_exact(oracle):
+    nz, nx, dh = 161, 257, 5.0
+    s, cs, sxs = ani_setup(nz, nx, dh, 5)
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    e.set_stiffness(cs)
+    rx = np.arange(0, nx, 7, dtype=np.float32) * dh
+    shots = [(float(x), 0.0, 0, nx, 0, 0) for x in sxs]
+    shots[2] = (float(sxs[2]), 0.0, 0, nx, 0, len(rx))
+    for sched in (SCHED_TILED, SCHED_PLAIN):
+        b = e.batch(shots, rx, np.zeros_like(rx), type=ANI, schedule=sched)
+        b.run()
+        pk = b.picks()
+        for k, x in enumerate(sxs):
+            Tr = oracle_ani(oracle, s, cs, dh, float(x), 0.0)
+            assert np.array_equal(b.T(k), Tr), (sched, k)
+            if k == 2:
+                assert np.array_equal(pk, oracle.picks(Tr, dh, dh, rx, np.zeros_like(rx)))
+        b.close()
+    T1, _ = e.solve(float(sxs[1]), 0.0, type=ANI)
+    assert np.array_equal(T1, oracle_ani(oracle, s, cs, dh, float(sxs[1]), 0.0))
+    e.close()
+
+
+def test_ani_full_size_c4_shot(oracle):
+    w = models.c4(nshots=2)
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    e.set_stiffness(w.stiffness)
+    sx = w.shots[1][0]
+    T, _ = e.solve(sx, 0.0, type=ANI)
+    assert np.array_equal(T, oracle_ani(oracle, w.s, w.stiffness, w.dx, sx, 0.0))
+    e.close()
+
+
+def test_invalid_inputs_rejected():
+    e = Eikonal(20, 30, 10.0, 10.0)
+    s = np.full((30, 20), 5e-4, np.float32)
+    s[3, 4] = 0.0
+    with pytest.raises(capi.SeisfatError) as ex:
+        e.set_slowness(s)
+    assert ex.value.code == capi.EINVAL
+    s[3, 4] = np.nan
+    with pytest.raises(capi.SeisfatError):
+        e.set_slowness(s)
+    with pytest.raises(capi.SeisfatError) as ex:  # no valid slowness yet
+        e.solve(10.0, 10.0)
+    assert ex.value.code == capi.ESTATE
+    e.set_slowness(np.full((30, 20), 5e-4, np.float32))
+    for sx, sz in ((-1.0, 0.0), (290.5, 0.0), (10.0, 191.0)):
+        with pytest.raises(capi.SeisfatError) as ex:
+            e.solve(sx, sz)
+        assert ex.value.code == capi.EINVAL
+    with pytest.raises(capi.SeisfatError) as ex:
+        e.solve(10.0, 10.0, rx=[400.0], rz=[0.0])
+    assert ex.value.code == capi.EINVAL
+    with pytest.raises(capi.SeisfatError) as ex:
+        e.solve(10.0, 10.0, type=ANI)
+    assert ex.value.code == capi.ESTATE
+    with pytest.raises(capi.SeisfatError) as ex:  # window outside the model
+        e.batch([(10.0, 0.0, 25, 10, 0, 0)])
+    assert ex.value.code == capi.EINVAL
+    with pytest.raises(capi.SeisfatError) as ex:  # source outside its window
+        e.batch([(10.0, 0.0, 5, 10, 0, 0)])
+    assert ex.value.code == capi.EINVAL
+    e.close()
+
+
+def test_device_slowness_from_torch(oracle, rng):
+    import torch
+    nz, nx, dh = 70, 90, 10.0
+    s = smooth_slowness(rng, nz, nx, dh)
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(torch.from_numpy(s).cuda())
+    torch.cuda.synchronize()
+    T, _ = e.solve(200.0, 50.0)
+    assert np.array_equal(T, oracle.solve_iso(oracle.expand_boundary(s), dh, dh, 200.0, 50.0))
+    bad = torch.from_numpy(s).cuda()
+    bad[5, 5] = -1.0
+    with pytest.raises(capi.SeisfatError):
+        e.set_slowness(bad)
+    e.close()
+
+
+def test_device_T_pointer_matches_copy():
+    nz, nx, dh = 40, 50, 10.0
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(np.full((nx, nz), 4e-4, np.float32))
+    T, _ = e.solve(100.0, 0.0)
+    p = e.device_T_ptr()
+    assert p
+    import torch
+    buf = torch.empty(T.size, dtype=torch.float32, device="cuda")
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    assert hip.hipMemcpy(C.c_void_p(buf.data_ptr()), C.c_void_p(p), T.nbytes, 3) == 0  # device to device
+    assert np.array_equal(buf.cpu().numpy().reshape(T.shape), T)
+    e.close()
