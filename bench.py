@@ -157,7 +157,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched iso FSM, {len(w.shots)} shots on windows 1200-4001 wide of the "
                                f"1001x4001 dh=5 m layered model (seeded), one pass of 4 sweeps, tiled "
-                               f"schedule, one HIP graph per step",
+                               f"schedule (persistent sweep kernels), one HIP graph per step",
                    "nz": w.nz, "nx": w.nx, "shots": len(w.shots), "cell_updates_per_step": cu},
         "shots_per_s": shots_per_s,
         "per_shot_plain_shots_per_s": plain_shots_per_s,
@@ -167,7 +167,7 @@ def main():
                                     cpu["gpu_windows_bit_exact"].split("/")[1]) else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_sweep_tiles", "launches": nlaunch,
+                     "kernel": "k_sweep_persist", "launches": nlaunch,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": avg_launch_s * 1e6},
         "cpu_baseline": cpu,

@@ -37,8 +37,9 @@ extern "C" {
 
 /* Sweep schedule of a solve.  Both compute the reference's single pass of four
  * sweeps (src/modeling/modeling.cu:123-159) and give bit-identical traveltimes. */
-#define SEISFAT_SCHED_PLAIN 0 /* one launch per anti-diagonal: the reference's schedule */
-#define SEISFAT_SCHED_TILED 1 /* 64x64 skewed LDS tiles, one launch per tile diagonal */
+#define SEISFAT_SCHED_PLAIN 0      /* one launch per anti-diagonal: the reference's schedule */
+#define SEISFAT_SCHED_TILED 1      /* 64x64 skewed LDS tiles, one persistent launch per sweep */
+#define SEISFAT_SCHED_TILED_DIAG 2 /* the same tiles, one launch per tile diagonal */
 
 typedef struct seisfat_eik_s* seisfat_eik_t;
 typedef struct seisfat_batch_s* seisfat_batch_t;
@@ -116,6 +117,9 @@ int seisfat_batch_num_sweep_launches(seisfat_batch_t b);
 /* Device time in ms of the sweep phase of the last completed run (from HIP
  * events recorded on the run's stream around the sweep launches). */
 int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms);
+/* Wait for the batch and report a failure the device flagged during its runs
+ * (a persistent sweep whose tile dependencies did not arrive in time). */
+int seisfat_batch_status(seisfat_batch_t b);
 int seisfat_batch_destroy(seisfat_batch_t b);
 
 #ifdef __cplusplus

@@ -14,7 +14,7 @@ from .build import EIK_LIB
 
 OK, EINVAL, EHIP, ENOMEM, ESTATE, EIO = 0, -1, -2, -3, -4, -5
 ISO, ANI = 0, 1
-SCHED_PLAIN, SCHED_TILED = 0, 1
+SCHED_PLAIN, SCHED_TILED, SCHED_TILED_DIAG = 0, 1, 2
 
 # Every function the header declares (checked by the CPU test suite).
 EXPORTS = [
@@ -23,7 +23,7 @@ EXPORTS = [
     "seisfat_eik_device_T", "seisfat_batch_create", "seisfat_batch_run", "seisfat_batch_picks",
     "seisfat_batch_device_picks", "seisfat_batch_device_T", "seisfat_batch_copy_T",
     "seisfat_batch_num_launches", "seisfat_batch_num_sweep_launches", "seisfat_batch_sweep_ms",
-    "seisfat_batch_destroy",
+    "seisfat_batch_status", "seisfat_batch_destroy",
 ]
 
 
@@ -71,6 +71,7 @@ def lib():
         L.seisfat_batch_num_sweep_launches.argtypes = [vp]
         L.seisfat_batch_sweep_ms.argtypes = [vp, C.POINTER(f)]
         L.seisfat_batch_destroy.argtypes = [vp]
+        L.seisfat_batch_status.argtypes = [vp]
         ptr_ret = {"seisfat_last_error", "seisfat_eik_device_T", "seisfat_batch_device_picks",
                    "seisfat_batch_device_T"}
         for name in EXPORTS:

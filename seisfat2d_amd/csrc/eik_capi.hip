@@ -72,6 +72,10 @@ struct seisfat_eik_s {
   float cmin[6] = {}, cmax[6] = {};
   bool has_c = false;
   int* d_flag = nullptr;
+  // skewed slowness copies, one per sweep orientation (tiled schedule)
+  float* d_sk[4] = {};
+  int KP = 0, NS = 0;
+  SkewView skew[4] = {};
   seisfat_batch_s* single = nullptr;  // batch of the last seisfat_eik_solve
 };
 
@@ -92,7 +96,13 @@ struct seisfat_batch_s {
   int* d_recshot = nullptr;
   float* d_picks = nullptr;
   int4* d_tiles = nullptr;
+  int ntiles = 0, nstrips = 0;
   std::vector<int> diag_off;
+  // persistent schedule state
+  unsigned long long* d_xch = nullptr;  // exchange words of every shot
+  size_t xch_words = 0;
+  int* d_ctl = nullptr;  // [0..7] ticket counters, [8] epoch, [9] error flag
+  PersistRun pr{};
   int launches = 0, sweep_launches = 0;
   hipEvent_t ev_sweep[2] = {nullptr, nullptr};
   // The tiled solve is recorded as three graphs: initialisation, sweeps (for
@@ -138,6 +148,8 @@ void free_batch(seisfat_batch_s* b) {
   dfree(b->d_recshot);
   dfree(b->d_picks);
   dfree(b->d_tiles);
+  dfree(b->d_xch);
+  dfree(b->d_ctl);
   delete b;
 }
 
@@ -148,24 +160,37 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
   seisfat_eik_s* h = b->h;
   const int nzz = h->nzz;
   int launches = 0, sweeps = 0;
-  auto one_solve = [&](ShotDev* d_tab, const std::vector<ShotDev>& tab) -> int {
-    launch_fill(b->d_T, b->tsize, TINF, st);
-    launch_init(d_tab, b->nshots, nzz, h->dx, h->dz, st);
-    launches += 2;
+  int solve = 0;
+  auto one_solve = [&](ShotDev* d_tab, const std::vector<ShotDev>& tab, const SkewView* skew) -> int {
+    if (b->schedule == SEISFAT_SCHED_PLAIN) {
+      // time_set + time_init as separate launches, as the reference does; the
+      // tiled sweeps generate the initial field inside their first sweep.
+      launch_fill(b->d_T, b->tsize, TINF, st);
+      launch_init(d_tab, b->nshots, nzz, h->dx, h->dz, st);
+      launches += 2;
+    }
     if (sweeps == 0)
       if (int rc = mark(0)) return rc;
     int n = 0;
     if (b->schedule == SEISFAT_SCHED_TILED) {
-      n = launch_sweeps_tiled(d_tab, b->d_tiles, b->diag_off, nzz, h->kc, h->eqh, st);
+      n = launch_sweeps_persist(d_tab, b->d_tiles, b->ntiles, b->nstrips, nzz, h->kc, h->eqh, skew, true, b->pr,
+                                solve, st);
+    } else if (b->schedule == SEISFAT_SCHED_TILED_DIAG) {
+      n = launch_sweeps_tiled(d_tab, b->d_tiles, b->diag_off, nzz, h->kc, h->eqh, skew, true, st);
     } else {
       for (int k = 0; k < b->nshots; ++k) n += launch_sweeps_plain(tab[k], nzz, h->kc, h->eqh, st);
     }
     launches += n;
     sweeps += n;
+    ++solve;
     HIPCHK(hipGetLastError());
     return SEISFAT_OK;
   };
-  int rc = one_solve(b->d_iso, b->iso);
+  if (b->schedule == SEISFAT_SCHED_TILED) {
+    launch_begin_run(b->pr, st);
+    ++launches;
+  }
+  int rc = one_solve(b->d_iso, b->iso, h->skew);
   if (rc) return rc;
   if (b->type == SEISFAT_ANI) {
     StiffDev sf;
@@ -176,7 +201,7 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
     }
     launch_quasi_slowness(b->d_iso, b->d_qp, b->nshots, nzz, h->nxx, h->dx, h->dz, sf, st);
     ++launches;
-    rc = one_solve(b->d_qp, b->qp);
+    rc = one_solve(b->d_qp, b->qp, nullptr);  // per-shot qP slowness: staged in LDS
     if (rc) return rc;
   }
   if ((rc = mark(1))) return rc;
@@ -217,8 +242,8 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
                  int nrec_total, const float* rx, const float* rz, bool use_graph, seisfat_batch_s** out) {
   *out = nullptr;
   if (type != SEISFAT_ISO && type != SEISFAT_ANI) return fail(SEISFAT_EINVAL, "type must be ISO (0) or ANI (1)");
-  if (schedule != SEISFAT_SCHED_PLAIN && schedule != SEISFAT_SCHED_TILED)
-    return fail(SEISFAT_EINVAL, "schedule must be PLAIN (0) or TILED (1)");
+  if (schedule != SEISFAT_SCHED_PLAIN && schedule != SEISFAT_SCHED_TILED && schedule != SEISFAT_SCHED_TILED_DIAG)
+    return fail(SEISFAT_EINVAL, "schedule must be PLAIN (0), TILED (1) or TILED_DIAG (2)");
   if (!h->has_s) return fail(SEISFAT_ESTATE, "slowness not set (seisfat_eik_set_slowness)");
   if (type == SEISFAT_ANI && !h->has_c) return fail(SEISFAT_ESTATE, "stiffness not set (seisfat_eik_set_stiffness)");
   if (nshots <= 0 || !shots) return fail(SEISFAT_EINVAL, "nshots must be positive");
@@ -273,10 +298,20 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
     sd.smax = s.width - 1;
     sd.sx = s.sx - (float)s.x0 * dx;
     sd.sz = s.sz;
+    sd.x0 = s.x0;
+    sd.w = s.width;
     off += (size_t)nzz * sd.nxx;
   }
   b->tsize = off;
   int rc;
+#define HIPCHK_B(expr)                                                            \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      free_batch(b);                                                              \
+      return fail(SEISFAT_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    }                                                                             \
+  } while (0)
 #define TRY(x)        \
   do {                \
     rc = (x);         \
@@ -294,7 +329,7 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   for (int k = 0; k < nshots; ++k) b->iso[k].T = b->d_T + b->toff[k];
   if (type == SEISFAT_ANI) {
     TRY(dalloc(&b->d_swork, b->tsize));
-    b->qp = b->iso;
+    b->qp = b->iso;  // exchange pointers are copied below once assigned
     for (int k = 0; k < nshots; ++k) {
       b->qp[k].sbase = b->d_swork + b->toff[k];
       b->qp[k].sofs = 0;
@@ -308,9 +343,42 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   TRY(dalloc(&b->d_recshot, nrec_total));
   TRY(dalloc(&b->d_picks, nrec_total));
   std::vector<int4> tiles;
-  if (schedule == SEISFAT_SCHED_TILED) {
+  const bool tiled = schedule != SEISFAT_SCHED_PLAIN;
+  if (tiled) {
     build_tiles(b, tiles);
+    b->ntiles = (int)tiles.size();
+    b->nstrips = (nzz - 2 + 63) / 64;
     TRY(dalloc(&b->d_tiles, tiles.size()));
+  }
+  if (schedule == SEISFAT_SCHED_TILED) {
+    // exchange words per shot: bottom row of each strip + double-buffered carries
+    std::vector<size_t> xoff(nshots);
+    size_t words = 0;
+    for (int k = 0; k < nshots; ++k) {
+      xoff[k] = words;
+      const int ldh = ((b->iso[k].nxx - 2) + 63) / 64 * 64;
+      words += (size_t)b->nstrips * ldh + (size_t)b->nstrips * 2 * 128;
+    }
+    b->xch_words = words;
+    TRY(dalloc(&b->d_xch, words));
+    TRY(dalloc(&b->d_ctl, 16));
+    for (int k = 0; k < nshots; ++k) {
+      const int ldh = ((b->iso[k].nxx - 2) + 63) / 64 * 64;
+      b->iso[k].xh = b->d_xch + xoff[k];
+      b->iso[k].xc = b->d_xch + xoff[k] + (size_t)b->nstrips * ldh;
+      b->iso[k].ldh = ldh;
+    }
+    for (int k = 0; k < (int)b->qp.size(); ++k) {
+      b->qp[k].xh = b->iso[k].xh;
+      b->qp[k].xc = b->iso[k].xc;
+      b->qp[k].ldh = b->iso[k].ldh;
+    }
+    b->pr.counters = b->d_ctl;
+    b->pr.epoch = (unsigned*)(b->d_ctl + 8);
+    b->pr.err = b->d_ctl + 9;
+    // all-ones words carry a tag no run ever uses; epoch and flags start at 0
+    HIPCHK_B(hipMemsetAsync(b->d_xch, 0xff, words * sizeof(unsigned long long), h->stream));
+    HIPCHK_B(hipMemsetAsync(b->d_ctl, 0, 16 * sizeof(int), h->stream));
   }
   auto up = [&](void* dst, const void* src, size_t bytes) -> int {
     if (bytes == 0) return SEISFAT_OK;
@@ -323,6 +391,9 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   TRY(up(b->d_rz, rzr.data(), nrec_total * sizeof(float)));
   TRY(up(b->d_recshot, recshot.data(), nrec_total * sizeof(int)));
   TRY(up(b->d_tiles, tiles.data(), tiles.size() * sizeof(int4)));
+  // The boundary ring of every grid holds 1e6 for good (sweeps never write
+  // it); the tiled schedule relies on this one fill instead of a per-run one.
+  launch_fill(b->d_T, b->tsize, TINF, h->stream);
   {
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
@@ -330,7 +401,7 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
       return fail(SEISFAT_EHIP, std::string("upload: ") + hipGetErrorString(e));
     }
   }
-  if (use_graph && schedule == SEISFAT_SCHED_TILED) {
+  if (use_graph && schedule != SEISFAT_SCHED_PLAIN) {
     hipStream_t st = h->stream;
     int part = 0;
     auto end_part = [&]() -> int {
@@ -367,6 +438,19 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   *out = b;
   return SEISFAT_OK;
 #undef TRY
+#undef HIPCHK_B
+}
+
+// Synchronise with the batch's work and turn a device-side failure flag into
+// an error.
+int batch_status(seisfat_batch_s* b) {
+  HIPCHK(hipDeviceSynchronize());
+  if (b->d_ctl) {
+    int err = 0;
+    HIPCHK(hipMemcpy(&err, b->d_ctl + 9, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(SEISFAT_EHIP, "persistent sweep: a tile dependency did not arrive within the spin limit");
+  }
+  return SEISFAT_OK;
 }
 
 int run_batch(seisfat_batch_s* b, hipStream_t st) {
@@ -415,9 +499,18 @@ int seisfat_eik_create(int nz, int nx, float dz, float dx, seisfat_eik_t* out) {
     delete h;
     return fail(SEISFAT_EHIP, std::string("device/stream: ") + hipGetErrorString(e));
   }
+  const int NRS = (h->nzz - 2 + 63) / 64;
+  h->KP = 64 * NRS + 64;
+  h->NS = nx + 2 * SK_OFF;
   int rc = dalloc(&h->d_szp, (size_t)h->nzz * nx);
   if (!rc) rc = dalloc(&h->d_flag, 1);
+  for (int o = 0; o < 4 && !rc; ++o) {
+    rc = dalloc(&h->d_sk[o], (size_t)h->NS * h->KP);
+    h->skew[o] = SkewView{h->d_sk[o], h->KP};
+  }
   if (rc) {
+    for (auto& p : h->d_sk) dfree(p);
+    dfree(h->d_flag);
     dfree(h->d_szp);
     (void)hipStreamDestroy(h->stream);
     delete h;
@@ -433,6 +526,7 @@ int seisfat_eik_destroy(seisfat_eik_t h) {
   free_batch(h->single);
   dfree(h->d_szp);
   dfree(h->d_flag);
+  for (auto& p : h->d_sk) dfree(p);
   for (auto& c : h->d_c) dfree(c);
   (void)hipStreamDestroy(h->stream);
   delete h;
@@ -446,6 +540,7 @@ int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* str
   HIPCHK(hipMemsetAsync(h->d_flag, 0, sizeof(int), st));
   launch_check_positive(d_s, (size_t)h->nz * h->nx, h->d_flag, st);
   launch_zpad(d_s, h->d_szp, h->nz, h->nx, st);
+  for (int o = 0; o < 4; ++o) launch_skew_s(h->d_szp, h->d_sk[o], h->nzz, h->nx, h->KP, h->NS, o, st);
   HIPCHK(hipGetLastError());
   int bad = 0;
   HIPCHK(hipMemcpyAsync(&bad, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -530,7 +625,7 @@ int seisfat_eik_solve(seisfat_eik_t h, int type, int schedule, float sx, float s
   if (rc) return rc;
   h->single = b;
   if ((rc = run_batch(b, h->stream))) return rc;
-  HIPCHK(hipStreamSynchronize(h->stream));
+  if ((rc = batch_status(b))) return rc;
   if (T_padded) HIPCHK(hipMemcpy(T_padded, b->d_T, b->tsize * sizeof(float), hipMemcpyDeviceToHost));
   if (nrec > 0) HIPCHK(hipMemcpy(picks, b->d_picks, nrec * sizeof(float), hipMemcpyDeviceToHost));
   return SEISFAT_OK;
@@ -560,7 +655,7 @@ int seisfat_batch_run(seisfat_batch_t b, void* stream) {
 int seisfat_batch_picks(seisfat_batch_t b, float* picks) {
   if (!b) return fail(SEISFAT_EINVAL, "null batch");
   if (b->nrec > 0 && !picks) return fail(SEISFAT_EINVAL, "null picks buffer");
-  HIPCHK(hipDeviceSynchronize());
+  if (int rc = batch_status(b)) return rc;
   if (b->nrec > 0) HIPCHK(hipMemcpy(picks, b->d_picks, b->nrec * sizeof(float), hipMemcpyDeviceToHost));
   return SEISFAT_OK;
 }
@@ -579,7 +674,7 @@ const float* seisfat_batch_device_T(seisfat_batch_t b, int shot, int* nzz, int* 
 
 int seisfat_batch_copy_T(seisfat_batch_t b, int shot, float* T_padded) {
   if (!b || shot < 0 || shot >= b->nshots || !T_padded) return fail(SEISFAT_EINVAL, "bad batch, shot or buffer");
-  HIPCHK(hipDeviceSynchronize());
+  if (int rc = batch_status(b)) return rc;
   const size_t n = (size_t)b->h->nzz * b->iso[shot].nxx;
   HIPCHK(hipMemcpy(T_padded, b->d_T + b->toff[shot], n * sizeof(float), hipMemcpyDeviceToHost));
   return SEISFAT_OK;
@@ -594,6 +689,11 @@ int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms) {
   HIPCHK(hipEventSynchronize(b->ev_sweep[1]));
   HIPCHK(hipEventElapsedTime(ms, b->ev_sweep[0], b->ev_sweep[1]));
   return SEISFAT_OK;
+}
+
+int seisfat_batch_status(seisfat_batch_t b) {
+  if (!b) return fail(SEISFAT_EINVAL, "null batch");
+  return batch_status(b);
 }
 
 int seisfat_batch_destroy(seisfat_batch_t b) {

@@ -38,6 +38,24 @@ struct ShotDev {
   int sofs, smax;
   int nxx;
   float sx, sz;  // source position relative to the shot's grid origin
+  int x0, w;     // model columns of the shot's window (skewed slowness addressing)
+  // persistent schedule: tagged exchange words of this shot
+  unsigned long long* xh;  // strip r's bottom row at [r * ldh + c]
+  unsigned long long* xc;  // strip r's carries at [(2r + block parity) * 128 + 2*lane + {0,1}]
+  int ldh;
+};
+
+// Skewed copy of the row-padded model slowness for one sweep orientation
+// (SA, SB).  In sweep-local cell coordinates (kap = cell row along the sweep's
+// z direction, model column m), cell (kap, m) is stored at
+//   base[(sig + SK_OFF) * KP + kap],  sig = m + (kap & 63)       for SB > 0
+//                                     sig = m - (kap & 63) + 63  for SB < 0
+// so the 64 cells a tile's lanes need at one step share one sig and sit in
+// consecutive words.
+constexpr int SK_OFF = 160;  // column margin: tile reach (66) + prefetch (8) + slack
+struct SkewView {
+  const float* base;
+  int KP;  // rows per skewed column (>= 64 * strips + 64)
 };
 
 // Pointers held in ShotDev are read from memory, so the compiler cannot see

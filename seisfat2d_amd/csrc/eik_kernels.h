@@ -23,8 +23,23 @@ void launch_check_positive(const float* s, size_t n, int* bad, hipStream_t st);
 void launch_init(const ShotDev* shots, int nshots, int nzz, float dx, float dz, hipStream_t st);
 // Returns the number of kernel launches issued.
 int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh, hipStream_t st);
+// skew: four skewed slowness views (one per sweep) or nullptr for slowness
+// staged in LDS from ShotDev's natural view; first_fused: sweep 0 generates
+// the initial traveltimes (no fill/init launches needed before it).
 int launch_sweeps_tiled(const ShotDev* shots, const int4* tiles, const std::vector<int>& diag_off, int nzz,
-                        const EikConsts& k, bool eqh, hipStream_t st);
+                        const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused, hipStream_t st);
+// Device state of the persistent schedule: 8 ticket counters (two solves x
+// four sweeps), the run epoch and the spin-limit error flag.
+struct PersistRun {
+  int* counters;
+  unsigned* epoch;
+  int* err;
+};
+int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, int nstrips, int nzz,
+                          const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused,
+                          const PersistRun& pr, int solve, hipStream_t st);
+void launch_begin_run(const PersistRun& pr, hipStream_t st);
+void launch_skew_s(const float* szp, float* sk, int nzz, int nx, int KP, int NS, int sweep, hipStream_t st);
 void launch_picks(const ShotDev* shots, const int* rec_shot, const float* rx, const float* rz, float* out,
                   int nrec, int nzz, float dx, float dz, hipStream_t st);
 void launch_quasi_slowness(const ShotDev* iso, ShotDev* qp, int nshots, int nzz, int nxx, float dx, float dz,

@@ -131,89 +131,348 @@ __device__ __forceinline__ float dpp_shr1(float old, float src) {
                                                     0xf, 0xf, false));
 }
 
-constexpr int RING_LD = 66;
+constexpr int RING_LD = 66;  // LDS row stride (words): 64 lanes + slowness row 64 + pad
+constexpr int PF = 8;        // slowness prefetch distance (steps) on the skewed path
 
-template <int SA, int SB, bool EQH>
-__global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ shots,
-                                                    const int4* __restrict__ tiles, int nzz,
-                                                    EikConsts k) {
-  __shared__ float sT[64 * RING_LD];
-  __shared__ float sS[64 * RING_LD];
-  __shared__ float sH[64];
+template <int SA>
+__device__ __forceinline__ int prow(int kk, int nzz) {
+  return (SA > 0) ? 1 + kk : nzz - 2 - kk;
+}
+template <int SB>
+__device__ __forceinline__ int pcol(int cc, int nxx) {
+  return (SB > 0) ? 1 + cc : nxx - 2 - cc;
+}
 
-  const int4 tl = tiles[blockIdx.x];
+// Address (in floats, relative to SkewView::base) of sweep-local cell
+// (kap, lam) of a shot in the skewed slowness copy; lam is clamped into the
+// shot's cell range and mapped through the window's replicated border.
+template <int SB>
+__device__ __forceinline__ int skew_addr(const ShotDev& sd, int nxx, int NC, int KP, int kap, int lam) {
+  lam = min(max(lam, 0), NC);
+  const int j = (SB > 0) ? lam : nxx - 2 - lam;
+  const int m = sd.x0 + min(max(j - NB, 0), sd.w - 1);
+  const int sig = (SB > 0) ? m + (kap & 63) : m - (kap & 63) + 63;
+  return (sig + SK_OFF) * KP + kap;
+}
+
+// One step of a tile: lane k updates sweep-local node (kg, c0 + t - k).  The
+// step's old traveltime, upwind-row value and slowness cells are loaded by the
+// caller ahead of time (software pipelining across the solver's branches).
+template <bool EQH>
+__device__ __forceinline__ void tile_step(float* __restrict__ sT, int t, int lane, bool row_ok, int c0, int NC,
+                                          float told, float h, float szo, float sxo, float& te, float& tev,
+                                          float& sref, const EikConsts& k) {
+  const float tv = dpp_shr1(h, te);
+  const int c = c0 + t - lane;
+  const bool act = row_ok && (unsigned)c < (unsigned)NC;
+  const float cand = eik_local<EQH>(tv, te, tev, sref, szo, sxo, k);
+  const float res = act ? fminf(told, cand) : told;
+  sT[t * RING_LD + lane] = res;
+  tev = tv;
+  te = res;
+  sref = szo;
+}
+
+// The 64 steps of a tile with slowness from the skewed global copy, loaded PF
+// steps ahead (loads past step 63 stay inside the copy's margin and are
+// unused).  FAST: the tile lies inside its window's unreplicated columns, so
+// at step t every lane's two cells share one skewed column and the addresses
+// advance by a uniform stride; otherwise each lane computes its clamped
+// address.
+template <int SB, bool EQH, bool FAST>
+__device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewView& sv, float* __restrict__ sT,
+                                                const float* __restrict__ sH, int lane, int kap, bool row_ok,
+                                                int c0, int nxx, int NC, float& te, float& tev, float& sref,
+                                                const EikConsts& k) {
+  const gfloat* __restrict__ SK = gptr(sv.base);
+  const int KP = sv.KP;
+  // fast path: z-other of step t at column sig0 + SB*t; lane 63's x-other cell
+  // (row kap+1 = next strip) sits one skew period (64 columns) away.
+  const int sig0 = (SB > 0) ? sd.x0 + c0 - 2 : sd.x0 + sd.w + 63 - c0;
+  const int colbase0 = (sig0 + SK_OFF) * KP;
+  const int offx = kap + 1 + ((lane == 63) ? -SB * 64 * KP : 0);
+  auto addr_z = [&](int t) -> int {
+    return FAST ? colbase0 + SB * t * KP + kap : skew_addr<SB>(sd, nxx, NC, KP, kap, c0 + t - lane + 1);
+  };
+  auto addr_x = [&](int t) -> int {
+    return FAST ? colbase0 + SB * t * KP + offx : skew_addr<SB>(sd, nxx, NC, KP, kap + 1, c0 + t - lane);
+  };
+  float zq[PF], xq[PF];
+#pragma unroll
+  for (int u = 0; u < PF; ++u) {
+    zq[u] = SK[addr_z(u)];
+    xq[u] = SK[addr_x(u)];
+  }
+  float told_n = sT[lane], h_n = sH[0];
+  for (int t0 = 0; t0 < 64; t0 += PF) {
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int t = t0 + u;
+      const float told = told_n, h = h_n;
+      told_n = sT[(t + 1) * RING_LD + lane];
+      h_n = sH[t + 1];
+      const float szo = zq[u], sxo = xq[u];
+      zq[u] = SK[addr_z(t + PF)];
+      xq[u] = SK[addr_x(t + PF)];
+      tile_step<EQH>(sT, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+    }
+  }
+}
+
+// Tag words of the persistent schedule's boundary exchange: value bits in the
+// low half, (epoch << 11 | sweep << 8 | block & 255) in the high half, written
+// and read as single 64-bit agent-scope atomics so a consumer that sees the
+// expected tag also sees the value (no fences needed).
+__device__ __forceinline__ unsigned long long xpack(float v, unsigned tag) {
+  return ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v);
+}
+__device__ __forceinline__ void xstore(unsigned long long* p, float v, unsigned tag) {
+  __hip_atomic_store(p, xpack(v, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+constexpr int SPIN_LIMIT = 1 << 16;  // ~0.1 s of polling (1000x a tile's run time); then flag and go on
+
+// Waits until every lane with `need` sees `tag` at *p; returns the value (1e6
+// for lanes without need).  The wait is bounded: past SPIN_LIMIT polls it sets
+// *err and returns whatever was read, so a broken dependency cannot hang the
+// device.
+__device__ __forceinline__ float xwait(const unsigned long long* p, bool need, unsigned tag, int* err) {
+  unsigned long long w = xpack(TINF, tag);
+  bool ok = !need;
+  for (int it = 0;; ++it) {
+    if (!ok) {
+      w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ok = (unsigned)(w >> 32) == tag;
+    }
+    if (__all(ok)) break;
+    if (it > SPIN_LIMIT) {
+      if (threadIdx.x == 0) atomicOr(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return need ? __uint_as_float((unsigned)w) : TINF;
+}
+
+// Control block of a persistent sweep launch.
+struct PersistCtl {
+  int* counter;            // tile ticket counter of this launch (zeroed by k_begin_run)
+  const unsigned* epoch;   // run counter (advanced by k_begin_run)
+  int* err;                // spin-limit flag
+  int ntiles;
+  int nstrips;             // strips per shot (NRS)
+  unsigned sweep_id;       // 0..7 within a run
+};
+
+// One 64x64 skewed tile of a sweep.
+//
+// In sweep-local coordinates (local row kk = 0..NR-1 along the sweep's z
+// direction, local column cc = 0..NC-1 along its x direction; NR = nzz-2,
+// NC = nxx-2 interior nodes) each node depends only on (kk-1,cc), (kk,cc-1)
+// and (kk-1,cc-1), and uses slowness cells (kk,cc), (kk,cc+1), (kk+1,cc).  A
+// tile is strip r (local rows 64r .. 64r+63, one per lane) by block bb: lane k
+// visits cc = 64*bb + t - k at step t = 0..63, so all 64 lanes stay busy and
+// lane k's z-upwind neighbour is the value lane k-1 produced one step earlier
+// (a wave_shr:1 DPP move).  Tile (r, bb) needs (r, bb-1) and (r-1, bb+1).
+//
+// LDS holds the tile's old traveltimes step-major (sT[t][k] = node visited by
+// lane k at step t); results overwrite them in place and are written back
+// column-pair by column-pair, so every global access touches at most two
+// contiguous 256-byte segments.  SKEW: slowness comes from the skewed global
+// copy (one coalesced load per cell per step, prefetched PF steps ahead);
+// otherwise it is staged step-major in LDS from the natural layout (sS[t][kap]
+// = cell (64r+kap, c0+t-kap+1)).  FIRST: the launch is the first sweep of a
+// solve, whose old traveltimes are time_set + time_init (1e6 except the 3x3
+// source stencil) and are generated here instead of read.  PERSIST: the
+// upwind row and the carry-in from block bb-1 arrive through the tagged
+// exchange buffers of the shot (and the tile publishes its own), instead of
+// being read from T after a launch boundary.
+template <int SA, int SB, bool EQH, bool SKEW, bool FIRST, bool PERSIST>
+__device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __restrict__ shots, int nzz,
+                                             const EikConsts& k, const SkewView& sv, const PersistCtl& ctl,
+                                             unsigned tagb, float* __restrict__ sT, float* __restrict__ sS,
+                                             float* __restrict__ sH) {
   const ShotDev sd = shots[tl.x];
   const int r = tl.y;
-  const int c0 = 64 * tl.z;
+  const int bb = tl.z;
+  const int c0 = 64 * bb;
   const int lane = threadIdx.x;
   const int nxx = sd.nxx;
   const int NR = nzz - 2, NC = nxx - 2;
   const int kg = 64 * r + lane;
   const bool row_ok = kg < NR;
   gfloat* __restrict__ T = gptr(sd.T);
-
-  // Staging.  Every load is unconditional: positions outside the interior are
-  // clamped onto the boundary ring (local row -1/NR, column -1/NC), whose
-  // traveltime is always 1e6, and slowness cells are clamped into the grid
-  // (those values only feed inactive lanes).  All 129 ring loads are issued
-  // before the first LDS write so their latencies overlap.
   const int kgc = min(kg, NR);
-  float tv_[64], sv_[64];
+
+  // ---- old traveltimes, step-major --------------------------------------
+  if (FIRST) {
+    const int sIdx = (int)((sd.sx + 0.5f * k.dx) / k.dx) + NB;
+    const int sIdz = (int)((sd.sz + 0.5f * k.dz) / k.dz) + NB;
+    // uniform test: does any of the 9 stencil nodes fall in this tile?
+    bool has_src = false;
+    for (int q = 0; q < 9; ++q) {
+      const int i = sIdz + q % 3 - 1, j = sIdx + q / 3 - 1;
+      const int kk = (SA > 0) ? i - 1 : nzz - 2 - i;
+      const int cc = (SB > 0) ? j - 1 : nxx - 2 - j;
+      const int kl = kk - 64 * r, tt = cc - c0 + kl;
+      if (kl >= 0 && kl < 64 && tt >= 0 && tt < 64) has_src = true;
+    }
+    const int i = prow<SA>(kg, nzz);
+    for (int t = 0; t < 64; ++t) {
+      float val = TINF;
+      if (has_src) {
+        const int c = c0 + t - lane;
+        const int j = pcol<SB>(c, nxx);
+        if (row_ok && c >= 0 && c < NC && abs(i - sIdz) <= 1 && abs(j - sIdx) <= 1) {
+          const float ddx = (float)(j - NB) * k.dx - sd.sx;
+          const float ddz = (float)(i - NB) * k.dz - sd.sz;
+          val = s_at(sd, nzz, i, j) * sqrtf(ddx * ddx + ddz * ddz);
+        }
+      }
+      sT[t * RING_LD + lane] = val;
+    }
+  } else {
+    // Pair loads: instruction v fetches, for lane k, column c = c0-k+((v+k)&63)
+    // (two contiguous column segments), i.e. the node of step (v+k)&63.
+    // Positions outside the interior clamp onto the boundary ring (always 1e6).
+    float tv_[64];
 #pragma unroll
-  for (int v = 0; v < 64; ++v) {
-    // old traveltimes: ring slot v of lane k holds column c = c0-k+((v+k)&63)
-    const int c = min(max(c0 - lane + ((v + lane) & 63), -1), NC);
-    tv_[v] = T[node_off<SA, SB>(kgc, c, nzz, nxx)];
+    for (int v = 0; v < 64; ++v) {
+      const int c = min(max(c0 - lane + ((v + lane) & 63), -1), NC);
+      tv_[v] = T[node_off<SA, SB>(kgc, c, nzz, nxx)];
+    }
+#pragma unroll
+    for (int v = 0; v < 64; ++v) sT[((v + lane) & 63) * RING_LD + lane] = tv_[v];
   }
-  const int kapc = min(64 * r + lane, NR);
-#pragma unroll
-  for (int v = 0; v < 64; ++v) {
-    // slowness cells: ring row kap (0..64) holds cell columns
-    // lam in [c0-kap+1, c0-kap+64]; slot = lam & 63
-    const int lam = min(max(c0 + 1 - lane + ((v + lane - 1) & 63), 0), NC);
-    sv_[v] = cell_at<SA, SB>(sd, nzz, kapc, lam);
+
+  // ---- upwind row above the strip (local row 64r-1) for lane 0, and the
+  // carry-in from block bb-1: te = T(kg, c0-k-1), tev = T(kg-1, c0-k-1).
+  float te, tev;
+  if (PERSIST) {
+    // halo: strip r-1's bottom row; lane t fetches column c0+t, lane 0 also c0-1
+    const bool hneed = r > 0 && c0 + lane < NC;
+    const unsigned long long* xh = sd.xh + (size_t)max(r - 1, 0) * sd.ldh;
+    sH[lane] = xwait(xh + min(c0 + lane, NC - 1), hneed, tagb, ctl.err);
+    const bool h1need = r > 0 && bb > 0 && lane == 0 && c0 - 1 < NC;
+    const float hprev = xwait(xh + min(max(c0 - 1, 0), NC - 1), h1need, tagb, ctl.err);
+    // carries of block bb-1: word 2k = lane k's step-63 result, 2k+1 = step 62
+    const bool cneed = bb > 0;
+    const unsigned long long* xc = sd.xc + (size_t)(2 * r + ((bb - 1) & 1)) * 128;
+    const unsigned ctag = tagb | ((unsigned)(bb - 1) & 255u);
+    te = xwait(xc + 2 * lane, cneed, ctag, ctl.err);
+    const float tprev = xwait(xc + 2 * max(lane - 1, 0) + 1, cneed && lane > 0, ctag, ctl.err);
+    tev = (lane == 0) ? hprev : tprev;
+  } else {
+    sH[lane] = T[node_off<SA, SB>(64 * r - 1, min(c0 + lane, NC), nzz, nxx)];
+    const int cin = min(max(c0 - lane - 1, -1), NC);
+    te = T[node_off<SA, SB>(kgc, cin, nzz, nxx)];
+    tev = T[node_off<SA, SB>(min(kg - 1, NR), cin, nzz, nxx)];
   }
-  // ring row 64 (lane = slot), the upwind row above the strip for lane 0
-  // (local row 64r-1, columns c0..c0+63) and the carry-in from block bb-1:
-  // te = T(kg, c0-k-1), tev = T(kg-1, c0-k-1), sref = cell (kg, c0-k).
-  const float s64 = cell_at<SA, SB>(sd, nzz, min(64 * r + 64, NR),
-                                    min(max(c0 + 1 - 64 + ((lane + 63) & 63), 0), NC));
-  const float hval = T[node_off<SA, SB>(64 * r - 1, min(c0 + lane, NC), nzz, nxx)];
-  const int cin = min(max(c0 - lane - 1, -1), NC);
-  float te = T[node_off<SA, SB>(kgc, cin, nzz, nxx)];
-  float tev = T[node_off<SA, SB>(min(kg - 1, NR), cin, nzz, nxx)];
-  float sref = cell_at<SA, SB>(sd, nzz, min(kg, NR), min(max(c0 - lane, 0), NC));
+
+  if (SKEW) {
+    const int kap = 64 * r + lane;
+    float sref = gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 - lane)];
+    __syncthreads();
+    // unreplicated cell columns: lam in [3, w+2] (SB > 0) or [2, w+1] (SB < 0)
+    const int lam_lo = max(0, c0 - 63), lam_hi = min(NC, c0 + 64);
+    const bool fast = (SB > 0) ? (lam_lo >= 3 && lam_hi <= sd.w + 2) : (lam_lo >= 2 && lam_hi <= sd.w + 1);
+    if (fast)
+      tile_steps_skew<SB, EQH, true>(sd, sv, sT, sH, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
+    else
+      tile_steps_skew<SB, EQH, false>(sd, sv, sT, sH, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
+  } else {
+    // Slowness cells step-major: sS[t][kap] = cell (64r+kap, c0+t-kap+1),
+    // kap = 0..64; lane k reads its z-other at [t][k] and x-other at [t][k+1].
+    const int kapc = min(64 * r + lane, NR);
+    float sv_[64];
 #pragma unroll
-  for (int v = 0; v < 64; ++v) sT[v * RING_LD + lane] = tv_[v];
+    for (int v = 0; v < 64; ++v) {
+      const int lam = min(max(c0 + 1 - lane + ((v + lane - 1) & 63), 0), NC);
+      sv_[v] = cell_at<SA, SB>(sd, nzz, kapc, lam);
+    }
+    const int lam64 = min(max(c0 + 1 - 64 + ((lane + 63) & 63), 0), NC);
+    const float s64 = cell_at<SA, SB>(sd, nzz, min(64 * r + 64, NR), lam64);
+    float sref = cell_at<SA, SB>(sd, nzz, kgc, min(max(c0 - lane, 0), NC));
 #pragma unroll
-  for (int v = 0; v < 64; ++v) sS[v * RING_LD + lane] = sv_[v];
-  sS[lane * RING_LD + 64] = s64;
-  sH[lane] = hval;
+    for (int v = 0; v < 64; ++v) sS[((v + lane - 1) & 63) * RING_LD + lane] = sv_[v];
+    sS[((lane + 63) & 63) * RING_LD + 64] = s64;
+    __syncthreads();
+    float told_n = sT[lane], h_n = sH[0], z_n = sS[lane], x_n = sS[lane + 1];
+    for (int t0 = 0; t0 < 64; t0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = t0 + u;
+        const float told = told_n, h = h_n, szo = z_n, sxo = x_n;
+        told_n = sT[(t + 1) * RING_LD + lane];
+        h_n = sH[t + 1];
+        z_n = sS[(t + 1) * RING_LD + lane];
+        x_n = sS[(t + 1) * RING_LD + lane + 1];
+        tile_step<EQH>(sT, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+      }
+    }
+  }
   __syncthreads();
 
-#pragma unroll 4
-  for (int t = 0; t < 64; ++t) {
-    const int slot = (t - lane) & 63;
-    const float told = sT[slot * RING_LD + lane];
-    const float szo = sS[((slot + 1) & 63) * RING_LD + lane];
-    const float sxo = sS[slot * RING_LD + lane + 1];
-    const float tv = dpp_shr1(sH[t], te);
-    const int c = c0 + t - lane;
-    const bool act = row_ok && (unsigned)c < (unsigned)NC;
-    const float cand = eik_local<EQH>(tv, te, tev, sref, szo, sxo, k);
-    const float res = act ? fminf(told, cand) : told;
-    sT[slot * RING_LD + lane] = res;
-    tev = tv;
-    te = res;
-    sref = szo;
+  if (PERSIST) {
+    // publish: lane 63's results are strip r+1's upwind row (columns c0+t-63),
+    // each lane's last two results are block bb+1's carry-in.
+    if (r + 1 < ctl.nstrips) {
+      const int c = c0 + lane - 63;
+      if (c >= 0 && c < NC) xstore(sd.xh + (size_t)r * sd.ldh + c, sT[lane * RING_LD + 63], tagb);
+    }
+    unsigned long long* xc = sd.xc + (size_t)(2 * r + (bb & 1)) * 128;
+    const unsigned ctag = tagb | ((unsigned)bb & 255u);
+    xstore(xc + 2 * lane, sT[63 * RING_LD + lane], ctag);
+    xstore(xc + 2 * lane + 1, sT[62 * RING_LD + lane], ctag);
   }
-  __syncthreads();
 
 #pragma unroll 8
   for (int v = 0; v < 64; ++v) {
-    const int c = c0 - lane + ((v + lane) & 63);
-    if (row_ok && c >= 0 && c < NC) T[node_off<SA, SB>(kg, c, nzz, nxx)] = sT[v * RING_LD + lane];
+    const int t = (v + lane) & 63;
+    const int c = c0 + t - lane;
+    if (row_ok && c >= 0 && c < NC) T[node_off<SA, SB>(kg, c, nzz, nxx)] = sT[t * RING_LD + lane];
   }
+}
+
+// One tile per workgroup; launch d runs the tiles of tile diagonal d = 2r+bb.
+template <int SA, int SB, bool EQH, bool SKEW, bool FIRST>
+__global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ shots,
+                                                    const int4* __restrict__ tiles, int nzz, EikConsts k,
+                                                    SkewView sv) {
+  __shared__ float sT[65 * RING_LD];  // row 64: landing slot of the last step's read-ahead
+  __shared__ float sS[SKEW ? 1 : 65 * RING_LD];
+  __shared__ float sH[65];
+  const PersistCtl ctl{};
+  tile_process<SA, SB, EQH, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, sH);
+}
+
+// Persistent sweep: every workgroup claims tiles from a ticket counter in
+// tile-diagonal order until the sweep's tiles run out.  A tile only waits on
+// tiles with smaller tickets, which running workgroups hold, so the sweep
+// always drains.
+template <int SA, int SB, bool EQH, bool SKEW, bool FIRST>
+__global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict__ shots,
+                                                      const int4* __restrict__ tiles, int nzz, EikConsts k,
+                                                      SkewView sv, PersistCtl ctl) {
+  __shared__ float sT[65 * RING_LD];
+  __shared__ float sS[SKEW ? 1 : 65 * RING_LD];
+  __shared__ float sH[65];
+  const unsigned tagb = (*ctl.epoch << 11) | (ctl.sweep_id << 8);
+  for (;;) {
+    int tk = 0;
+    if (threadIdx.x == 0) tk = atomicAdd(ctl.counter, 1);
+    tk = __builtin_amdgcn_readfirstlane(tk);
+    if (tk >= ctl.ntiles) break;
+    tile_process<SA, SB, EQH, SKEW, FIRST, true>(tiles[tk], shots, nzz, k, sv, ctl, tagb, sT, sS, sH);
+    __syncthreads();
+  }
+}
+
+// Start of a run: advance the epoch (tags of the previous run become stale)
+// and zero the ticket counters.
+__global__ void k_begin_run(int* __restrict__ counters, int n, unsigned* __restrict__ epoch) {
+  const int i = threadIdx.x;
+  if (i < n) counters[i] = 0;
+  if (i == 0) *epoch = (*epoch + 1) % 2097151u;
 }
 
 // ---------------------------------------------------------------------------
@@ -372,35 +631,115 @@ int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh
   return launches;
 }
 
-template <int SA, int SB, bool EQH>
+template <int SA, int SB, bool EQH, bool SKEW, bool FIRST>
 static void tiles_diag(const ShotDev* shots, const int4* tiles, int n, int nzz, const EikConsts& k,
-                       hipStream_t st) {
-  hipLaunchKernelGGL((k_sweep_tiles<SA, SB, EQH>), dim3(n), dim3(64), 0, st, shots, tiles, nzz, k);
+                       const SkewView& sv, const PersistCtl* ctl, hipStream_t st) {
+  if (ctl) {
+    static int grid = 0;
+    if (grid == 0) {
+      int per_cu = 0, dev = 0, cus = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persist<SA, SB, EQH, SKEW, FIRST>, 64,
+                                                       0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+      (void)hipGetDevice(&dev);
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+      grid = per_cu * cus;
+    }
+    hipLaunchKernelGGL((k_sweep_persist<SA, SB, EQH, SKEW, FIRST>), dim3(std::min(grid, ctl->ntiles)), dim3(64),
+                       0, st, shots, tiles, nzz, k, sv, *ctl);
+  } else {
+    hipLaunchKernelGGL((k_sweep_tiles<SA, SB, EQH, SKEW, FIRST>), dim3(n), dim3(64), 0, st, shots, tiles, nzz, k,
+                       sv);
+  }
 }
 
-int launch_sweeps_tiled(const ShotDev* shots, const int4* tiles, const std::vector<int>& diag_off,
-                        int nzz, const EikConsts& k, bool eqh, hipStream_t st) {
+template <bool EQH, bool SKEW>
+static void tiles_dispatch(int sweep, bool first, const ShotDev* shots, const int4* tl, int n, int nzz,
+                           const EikConsts& k, const SkewView& sv, const PersistCtl* ctl, hipStream_t st) {
+  switch (sweep) {
+    case 0:
+      if (first)
+        tiles_diag<1, 1, EQH, SKEW, true>(shots, tl, n, nzz, k, sv, ctl, st);
+      else
+        tiles_diag<1, 1, EQH, SKEW, false>(shots, tl, n, nzz, k, sv, ctl, st);
+      break;
+    case 1: tiles_diag<-1, 1, EQH, SKEW, false>(shots, tl, n, nzz, k, sv, ctl, st); break;
+    case 2: tiles_diag<1, -1, EQH, SKEW, false>(shots, tl, n, nzz, k, sv, ctl, st); break;
+    default: tiles_diag<-1, -1, EQH, SKEW, false>(shots, tl, n, nzz, k, sv, ctl, st); break;
+  }
+}
+
+static void tiles_launch(int sweep, bool first, bool eqh, const SkewView* skew, const ShotDev* shots,
+                         const int4* tl, int n, int nzz, const EikConsts& k, const PersistCtl* ctl,
+                         hipStream_t st) {
+  const SkewView sv = skew ? skew[sweep] : SkewView{nullptr, 0};
+  if (skew) {
+    if (eqh) tiles_dispatch<true, true>(sweep, first, shots, tl, n, nzz, k, sv, ctl, st);
+    else tiles_dispatch<false, true>(sweep, first, shots, tl, n, nzz, k, sv, ctl, st);
+  } else {
+    if (eqh) tiles_dispatch<true, false>(sweep, first, shots, tl, n, nzz, k, sv, ctl, st);
+    else tiles_dispatch<false, false>(sweep, first, shots, tl, n, nzz, k, sv, ctl, st);
+  }
+}
+
+int launch_sweeps_tiled(const ShotDev* shots, const int4* tiles, const std::vector<int>& diag_off, int nzz,
+                        const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused, hipStream_t st) {
   int launches = 0;
   const int D = (int)diag_off.size() - 1;
   for (int sweep = 0; sweep < 4; ++sweep) {
+    const bool first = first_fused && sweep == 0;
     for (int d = 0; d < D; ++d) {
       const int n = diag_off[d + 1] - diag_off[d];
       if (n <= 0) continue;
-      const int4* tl = tiles + diag_off[d];
-      switch (sweep * 2 + (eqh ? 1 : 0)) {
-        case 0: tiles_diag<1, 1, false>(shots, tl, n, nzz, k, st); break;
-        case 1: tiles_diag<1, 1, true>(shots, tl, n, nzz, k, st); break;
-        case 2: tiles_diag<-1, 1, false>(shots, tl, n, nzz, k, st); break;
-        case 3: tiles_diag<-1, 1, true>(shots, tl, n, nzz, k, st); break;
-        case 4: tiles_diag<1, -1, false>(shots, tl, n, nzz, k, st); break;
-        case 5: tiles_diag<1, -1, true>(shots, tl, n, nzz, k, st); break;
-        case 6: tiles_diag<-1, -1, false>(shots, tl, n, nzz, k, st); break;
-        default: tiles_diag<-1, -1, true>(shots, tl, n, nzz, k, st); break;
-      }
+      tiles_launch(sweep, first, eqh, skew, shots, tiles + diag_off[d], n, nzz, k, nullptr, st);
       ++launches;
     }
   }
   return launches;
+}
+
+int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, int nstrips, int nzz,
+                          const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused,
+                          const PersistRun& pr, int solve, hipStream_t st) {
+  for (int sweep = 0; sweep < 4; ++sweep) {
+    PersistCtl ctl;
+    ctl.counter = pr.counters + solve * 4 + sweep;
+    ctl.epoch = pr.epoch;
+    ctl.err = pr.err;
+    ctl.ntiles = ntiles;
+    ctl.nstrips = nstrips;
+    ctl.sweep_id = (unsigned)(solve * 4 + sweep);
+    tiles_launch(sweep, first_fused && sweep == 0, eqh, skew, shots, tiles, ntiles, nzz, k, &ctl, st);
+  }
+  return 4;
+}
+
+void launch_begin_run(const PersistRun& pr, hipStream_t st) {
+  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(64), 0, st, pr.counters, 8, pr.epoch);
+}
+
+// Skewed slowness copy for sweep orientation (SA, SB): one thread per entry.
+__global__ void k_skew_s(const float* __restrict__ szp, float* __restrict__ sk, int nzz, int nx, int KP,
+                         int NS, int SA, int SB) {
+  const size_t n = (size_t)NS * KP;
+  size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; idx < n; idx += stride) {
+    const int kap = (int)(idx % KP);
+    const int sig = (int)(idx / KP) - SK_OFF;
+    const int m = (SB > 0) ? sig - (kap & 63) : sig + (kap & 63) - 63;
+    const int i = min(max((SA > 0) ? kap : nzz - 2 - kap, 0), nzz - 1);
+    sk[idx] = szp[i + (size_t)min(max(m, 0), nx - 1) * nzz];
+  }
+}
+
+void launch_skew_s(const float* szp, float* sk, int nzz, int nx, int KP, int NS, int sweep, hipStream_t st) {
+  const int SA = (sweep == 0 || sweep == 2) ? 1 : -1;
+  const int SB = (sweep == 0 || sweep == 1) ? 1 : -1;
+  const size_t n = (size_t)NS * KP;
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_skew_s, dim3(blocks), dim3(256), 0, st, szp, sk, nzz, nx, KP, NS, SA, SB);
 }
 
 void launch_picks(const ShotDev* shots, const int* rec_shot, const float* rx, const float* rz, float* out,

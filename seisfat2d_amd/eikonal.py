@@ -12,9 +12,9 @@ import ctypes as C
 import numpy as np
 
 from . import capi
-from .capi import ANI, ISO, SCHED_PLAIN, SCHED_TILED, Shot, check, lib
+from .capi import ANI, ISO, SCHED_PLAIN, SCHED_TILED, SCHED_TILED_DIAG, Shot, check, lib
 
-__all__ = ["Eikonal", "Batch", "ISO", "ANI", "SCHED_PLAIN", "SCHED_TILED", "NB"]
+__all__ = ["Eikonal", "Batch", "ISO", "ANI", "SCHED_PLAIN", "SCHED_TILED", "SCHED_TILED_DIAG", "NB"]
 
 NB = 3
 
@@ -144,6 +144,10 @@ class Batch:
         ms = C.c_float()
         check(lib().seisfat_batch_sweep_ms(self._b, C.byref(ms)))
         return ms.value
+
+    def status(self):
+        """Synchronise and raise if the device flagged a failure during the runs."""
+        check(lib().seisfat_batch_status(self._b))
 
     def close(self):
         if getattr(self, "_b", None):

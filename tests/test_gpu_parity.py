@@ -13,7 +13,9 @@ import numpy as np
 import pytest
 from scipy.ndimage import gaussian_filter
 
-from seisfat2d_amd import ANI, ISO, SCHED_PLAIN, SCHED_TILED, Eikonal, capi, models
+from seisfat2d_amd import ANI, ISO, SCHED_PLAIN, SCHED_TILED, SCHED_TILED_DIAG, Eikonal, capi, models
+
+SCHEDULES = (SCHED_TILED, SCHED_TILED_DIAG, SCHED_PLAIN)
 
 pytestmark = pytest.mark.gpu
 NB = 3
@@ -30,7 +32,7 @@ def solve_both(oracle, s, dx, dz, sx, sz, rx=None, rz=None):
     e.set_slowness(s)
     Tref = oracle.solve_iso(oracle.expand_boundary(s), dx, dz, sx, sz)
     out = {}
-    for sched in (SCHED_TILED, SCHED_PLAIN):
+    for sched in SCHEDULES:
         out[sched] = e.solve(sx, sz, schedule=sched, rx=rx, rz=rz)
     e.close()
     return Tref, out
@@ -97,6 +99,7 @@ def test_c3_all_256_windows_bit_exact(oracle):
     picks = b.picks()
     b.run()  # graph replay must reproduce the same bits
     picks2 = b.picks()
+    b.status()
     assert np.array_equal(picks, picks2)
     sx = [s[0] for s in w.shots]
     x0 = [s[2] for s in w.shots]
@@ -151,7 +154,7 @@ def test_ani_batch_bit_exact(oracle):
     rx = np.arange(0, nx, 7, dtype=np.float32) * dh
     shots = [(float(x), 0.0, 0, nx, 0, 0) for x in sxs]
     shots[2] = (float(sxs[2]), 0.0, 0, nx, 0, len(rx))
-    for sched in (SCHED_TILED, SCHED_PLAIN):
+    for sched in SCHEDULES:
         b = e.batch(shots, rx, np.zeros_like(rx), type=ANI, schedule=sched)
         b.run()
         pk = b.picks()
