@@ -65,7 +65,12 @@ def main():
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--plain-shots", type=int, default=3, help="windows timed with the per-diagonal schedule")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU oracle budget (0 = skip)")
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler run: the batched steps only, plus a 256 MiB device copy that calibrates "
+                         "per-dispatch byte counters")
     args = ap.parse_args()
+    if args.profile:
+        args.plain_shots, args.cpu_seconds = 0, 0.0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(shots would shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
@@ -110,6 +115,17 @@ def main():
     avg_launch_s = sweep_ms / 1e3 / nlaunch
     achieved = bytes_per_launch / avg_launch_s / 1e9
     log(f"sweep phase {sweep_ms:.2f} ms over {nlaunch} launches -> {achieved:.0f} GB/s algorithmic")
+
+    if args.profile:
+        # counter calibration: a plain copy of exactly 256 MiB (reads 256 MiB, writes 256 MiB)
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"profile: calibration copy of {src.numel() * 4} bytes done")
+        b.close()
+        e.close()
+        return
 
     # per-shot comparisons on the first windows
     nps = max(1, min(args.plain_shots, len(w.shots)))

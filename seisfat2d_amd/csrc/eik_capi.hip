@@ -119,6 +119,15 @@ int check_handle(seisfat_eik_t h) {
   return SEISFAT_OK;
 }
 
+// Operating range of the local solver's normal-range division and square
+// root (see eik_device.h): the traveltime increments slowness*spacing and the
+// gradient scale slowness/spacing stay far from fp32 under- and overflow.
+bool slowness_ok(float v, float hmin, float hmax) {
+  if (!(v > 0.0f) || !std::isfinite(v)) return false;
+  const double s = v;
+  return s * hmin >= 1e-12 && s * hmax <= 1e12 && s / hmax >= 1e-12 && s / hmin <= 1e12;
+}
+
 EikConsts make_consts(float dx, float dz) {
   EikConsts k;
   k.dx = dx;
@@ -481,8 +490,8 @@ int seisfat_eik_create(int nz, int nx, float dz, float dx, seisfat_eik_t* out) {
   *out = nullptr;
   if (nz < 2 || nx < 2) return fail(SEISFAT_EINVAL, "model needs at least 2x2 samples");
   if ((long long)(nz + 6) * (nx + 6) > (1LL << 30)) return fail(SEISFAT_EINVAL, "model too large (> 2^30 padded nodes)");
-  if (!(dz > 0.f) || !(dx > 0.f) || !std::isfinite(dz) || !std::isfinite(dx))
-    return fail(SEISFAT_EINVAL, "spacings must be positive and finite");
+  if (!(dz >= 1e-6f && dz <= 1e6f) || !(dx >= 1e-6f && dx <= 1e6f))
+    return fail(SEISFAT_EINVAL, "spacings must lie in [1e-6, 1e6]");
   seisfat_eik_s* h = new (std::nothrow) seisfat_eik_s();
   if (!h) return fail(SEISFAT_ENOMEM, "host allocation");
   h->nz = nz;
@@ -516,6 +525,28 @@ int seisfat_eik_create(int nz, int nx, float dz, float dx, seisfat_eik_t* out) {
     delete h;
     return rc;
   }
+  // refined reciprocals of the local solver's divisors, computed on the device
+  {
+    const float divs[4] = {h->kc.sum, h->kc.hyp, h->kc.dz, h->kc.dx};
+    float rec[4] = {};
+    float* d = nullptr;
+    hipError_t e2 = hipMalloc((void**)&d, 8 * sizeof(float));
+    if (e2 == hipSuccess) e2 = hipMemcpy(d, divs, sizeof(divs), hipMemcpyHostToDevice);
+    if (e2 == hipSuccess) {
+      launch_recip(d, d + 4, 4, h->stream);
+      e2 = hipStreamSynchronize(h->stream);
+    }
+    if (e2 == hipSuccess) e2 = hipMemcpy(rec, d + 4, sizeof(rec), hipMemcpyDeviceToHost);
+    if (d) (void)hipFree(d);
+    if (e2 != hipSuccess) {
+      seisfat_eik_destroy(h);
+      return fail(SEISFAT_EHIP, std::string("reciprocal setup: ") + hipGetErrorString(e2));
+    }
+    h->kc.r_sum = rec[0];
+    h->kc.r_hyp = rec[1];
+    h->kc.r_dz = rec[2];
+    h->kc.r_dx = rec[3];
+  }
   *out = h;
   return SEISFAT_OK;
 }
@@ -538,7 +569,7 @@ int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* str
   if (!d_s) return fail(SEISFAT_EINVAL, "null slowness pointer");
   hipStream_t st = stream ? (hipStream_t)stream : h->stream;
   HIPCHK(hipMemsetAsync(h->d_flag, 0, sizeof(int), st));
-  launch_check_positive(d_s, (size_t)h->nz * h->nx, h->d_flag, st);
+  launch_check_positive(d_s, (size_t)h->nz * h->nx, std::min(h->dx, h->dz), std::max(h->dx, h->dz), h->d_flag, st);
   launch_zpad(d_s, h->d_szp, h->nz, h->nx, st);
   for (int o = 0; o < 4; ++o) launch_skew_s(h->d_szp, h->d_sk[o], h->nzz, h->nx, h->KP, h->NS, o, st);
   HIPCHK(hipGetLastError());
@@ -547,7 +578,8 @@ int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* str
   HIPCHK(hipStreamSynchronize(st));
   if (bad) {
     h->has_s = false;
-    return fail(SEISFAT_EINVAL, "slowness must be positive and finite everywhere");
+    return fail(SEISFAT_EINVAL, "slowness must be positive and finite, with slowness*spacing and "
+                                "slowness/spacing in [1e-12, 1e12], everywhere");
   }
   h->has_s = true;
   return SEISFAT_OK;
@@ -557,10 +589,12 @@ int seisfat_eik_set_slowness(seisfat_eik_t h, const float* s) {
   if (int rc = check_handle(h)) return rc;
   if (!s) return fail(SEISFAT_EINVAL, "null slowness pointer");
   const size_t n = (size_t)h->nz * h->nx;
+  const float hmin = std::min(h->dx, h->dz), hmax = std::max(h->dx, h->dz);
   for (size_t k = 0; k < n; ++k)
-    if (!(s[k] > 0.0f) || !std::isfinite(s[k])) {
+    if (!slowness_ok(s[k], hmin, hmax)) {
       h->has_s = false;
-      return fail(SEISFAT_EINVAL, "slowness must be positive and finite everywhere (index " + std::to_string(k) + ")");
+      return fail(SEISFAT_EINVAL, "slowness must be positive and finite, with slowness*spacing and "
+                                  "slowness/spacing in [1e-12, 1e12] (index " + std::to_string(k) + ")");
     }
   float* d_s = nullptr;
   if (int rc = dalloc(&d_s, n)) return rc;

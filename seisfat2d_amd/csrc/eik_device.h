@@ -24,7 +24,53 @@ struct EikConsts {
   float hyp;         // sqrtf(dx*dx + dz*dz)
   float sum;         // dz2i + dx2i
   float prod;        // dz2i * dx2i
+  // refined reciprocals of the four divisors (see rdiv below), produced on
+  // the device by k_recip so they are exactly the values the hardware
+  // division sequence would build
+  float r_sum, r_hyp, r_dz, r_dx;
 };
+
+// ---------------------------------------------------------------------------
+// Correctly rounded division and square root for normal-range operands.
+//
+// HIP's correctly rounded fp32 division is, for operands whose exponents need
+// no rescaling and whose result is finite and normal (v_div_scale and
+// v_div_fixup are then identities):
+//   r0 = rcp(b); e = fma(-b, r0, 1); r = fma(e, r0, r0)        (refined 1/b)
+//   q0 = a*r; e1 = fma(-b, q0, a); q1 = fma(e1, r, q0); e2 = fma(-b, q1, a);
+//   q  = fma(e2, r, q1)
+// Every divisor of the local solver is a per-model constant, so r is computed
+// once (k_recip) and a division costs five instructions.  The square root
+// likewise keeps only the hardware estimate and its two one-ulp corrections
+// (no denormal rescaling, no zero/inf class fix-up).  The host rejects models
+// whose slowness x spacing leaves the range where these identities hold, and
+// the GPU parity tests check every node against the CPU oracle's IEEE
+// operations bit for bit.
+__device__ __forceinline__ float rdiv(float a, float b, float r) {
+  const float q0 = a * r;
+  const float e1 = __builtin_fmaf(-b, q0, a);
+  const float q1 = __builtin_fmaf(e1, r, q0);
+  const float e2 = __builtin_fmaf(-b, q1, a);
+  return __builtin_fmaf(e2, r, q1);
+}
+
+__device__ __forceinline__ float rsqrt_rn(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sdn = __int_as_float(__float_as_int(s) - 1);
+  const float sup = __int_as_float(__float_as_int(s) + 1);
+  const float r1 = __builtin_fmaf(-sdn, s, x);
+  const float r2 = __builtin_fmaf(-sup, s, x);
+  const float t = (r1 <= 0.0f) ? sdn : s;
+  return (r2 > 0.0f) ? sup : t;
+}
+
+// IEEE minNum of two non-NaN values without the sNaN-quieting canonicalise
+// the compiler would add for loaded operands (no operand here is a NaN).
+__device__ __forceinline__ float vmin(float a, float b) {
+  float r;
+  asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 
 // A shot as the kernels see it.  T is the shot's padded traveltime grid
 // (nzz x nxx, column stride nzz).  Slowness of padded node/cell (i, j) is
@@ -81,9 +127,9 @@ __device__ __forceinline__ float s_at(const ShotDev& sd, int nzz, int i, int j) 
 template <bool EQH>
 __device__ __forceinline__ float eik_local(float tv, float te, float tev, float sref, float szo,
                                            float sxo, const EikConsts& k) {
-  const float t1d1 = tv + k.dz * fminf(sref, szo);
-  const float t1d2 = te + k.dx * fminf(sref, sxo);
-  const float t1D = fminf(t1d1, t1d2);
+  const float t1d1 = tv + k.dz * vmin(sref, szo);
+  const float t1d2 = te + k.dx * vmin(sref, sxo);
+  const float t1D = vmin(t1d1, t1d2);
   const float de = te - tev;
   const float dv = tv - tev;
   float t2D = TINF;
@@ -92,27 +138,29 @@ __device__ __forceinline__ float eik_local(float tv, float te, float tev, float 
     const float ta = tev + te - tv;
     const float tb = tev - te + tv;
     const float dab = ta - tb;
-    t2D = ((tb * k.dz2i + ta * k.dx2i) + sqrtf(4.0f * sref * sref * k.sum - k.prod * dab * dab)) / k.sum;
+    t2D = rdiv((tb * k.dz2i + ta * k.dx2i) + rsqrt_rn(4.0f * sref * sref * k.sum - k.prod * dab * dab), k.sum,
+               k.r_sum);
   } else {
     // branches B (from te) and C (from tv) are exclusive else-ifs in the
     // reference; one evaluation serves whichever a lane takes.
-    const float thrB = sref * k.dz * k.dz / k.hyp;
+    const float thrB = rdiv(sref * k.dz * k.dz, k.hyp, k.r_hyp);
     const bool cB = (de <= thrB) && (de > 0.0f);
     bool cC = false;
     if (!cB) {
-      const float thrC = EQH ? thrB : sref * k.dx * k.dx / k.hyp;
+      const float thrC = EQH ? thrB : rdiv(sref * k.dx * k.dx, k.hyp, k.r_hyp);
       cC = (dv <= thrC) && (dv > 0.0f);
     }
     if (cB || cC) {
       const float dd = cB ? de : dv;
       const float h1 = cB ? k.dz : k.dx;
+      const float r1 = cB ? k.r_dz : k.r_dx;
       const float h2 = cB ? k.dx : k.dz;
       const float t0 = cB ? te : tv;
-      const float q = dd / h1;
-      t2D = t0 + h2 * sqrtf(sref * sref - q * q);
+      const float q = rdiv(dd, h1, r1);
+      t2D = t0 + h2 * rsqrt_rn(sref * sref - q * q);
     }
   }
-  return fminf(t1D, fminf(t2D, TINF));
+  return vmin(t1D, vmin(t2D, TINF));
 }
 
 }  // namespace seisfat

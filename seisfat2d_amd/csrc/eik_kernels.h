@@ -18,8 +18,9 @@ struct StiffDev {
 };
 
 void launch_fill(float* p, size_t n, float v, hipStream_t st);
+void launch_recip(const float* b, float* r, int n, hipStream_t st);
 void launch_zpad(const float* s, float* szp, int nz, int nx, hipStream_t st);
-void launch_check_positive(const float* s, size_t n, int* bad, hipStream_t st);
+void launch_check_positive(const float* s, size_t n, float hmin, float hmax, int* bad, hipStream_t st);
 void launch_init(const ShotDev* shots, int nshots, int nzz, float dx, float dz, hipStream_t st);
 // Returns the number of kernel launches issued.
 int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh, hipStream_t st);

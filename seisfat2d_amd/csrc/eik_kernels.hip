@@ -36,14 +36,21 @@ __global__ void k_zpad(const float* __restrict__ s, float* __restrict__ szp, int
   }
 }
 
-// Flags non-finite or non-positive slowness (bit 0 of *bad).
-__global__ void k_check_positive(const float* __restrict__ s, size_t n, int* __restrict__ bad) {
+// Flags slowness outside the solver's operating range: non-finite or
+// non-positive, or slowness*spacing / slowness/spacing outside [1e-12, 1e12].
+__global__ void k_check_positive(const float* __restrict__ s, size_t n, float hmin, float hmax,
+                                 int* __restrict__ bad) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   int flag = 0;
   for (; i < n; i += stride) {
     const float v = s[i];
-    if (!(v > 0.0f) || !(v < __builtin_inff())) flag = 1;
+    if (!(v > 0.0f) || !(v < __builtin_inff())) {
+      flag = 1;
+    } else {
+      const double d = v;
+      if (d * hmin < 1e-12 || d * hmax > 1e12 || d / hmax < 1e-12 || d / hmin > 1e12) flag = 1;
+    }
   }
   if (flag) atomicOr(bad, 1);
 }
@@ -90,7 +97,7 @@ __global__ __launch_bounds__(256) void k_sweep_plain(ShotDev sd, int nzz, int z_
     const float szo = s_at(sd, nzz, i1, j1 + b);
     const float sxo = s_at(sd, nzz, i1 + a, j1);
     const size_t idx = i + (size_t)j * nzz;
-    T[idx] = fminf(T[idx], eik_local<EQH>(tv, te, tev, sref, szo, sxo, k));
+    T[idx] = vmin(T[idx], eik_local<EQH>(tv, te, tev, sref, szo, sxo, k));
   }
 }
 
@@ -166,7 +173,7 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int t, int lan
   const int c = c0 + t - lane;
   const bool act = row_ok && (unsigned)c < (unsigned)NC;
   const float cand = eik_local<EQH>(tv, te, tev, sref, szo, sxo, k);
-  const float res = act ? fminf(told, cand) : told;
+  const float res = act ? vmin(told, cand) : told;
   sT[t * RING_LD + lane] = res;
   tev = tv;
   te = res;
@@ -565,8 +572,23 @@ __global__ void k_quasi_slowness(const ShotDev* __restrict__ iso, ShotDev* __res
   }
 }
 
+// Refined reciprocals of the local solver's divisors (the first three steps of
+// the hardware division sequence; see rdiv).
+__global__ void k_recip(const float* __restrict__ b, float* __restrict__ r, int n) {
+  const int i = threadIdx.x;
+  if (i < n) {
+    const float r0 = __builtin_amdgcn_rcpf(b[i]);
+    const float e = __builtin_fmaf(-b[i], r0, 1.0f);
+    r[i] = __builtin_fmaf(e, r0, r0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
+void launch_recip(const float* b, float* r, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_recip, dim3(1), dim3(64), 0, st, b, r, n);
+}
+
 void launch_fill(float* p, size_t n, float v, hipStream_t st) {
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
   hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, n, v);
@@ -578,9 +600,9 @@ void launch_zpad(const float* s, float* szp, int nz, int nx, hipStream_t st) {
   hipLaunchKernelGGL(k_zpad, dim3(blocks), dim3(256), 0, st, s, szp, nz, nx);
 }
 
-void launch_check_positive(const float* s, size_t n, int* bad, hipStream_t st) {
+void launch_check_positive(const float* s, size_t n, float hmin, float hmax, int* bad, hipStream_t st) {
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_check_positive, dim3(blocks), dim3(256), 0, st, s, n, bad);
+  hipLaunchKernelGGL(k_check_positive, dim3(blocks), dim3(256), 0, st, s, n, hmin, hmax, bad);
 }
 
 void launch_init(const ShotDev* shots, int nshots, int nzz, float dx, float dz, hipStream_t st) {

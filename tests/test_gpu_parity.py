@@ -190,6 +190,9 @@ def test_invalid_inputs_rejected():
     s[3, 4] = np.nan
     with pytest.raises(capi.SeisfatError):
         e.set_slowness(s)
+    s[3, 4] = 1e-20  # slowness*spacing below the solver's operating range
+    with pytest.raises(capi.SeisfatError):
+        e.set_slowness(s)
     with pytest.raises(capi.SeisfatError) as ex:  # no valid slowness yet
         e.solve(10.0, 10.0)
     assert ex.value.code == capi.ESTATE
