@@ -91,6 +91,10 @@ int seisfat_eik_solve(seisfat_eik_t h, int type, int schedule, float sx, float s
  * reference's public Modeling::d_T, read by migration,
  * src/migration/migration.cu:130,167).  Valid until the next solve/destroy. */
 const float* seisfat_eik_device_T(seisfat_eik_t h);
+/* Copy the padded traveltimes of the last seisfat_eik_solve to host memory
+ * ((nz+6)*(nx+6) floats): the D2H copy of Modeling::compute_seismogram
+ * (src/modeling/modeling.cu:165) that leaves the public Modeling::T valid. */
+int seisfat_eik_copy_T(seisfat_eik_t h, float* T_padded);
 
 /* Batched solve of many shots (no reference counterpart: the reference loops
  * shots one at a time, src/modeling_main.cpp:21-29).  create() validates the

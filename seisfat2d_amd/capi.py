@@ -20,7 +20,7 @@ SCHED_PLAIN, SCHED_TILED, SCHED_TILED_DIAG = 0, 1, 2
 EXPORTS = [
     "seisfat_last_error", "seisfat_eik_create", "seisfat_eik_destroy", "seisfat_eik_set_slowness",
     "seisfat_eik_set_slowness_device", "seisfat_eik_set_stiffness", "seisfat_eik_solve",
-    "seisfat_eik_device_T", "seisfat_batch_create", "seisfat_batch_run", "seisfat_batch_picks",
+    "seisfat_eik_device_T", "seisfat_eik_copy_T", "seisfat_batch_create", "seisfat_batch_run", "seisfat_batch_picks",
     "seisfat_batch_device_picks", "seisfat_batch_device_T", "seisfat_batch_copy_T",
     "seisfat_batch_num_launches", "seisfat_batch_num_sweep_launches", "seisfat_batch_sweep_ms",
     "seisfat_batch_status", "seisfat_batch_destroy",
@@ -59,6 +59,7 @@ def lib():
         L.seisfat_eik_solve.argtypes = [vp, i, i, f, f, vp, vp, vp, i, vp]
         L.seisfat_eik_device_T.restype = vp
         L.seisfat_eik_device_T.argtypes = [vp]
+        L.seisfat_eik_copy_T.argtypes = [vp, vp]
         L.seisfat_batch_create.argtypes = [vp, i, i, i, C.POINTER(Shot), i, vp, vp, C.POINTER(vp)]
         L.seisfat_batch_run.argtypes = [vp, vp]
         L.seisfat_batch_picks.argtypes = [vp, vp]

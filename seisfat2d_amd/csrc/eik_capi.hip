@@ -463,6 +463,7 @@ int batch_status(seisfat_batch_s* b) {
 }
 
 int run_batch(seisfat_batch_s* b, hipStream_t st) {
+  if (!b->h->has_s) return fail(SEISFAT_ESTATE, "slowness rejected or not set since the batch was created");
   if (!st) st = b->h->stream;
   if (b->gexec[0]) {
     HIPCHK(hipGraphLaunch(b->gexec[0], st));
@@ -671,6 +672,13 @@ const float* seisfat_eik_device_T(seisfat_eik_t h) {
     return nullptr;
   }
   return h->single->d_T;
+}
+
+int seisfat_eik_copy_T(seisfat_eik_t h, float* T_padded) {
+  if (int rc = check_handle(h)) return rc;
+  if (!h->single) return fail(SEISFAT_ESTATE, "no solve yet");
+  if (!T_padded) return fail(SEISFAT_EINVAL, "null output buffer");
+  return seisfat_batch_copy_T(h->single, 0, T_padded);
 }
 
 int seisfat_batch_create(seisfat_eik_t h, int type, int schedule, int nshots, const seisfat_shot* shots,

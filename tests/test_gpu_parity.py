@@ -118,6 +118,37 @@ def test_c3_all_256_windows_bit_exact(oracle):
     e.close()
 
 
+def test_batch_rerun_after_new_slowness(oracle, rng):
+    """A batch reads the handle's model at run time: after set_slowness with a
+    new model, the next run (same graph) solves the new model; after a rejected
+    model it refuses to run."""
+    nz, nx, dh = 90, 300, 10.0
+    shots = [(600.0, 0.0, 0, 160, 0, 0), (1500.0, 20.0, 100, 200, 0, 0), (2500.0, 0.0, 0, nx, 0, 0)]
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(smooth_slowness(rng, nz, nx, dh))
+    for sched in (SCHED_TILED, SCHED_PLAIN):
+        b = e.batch(shots, schedule=sched)
+        b.run()
+        for it in range(2):
+            s = smooth_slowness(rng, nz, nx, dh)
+            e.set_slowness(s)
+            b.run()
+            b.status()
+            for k, (sx, sz, x0, wd, _, _) in enumerate(shots):
+                Tr = oracle.solve_windows(s, dh, dh, [sx], [sz], [x0], [wd], nthreads=1)[0]
+                assert np.array_equal(b.T(k), Tr), (sched, it, k)
+        bad = smooth_slowness(rng, nz, nx, dh)
+        bad[7, 7] = -1.0
+        with pytest.raises(capi.SeisfatError):
+            e.set_slowness(bad)
+        with pytest.raises(capi.SeisfatError) as ex:
+            b.run()
+        assert ex.value.code == capi.ESTATE
+        e.set_slowness(s)
+        b.close()
+    e.close()
+
+
 def ani_setup(nz, nx, dh, nshots, seed=7):
     from seisfat2d_amd.stiffness import compute_stiffness
     rng = np.random.default_rng(seed)
