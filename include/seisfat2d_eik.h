@@ -12,8 +12,8 @@
  *   - positions are in metres from the model origin, x along columns and z
  *     along rows, as in the reference's SPS/RPS files.
  *   - a handle is used by one host thread at a time; buffers are caller owned.
- *   - "stream" arguments are hipStream_t values passed as void* (NULL = the
- *     handle's own stream).
+ *   - "stream" arguments are hipStream_t values passed as void* (NULL = HIP's
+ *     default stream); work is ordered on that stream only.
  */
 #ifndef SEISFAT2D_EIK_H
 #define SEISFAT2D_EIK_H
@@ -125,6 +125,32 @@ int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms);
  * (a persistent sweep whose tile dependencies did not arrive in time). */
 int seisfat_batch_status(seisfat_batch_t b);
 int seisfat_batch_destroy(seisfat_batch_t b);
+
+/* Adjoint-state traveltime gradient (isotropic batches).  Replaces the
+ * ray-traced sensitivity of the reference's tomography
+ * (Inversion::gradient_ray_tracing, src/inversion/inversion.cpp:95-177, and
+ * Tomography_ISO::set_sensitivity_matrix, src/inversion/tomography_iso.cpp:
+ * 15-68) with the exact derivative of the single-pass solve.
+ *
+ * run_record: the batch's forward solve (same traveltimes and picks as
+ * seisfat_batch_run) followed by an fp64 solve of the same single-pass scheme
+ * whose trajectory (initial field and the state after each sweep) the
+ * adjoint linearises; asynchronous on `stream`.  The first call allocates the
+ * adjoint state (about 20 floats per padded node per shot). */
+int seisfat_batch_run_record(seisfat_batch_t b, void* stream);
+/* Vector-Jacobian product of the picks of the recorded run:
+ *   d_grad[iz + ix*nz] = sum_r d_w[r] * d pick_r / d s[iz + ix*nz]
+ * over the unpadded model (nz*nx doubles, overwritten; fp32 local derivatives,
+ * fp64 accumulation over shots).  d_w: nrec_total floats.  Device pointers,
+ * asynchronous on `stream`.  SEISFAT_ESTATE if there is no recorded run or the
+ * slowness changed since it. */
+int seisfat_batch_vjp_device(seisfat_batch_t b, const float* d_w, double* d_grad, void* stream);
+/* Misfit J = 1/2 sum_r (pick_r - obs_r)^2 and its gradient dJ/ds (nz*nx
+ * doubles) at the handle's current slowness: run_record, residuals and the
+ * vector-Jacobian product, host buffers, synchronous.  J and the residuals use
+ * the picks of the recorded fp64 trajectory (the map the gradient is exact
+ * for); picks (nullable) receives the batch's fp32 picks. */
+int seisfat_batch_misfit_gradient(seisfat_batch_t b, const float* obs, double* J, double* grad, float* picks);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,8 @@
 //                                  equal the source-receiver distance
 //   adjoint ...................... FP64 central finite differences of the
 //                                  FP64 single-pass forward
+//   batch_misfit ................. FP64 central finite differences of its own
+//                                  J over window shots (pins the window fold)
 //
 // Reference files restated (paths relative to the SeisFAT2D repository):
 //   src/modeling/modeling.cu, src/modeling/eikonal_iso.cu,
@@ -338,20 +340,24 @@ inline void cubic_weights(double d, double w[4]) {
 // ---------------------------------------------------------------------------
 // Discrete adjoint of one single-pass solve (init + 4 sweeps in raster order).
 // J = 1/2 sum_r (pick_r - obs_r)^2; returns J and accumulates dJ/dS (padded
-// grid, double) into g.  Tie rule: T_old wins when T_old <= candidate; within a
-// min of two slownesses the first argument wins on ties.
+// grid, double) into g.  Tie rules: T_old wins when T_old <= candidate; a min
+// of two exactly equal slownesses splits dS half and half between the two
+// cells (the derivative central differences converge to: such ties are
+// systematic wherever the model is constant along an axis).
 template <class R>
 struct NodeDeriv {
   int kind;  // 0 old, 1 d1z, 2 d1x, 3 branch A, 4 branch B, 5 branch C, 6 constant
   double dtv, dte, dtev;  // d(new)/d(neighbour)
   int scell;              // slowness cell index receiving dS
-  double ds;              // d(new)/dS[scell]
+  int scell2;             // second cell of an exact slowness tie, or -1
+  double ds;              // d(new)/dS[scell] (halved between scell and scell2)
 };
 
 template <class R>
 NodeDeriv<R> node_deriv(const R* T, const R* S, int i, int j, int nzz, int nxx, int a, int b,
                         const Consts<R>& c, R told) {
   NodeDeriv<R> d{};
+  d.scell2 = -1;
   const int i1 = i - (a + 1) / 2;
   const int j1 = j - (b + 1) / 2;
   const R tv = T[(i - a) + (size_t)j * nzz];
@@ -411,11 +417,13 @@ NodeDeriv<R> node_deriv(const R* T, const R* S, int i, int j, int nzz, int nxx, 
       d.kind = 1;
       d.dtv = 1.0;
       d.scell = (int)((S[cz_a] <= S[cz_b]) ? cz_a : cz_b);
+      if (S[cz_a] == S[cz_b]) d.scell2 = (int)cz_b;
       d.ds = (double)c.dz;
     } else {
       d.kind = 2;
       d.dte = 1.0;
       d.scell = (int)((S[cx_a] <= S[cx_b]) ? cx_a : cx_b);
+      if (S[cx_a] == S[cx_b]) d.scell2 = (int)cx_b;
       d.ds = (double)c.dx;
     }
     return d;
@@ -494,7 +502,12 @@ double adjoint(const R* S, int nzz, int nxx, R dx, R dz, R sx, R sz, const R* rx
         if (d.dtv != 0.0) L[(i - a) + (size_t)j * nzz] += lam * d.dtv;
         if (d.dte != 0.0) L[i + (size_t)(j - b) * nzz] += lam * d.dte;
         if (d.dtev != 0.0) L[(i - a) + (size_t)(j - b) * nzz] += lam * d.dtev;
-        g[d.scell] += lam * d.ds;
+        if (d.scell2 >= 0) {
+          g[d.scell] += lam * d.ds * 0.5;
+          g[d.scell2] += lam * d.ds * 0.5;
+        } else {
+          g[d.scell] += lam * d.ds;
+        }
       }
     }
   }
@@ -507,6 +520,71 @@ double adjoint(const R* S, int nzz, int nxx, R dx, R dz, R sx, R sz, const R* rx
       const R ddz = (R)(zi - NB) * dz - sz;
       g[index] += L[index] * (double)rsqrt_(ddx * ddx + ddz * ddz);
     }
+  return J;
+}
+
+// Batch of window shots with the batched GPU path's semantics: shot k solves
+// columns [x0[k], x0[k]+w[k]) of the unpadded model s (nz x nx), padded as a
+// standalone model, and picks receivers rec_off[k] .. rec_off[k]+nrec[k]-1
+// (absolute positions).  Returns J = 1/2 sum (pick - obs)^2 over all
+// receivers; when grad is non-null it also writes dJ/ds (nz*nx, unpadded):
+// each shot's padded gradient folded onto the model cells its padding
+// replicates.  Shots run in parallel; sums are taken in shot order.
+template <class R>
+double batch_misfit(const R* s, int nz, int nx, R dx, R dz, int nshots, const R* sx, const R* sz, const int* x0,
+                    const int* w, const int* rec_off, const int* nrec, const R* rx, const R* rz, const R* obs,
+                    double* grad, R* picks, int nthreads) {
+  std::vector<double> Jk(nshots, 0.0);
+  std::vector<std::vector<double>> gk(grad ? nshots : 0);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads != 1)
+  for (int k = 0; k < nshots; ++k) {
+    const int nzz = nz + 2 * NB, nxx = w[k] + 2 * NB;
+    std::vector<R> crop((size_t)nz * w[k]), S((size_t)nzz * nxx);
+    std::memcpy(crop.data(), s + (size_t)x0[k] * nz, crop.size() * sizeof(R));
+    expand_boundary(crop.data(), S.data(), nz, w[k]);
+    const R shift = (R)x0[k] * dx;
+    const int n = nrec[k], ro = rec_off[k];
+    std::vector<R> rxk(n), rzk(n), obk(n), pk(n);
+    for (int r = 0; r < n; ++r) {
+      rxk[r] = rx[ro + r] - shift;
+      rzk[r] = rz[ro + r];
+      obk[r] = obs[ro + r];
+    }
+    const R sxk = sx[k] - shift;
+    if (grad) {
+      std::vector<double> g((size_t)nzz * nxx, 0.0);
+      Jk[k] = adjoint(S.data(), nzz, nxx, dx, dz, sxk, sz[k], rxk.data(), rzk.data(), n, obk.data(), g.data(),
+                      pk.data());
+      std::vector<double>& out = gk[k];
+      out.assign((size_t)nz * w[k], 0.0);
+      for (int j = 0; j < nxx; ++j) {
+        const int jm = std::min(std::max(j - NB, 0), w[k] - 1);
+        for (int i = 0; i < nzz; ++i) {
+          const int im = std::min(std::max(i - NB, 0), nz - 1);
+          out[im + (size_t)jm * nz] += g[i + (size_t)j * nzz];
+        }
+      }
+    } else {
+      std::vector<R> T((size_t)nzz * nxx);
+      init_time(T.data(), S.data(), nzz, nxx, dx, dz, sxk, sz[k], nullptr, nullptr);
+      fsm(T.data(), S.data(), nzz, nxx, dx, dz, 0, 1, 0);
+      double J = 0.0;
+      for (int r = 0; r < n; ++r) {
+        pk[r] = pick_one(T.data(), nzz, dx, dz, rxk[r], rzk[r]);
+        const double res = (double)pk[r] - (double)obk[r];
+        J += 0.5 * res * res;
+      }
+      Jk[k] = J;
+    }
+    if (picks) std::copy(pk.begin(), pk.end(), picks + ro);
+  }
+  double J = 0.0;
+  for (int k = 0; k < nshots; ++k) J += Jk[k];
+  if (grad) {
+    std::fill(grad, grad + (size_t)nz * nx, 0.0);
+    for (int k = 0; k < nshots; ++k)
+      for (size_t c = 0; c < gk[k].size(); ++c) grad[(size_t)x0[k] * nz + c] += gk[k][c];
+  }
   return J;
 }
 
@@ -674,6 +752,19 @@ double orc_adjoint_f32(const float* S, int nzz, int nxx, float dx, float dz, flo
 double orc_adjoint_f64(const double* S, int nzz, int nxx, double dx, double dz, double sx, double sz, const double* rx,
                        const double* rz, int nrec, const double* obs, double* g, double* picks) {
   return adjoint(S, nzz, nxx, dx, dz, sx, sz, rx, rz, nrec, obs, g, picks);
+}
+
+double orc_batch_misfit_f32(const float* s, int nz, int nx, float dx, float dz, int nshots, const float* sx,
+                            const float* sz, const int* x0, const int* w, const int* rec_off, const int* nrec,
+                            const float* rx, const float* rz, const float* obs, double* grad, float* picks,
+                            int nthreads) {
+  return batch_misfit(s, nz, nx, dx, dz, nshots, sx, sz, x0, w, rec_off, nrec, rx, rz, obs, grad, picks, nthreads);
+}
+double orc_batch_misfit_f64(const double* s, int nz, int nx, double dx, double dz, int nshots, const double* sx,
+                            const double* sz, const int* x0, const int* w, const int* rec_off, const int* nrec,
+                            const double* rx, const double* rz, const double* obs, double* grad, double* picks,
+                            int nthreads) {
+  return batch_misfit(s, nz, nx, dx, dz, nshots, sx, sz, x0, w, rec_off, nrec, rx, rz, obs, grad, picks, nthreads);
 }
 
 int orc_ray_paths(const float* T, int nz, int nx, float dx, float dz, float sx, float sz, const float* rx,

@@ -56,6 +56,10 @@ def lib():
             "orc_adjoint_f32": (d, [f32p, i, i, f, f, f, f, f32p, f32p, i, f32p, f64p, f32p]),
             "orc_adjoint_f64": (d, [f64p, i, i, d, d, d, d, f64p, f64p, i, f64p, f64p, f64p]),
             "orc_ray_paths": (i, [f32p, i, i, f, f, f, f, f32p, f32p, i, i, i32p, i32p, f32p, i]),
+            "orc_batch_misfit_f32": (d, [f32p, i, i, f, f, i, f32p, f32p, i32p, i32p, i32p, i32p, f32p, f32p,
+                                         f32p, C.c_void_p, C.c_void_p, i]),
+            "orc_batch_misfit_f64": (d, [f64p, i, i, d, d, i, f64p, f64p, i32p, i32p, i32p, i32p, f64p, f64p,
+                                         f64p, C.c_void_p, C.c_void_p, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -190,6 +194,24 @@ def adjoint(S, dx, dz, sx, sz, rx, rz, obs):
     J = getattr(lib(), "orc_adjoint_" + _suffix(dt))(
         S, nzz, nxx, dx, dz, sx, sz, np.ascontiguousarray(rx, dt), np.ascontiguousarray(rz, dt),
         len(rx), np.ascontiguousarray(obs, dt), g, pk)
+    return J, g, pk
+
+
+def batch_misfit(s, dx, dz, shots, rx, rz, obs, grad=True, nthreads=16):
+    """Window-shot batch (GPU batch semantics) on an unpadded (nx, nz) model of
+    dtype float32 or float64.  shots: (sx, sz, x0, width, rec_offset, nrec)
+    tuples.  Returns (J, dJ/ds as (nx, nz) float64 or None, picks)."""
+    s = np.ascontiguousarray(s)
+    dt = s.dtype
+    nx, nz = s.shape
+    col = lambda k, t: np.ascontiguousarray([sh[k] for sh in shots], t)
+    g = np.zeros((nx, nz), np.float64) if grad else None
+    pk = np.zeros(len(rx), dt)
+    J = getattr(lib(), "orc_batch_misfit_" + _suffix(dt))(
+        s, nz, nx, dx, dz, len(shots), col(0, dt), col(1, dt), col(2, np.int32), col(3, np.int32),
+        col(4, np.int32), col(5, np.int32), np.ascontiguousarray(rx, dt), np.ascontiguousarray(rz, dt),
+        np.ascontiguousarray(obs, dt), None if g is None else g.ctypes.data_as(C.c_void_p),
+        pk.ctypes.data_as(C.c_void_p), nthreads)
     return J, g, pk
 
 

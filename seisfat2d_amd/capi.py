@@ -23,7 +23,8 @@ EXPORTS = [
     "seisfat_eik_device_T", "seisfat_eik_copy_T", "seisfat_batch_create", "seisfat_batch_run", "seisfat_batch_picks",
     "seisfat_batch_device_picks", "seisfat_batch_device_T", "seisfat_batch_copy_T",
     "seisfat_batch_num_launches", "seisfat_batch_num_sweep_launches", "seisfat_batch_sweep_ms",
-    "seisfat_batch_status", "seisfat_batch_destroy",
+    "seisfat_batch_status", "seisfat_batch_destroy", "seisfat_batch_run_record", "seisfat_batch_vjp_device",
+    "seisfat_batch_misfit_gradient",
 ]
 
 
@@ -73,6 +74,9 @@ def lib():
         L.seisfat_batch_sweep_ms.argtypes = [vp, C.POINTER(f)]
         L.seisfat_batch_destroy.argtypes = [vp]
         L.seisfat_batch_status.argtypes = [vp]
+        L.seisfat_batch_run_record.argtypes = [vp, vp]
+        L.seisfat_batch_vjp_device.argtypes = [vp, vp, vp, vp]
+        L.seisfat_batch_misfit_gradient.argtypes = [vp, vp, vp, vp, vp]
         ptr_ret = {"seisfat_last_error", "seisfat_eik_device_T", "seisfat_batch_device_picks",
                    "seisfat_batch_device_T"}
         for name in EXPORTS:

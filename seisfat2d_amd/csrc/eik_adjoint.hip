@@ -1,0 +1,533 @@
+// Adjoint-state (reverse-mode) gradient kernels of the single-pass eikonal
+// solve.  The forward is SeisFAT2D's time_init + one pass of four sweeps
+// (src/modeling/modeling.cu:110-159, local solver :308-357); the gradient is
+// the exact derivative of that discrete map, so it is what a finite
+// difference of the forward converges to.  The reference's tomography uses a
+// ray-traced sensitivity instead (src/inversion/inversion.cpp:95-177), which
+// this gradient replaces.
+//
+// Branch decisions and local derivatives come from an fp64 trajectory of the
+// solve recorded on the device; the adjoint recurrence runs in fp32 and the
+// cross-shot accumulation in fp64.  Every reduction runs in a fixed order, so
+// the gradient is deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "eik_adjoint.h"
+
+namespace seisfat {
+
+// ---------------------------------------------------------------------------
+__global__ void k_fill_d(double* __restrict__ p, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_adj_seed(const size_t* __restrict__ node, const int* __restrict__ ptr,
+                           const int* __restrict__ rec, const float* __restrict__ wt, int nseed,
+                           const float* __restrict__ w, float* __restrict__ lam) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nseed) return;
+  float acc = 0.0f;
+  for (int e = ptr[n]; e < ptr[n + 1]; ++e) acc += w[rec[e]] * wt[e];
+  lam[node[n]] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// The recorded trajectory is an fp64 solve of the same single-pass scheme:
+// fp32 rounding decides near-tied branch choices (two sweeps' candidates
+// agreeing to a few ulps, common where the wave travels along a grid axis),
+// and the derivative of the fp32 map then differs locally from the derivative
+// the fp64 map (and finite differences of it) has.  Decisions and partials
+// are taken from the fp64 trajectory; the adjoint recurrence runs in fp32.
+struct ConstsD {
+  double dx, dz, dx2i, dz2i, sum, prod, hyp;
+};
+
+__device__ __forceinline__ double dpp_shr1_d(double old, double src) {
+  const long long o = __double_as_longlong(old), s = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)s, 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(s >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The local solver (src/modeling/modeling.cu:308-357) in fp64, operation for
+// operation as the fp64 oracle evaluates it.
+__device__ __forceinline__ double local_d(double tv, double te, double tev, double sref, double szo, double sxo,
+                                          const ConstsD& k) {
+  const double t1d1 = tv + k.dz * fmin(sref, szo);
+  const double t1d2 = te + k.dx * fmin(sref, sxo);
+  const double t1D = fmin(t1d1, t1d2);
+  double t2D = 1e6;
+  if ((tv <= te + k.dx * sref) && (te <= tv + k.dz * sref) && (te - tev >= 0.0) && (tv - tev >= 0.0)) {
+    const double ta = tev + te - tv;
+    const double tb = tev - te + tv;
+    t2D = ((tb * k.dz2i + ta * k.dx2i) + sqrt(4.0 * sref * sref * k.sum - k.prod * (ta - tb) * (ta - tb))) / k.sum;
+  } else if ((te - tev <= sref * k.dz * k.dz / k.hyp) && (te - tev > 0.0)) {
+    t2D = te + k.dx * sqrt(sref * sref - ((te - tev) / k.dz) * ((te - tev) / k.dz));
+  } else if ((tv - tev <= sref * k.dx * k.dx / k.hyp) && (tv - tev > 0.0)) {
+    t2D = tv + k.dz * sqrt(sref * sref - ((tv - tev) / k.dx) * ((tv - tev) / k.dx));
+  }
+  return fmin(t1D, fmin(t2D, 1e6));
+}
+
+// time_set + time_init in fp64 (T0 holds 1e6 elsewhere for good).
+__global__ void k_fwd64_init(const AdjShot* __restrict__ shots, int nshots, int nzz, double dx, double dz,
+                             double* __restrict__ T0) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nshots * 9) return;
+  const AdjShot sh = shots[t / 9];
+  const int ii = (t % 9) % 3, jj = (t % 9) / 3;
+  const double sx = sh.sd.sx, sz = sh.sd.sz;
+  const int sIdx = (int)((sx + 0.5 * dx) / dx) + NB;
+  const int sIdz = (int)((sz + 0.5 * dz) / dz) + NB;
+  const int zi = sIdz + (ii - 1), xi = sIdx + (jj - 1);
+  const double ddx = (double)(xi - NB) * dx - sx;
+  const double ddz = (double)(zi - NB) * dz - sz;
+  T0[sh.toff + zi + (size_t)xi * nzz] = (double)s_at(sh.sd, nzz, zi, xi) * sqrt(ddx * ddx + ddz * ddz);
+}
+
+// One fp64 sweep, Tnew = sweep(Told), out of place (ring nodes of Tnew hold
+// 1e6 for good).  One wave per shot walks the strips of 64 sweep-local rows;
+// lane k holds row 64r + k and visits column t - k at step t: its z-upwind
+// and diagonal neighbours are lane k-1's results one and two steps back (DPP
+// shift), its x-upwind neighbour its own previous result, and lane 0 reads
+// the previous strip's row from Tnew.  Inputs are loaded ADJ_PF steps ahead.
+template <int SA, int SB>
+__global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ shots, int nzz, int nstr,
+                                                    ConstsD K, const double* __restrict__ Told,
+                                                    double* __restrict__ Tnew) {
+  const AdjShot sh = shots[blockIdx.x];
+  const int lane = threadIdx.x, nxx = sh.sd.nxx;
+  const double* __restrict__ To = Told + sh.toff;
+  double* __restrict__ Tn = Tnew + sh.toff;
+  const int NT = (nxx + 63 + ADJ_PF - 1) / ADJ_PF * ADJ_PF;
+  for (int r = 0; r < nstr; ++r) {
+    const int fi = 64 * r + lane;
+    const bool row_int = fi >= 1 && fi <= nzz - 2;
+    const int fic = min(max(fi, 1), nzz - 2);
+    const int i = SA > 0 ? fic : nzz - 1 - fic;
+    const int i1 = i - (SA + 1) / 2;
+    const bool gl = lane == 0 && r > 0;
+    double qto[ADJ_PF], qgv[ADJ_PF], qge[ADJ_PF];
+    float qsr[ADJ_PF], qsz[ADJ_PF], qsx[ADJ_PF];
+    // inputs of step t at clamped (always valid) coordinates; only interior
+    // nodes use them
+    auto load = [&](int t, double& to, double& gv, double& ge, float& sr, float& sz, float& sx) {
+      const int fjc = min(max(t - lane, 1), nxx - 2);
+      const int j = SB > 0 ? fjc : nxx - 1 - fjc;
+      const int j1 = j - (SB + 1) / 2;
+      to = To[i + (size_t)j * nzz];
+      sr = s_at(sh.sd, nzz, i1, j1);
+      sz = s_at(sh.sd, nzz, i1, j1 + SB);
+      sx = s_at(sh.sd, nzz, i1 + SA, j1);
+      gv = 0.0;
+      ge = 0.0;
+      if (gl) {
+        gv = Tn[(i - SA) + (size_t)j * nzz];
+        ge = Tn[(i - SA) + (size_t)(j - SB) * nzz];
+      }
+    };
+#pragma unroll
+    for (int u = 0; u < ADJ_PF; ++u) load(u, qto[u], qgv[u], qge[u], qsr[u], qsz[u], qsx[u]);
+    double prev = 1e6, prev2 = 1e6;
+    for (int t0 = 0; t0 < NT; t0 += ADJ_PF) {
+#pragma unroll
+      for (int u = 0; u < ADJ_PF; ++u) {
+        const int t = t0 + u;
+        const double to = qto[u], gv = qgv[u], ge = qge[u];
+        const float sr = qsr[u], sz = qsz[u], sx = qsx[u];
+        load(min(t + ADJ_PF, NT - 1), qto[u], qgv[u], qge[u], qsr[u], qsz[u], qsx[u]);
+        const double tv = dpp_shr1_d(gv, prev);
+        const double tev = dpp_shr1_d(ge, prev2);
+        const int fj = t - lane;
+        double out = 1e6;
+        if (row_int && fj >= 1 && fj <= nxx - 2) {
+          out = fmin(to, local_d(tv, prev, tev, sr, sz, sx, K));
+          const int j = SB > 0 ? fj : nxx - 1 - fj;
+          Tn[i + (size_t)j * nzz] = out;
+        }
+        prev2 = prev;
+        prev = out;
+      }
+    }
+    // the next strip's lane 0 reads this strip's last row
+    __threadfence_block();
+  }
+}
+
+// Catmull-Rom picks of the fp64 trajectory's final traveltimes (the residuals
+// of the misfit gradient), as the fp64 oracle evaluates them.
+__device__ __forceinline__ double cubic1d_d(const double P[4], double d) {
+  return P[1] + 0.5 * d * (P[2] - P[0] + d * (2.0 * P[0] - 5.0 * P[1] + 4.0 * P[2] - P[3] +
+                                             d * (3.0 * (P[1] - P[2]) + P[3] - P[0])));
+}
+
+__global__ void k_picks64(const AdjShot* __restrict__ shots, const int* __restrict__ rec_shot,
+                          const float* __restrict__ rx, const float* __restrict__ rz, int nrec, int nzz, double dx,
+                          double dz, const double* __restrict__ T4, double* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  const double* __restrict__ T = T4 + shots[rec_shot[r]].toff;
+  const double x = rx[r], z = rz[r];
+  const double x0 = floor((x + 0.5 * dx) / dx) * dx;
+  const double z0 = floor((z + 0.5 * dz) / dz) * dz;
+  const double x1 = floor((x + 0.5 * dx) / dx) * dx + dx;
+  const double z1 = floor((z + 0.5 * dz) / dz) * dz + dz;
+  const double xd = (x - x0) / (x1 - x0);
+  const double zd = (z - z0) / (z1 - z0);
+  const int i = (int)((z + 0.5 * dz) / dz) + NB;
+  const int j = (int)((x + 0.5 * dx) / dx) + NB;
+  double p[4];
+  for (int pIdx = 0; pIdx < 4; ++pIdx) {
+    double P[4];
+    for (int pIdz = 0; pIdz < 4; ++pIdz) P[pIdz] = T[(i + pIdz - 1) + (size_t)(j + pIdx - 1) * nzz];
+    p[pIdx] = cubic1d_d(P, zd);
+  }
+  out[r] = cubic1d_d(p, xd);
+}
+
+// ---------------------------------------------------------------------------
+// Local partials of one node.  ds < 0 marks a node the sweep left unchanged
+// (T_old kept: the adjoint passes through); bit (di | dj << 1) of code marks
+// slowness cell (i - di, j - dj) as receiving ds, split evenly when two bits
+// are set.
+struct AdjCoef {
+  float dtv, dte, dtev, ds;
+  int code;
+};
+
+// The branch the fp64 forward took at a node (same expressions and
+// comparisons as local_d) and its partial derivatives.  Ties follow the
+// oracle: T_old wins when T_old <= candidate; 1-D before 2-D when equal;
+// z-edge before x-edge when equal; min(S, S') of two equal cells splits dS
+// half and half (what central differences measure).
+template <int SA, int SB>
+__device__ __forceinline__ AdjCoef adj_local(double tv, double te, double tev, double sref, double szo,
+                                             double sxo, double told, const ConstsD& k) {
+  constexpr int di_ref = (SA + 1) / 2, dj_ref = (SB + 1) / 2;
+  constexpr int c_ref = di_ref | (dj_ref << 1);
+  constexpr int c_zo = di_ref | (((1 - SB) / 2) << 1);
+  constexpr int c_xo = ((1 - SA) / 2) | (dj_ref << 1);
+  const double t1d1 = tv + k.dz * fmin(sref, szo);
+  const double t1d2 = te + k.dx * fmin(sref, sxo);
+  const double t1D = fmin(t1d1, t1d2);
+  double tbr = 1e6;
+  bool taken = false;
+  double p_tv = 0.0, p_te = 0.0, p_tev = 0.0, p_s = 0.0;
+  if ((tv <= te + k.dx * sref) && (te <= tv + k.dz * sref) && (te - tev >= 0.0) && (tv - tev >= 0.0)) {
+    const double ta = tev + te - tv;
+    const double tb = tev - te + tv;
+    const double sq = sqrt(4.0 * sref * sref * k.sum - k.prod * (ta - tb) * (ta - tb));
+    tbr = ((tb * k.dz2i + ta * k.dx2i) + sq) / k.sum;
+    taken = true;
+    // t = ((tb dz2i + ta dx2i) + sqrt(4 S^2 D - dz2i dx2i (ta - tb)^2)) / D
+    const double kk = 2.0 * k.prod * (ta - tb) / sq;
+    p_te = (k.dx2i - k.dz2i - kk) / k.sum;
+    p_tv = (k.dz2i - k.dx2i + kk) / k.sum;
+    p_tev = 1.0;
+    p_s = 4.0 * sref / sq;
+  } else if ((te - tev <= sref * k.dz * k.dz / k.hyp) && (te - tev > 0.0)) {
+    // t = te + dx sqrt(S^2 - q^2), q = (te - tev) / dz
+    const double q = (te - tev) / k.dz;
+    const double Q = sqrt(sref * sref - q * q);
+    tbr = te + k.dx * Q;
+    taken = true;
+    const double r = k.dx * q / (k.dz * Q);
+    p_te = 1.0 - r;
+    p_tev = r;
+    p_s = k.dx * sref / Q;
+  } else if ((tv - tev <= sref * k.dx * k.dx / k.hyp) && (tv - tev > 0.0)) {
+    const double p = (tv - tev) / k.dx;
+    const double P = sqrt(sref * sref - p * p);
+    tbr = tv + k.dz * P;
+    taken = true;
+    const double r = k.dz * p / (k.dx * P);
+    p_tv = 1.0 - r;
+    p_tev = r;
+    p_s = k.dz * sref / P;
+  }
+  const double t2D = fmin(tbr, 1e6);
+  const double cand = fmin(t1D, t2D);
+  AdjCoef c{0.0f, 0.0f, 0.0f, -1.0f, 0};
+  if (!(cand < told)) return c;
+  if (t1D <= t2D) {
+    if (t1d1 <= t1d2) {
+      c.dtv = 1.0f;
+      c.ds = (float)k.dz;
+      c.code = sref == szo ? (1 << c_ref) | (1 << c_zo) : sref < szo ? 1 << c_ref : 1 << c_zo;
+    } else {
+      c.dte = 1.0f;
+      c.ds = (float)k.dx;
+      c.code = sref == sxo ? (1 << c_ref) | (1 << c_xo) : sref < sxo ? 1 << c_ref : 1 << c_xo;
+    }
+    return c;
+  }
+  if (!taken || !(tbr < 1e6)) {  // the 1e6 constant won: no dependence
+    c.ds = 0.0f;
+    return c;
+  }
+  c.dtv = (float)p_tv;
+  c.dte = (float)p_te;
+  c.dtev = (float)p_tev;
+  c.ds = (float)p_s;
+  c.code = 1 << c_ref;
+  return c;
+}
+
+template <int SA, int SB>
+__global__ __launch_bounds__(256) void k_adj_coef(const AdjShot* __restrict__ shots, int nzz, ConstsD k,
+                                                  const double* __restrict__ Tnew, const double* __restrict__ Told,
+                                                  int pa, int pb, const float* __restrict__ lam_src,
+                                                  float* __restrict__ lam_dst, float4* __restrict__ coef,
+                                                  uint8_t* __restrict__ code) {
+  const AdjShot sh = shots[blockIdx.y];
+  const int nxx = sh.sd.nxx;
+  const int n = nzz * nxx;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int i = idx % nzz, j = idx / nzz;
+  const size_t ko = sh.koff + adj_kidx(adj_ip(i, SA, nzz), adj_jp(j, SB, nxx), sh.NTP);
+  if (lam_src) lam_dst[ko] = lam_src[sh.koff + adj_kidx(adj_ip(i, pa, nzz), adj_jp(j, pb, nxx), sh.NTP)];
+  AdjCoef c{0.0f, 0.0f, 0.0f, -1.0f, 0};
+  if (i >= 1 && i <= nzz - 2 && j >= 1 && j <= nxx - 2) {
+    const double* __restrict__ Tn = Tnew + sh.toff;
+    const double tv = Tn[(i - SA) + (size_t)j * nzz];
+    const double te = Tn[i + (size_t)(j - SB) * nzz];
+    const double tev = Tn[(i - SA) + (size_t)(j - SB) * nzz];
+    const double told = Told[sh.toff + idx];
+    const int i1 = i - (SA + 1) / 2, j1 = j - (SB + 1) / 2;
+    const double sref = s_at(sh.sd, nzz, i1, j1);
+    const double szo = s_at(sh.sd, nzz, i1, j1 + SB);
+    const double sxo = s_at(sh.sd, nzz, i1 + SA, j1);
+    c = adj_local<SA, SB>(tv, te, tev, sref, szo, sxo, told, k);
+  }
+  coef[ko] = make_float4(c.dtv, c.dte, c.dtev, c.ds);
+  code[ko] = (uint8_t)c.code;
+}
+
+// ---------------------------------------------------------------------------
+// Reverse sweep: one wave per shot walks the strips of 64 reverse-local rows
+// one after another; lane k holds row 64r + k and visits column t - k at step
+// t.  The z-upwind and diagonal messages come from lane k-1 (DPP shift), the
+// x message from the lane's own previous step, and lane 0 reads the previous
+// strip's lane-63 messages from xmsg (slot r; slot 0 stays zero).  Inputs of
+// ADJ_PF steps ahead are kept in flight.  Nodes outside the grid hold zero
+// inputs and produce zero outputs.
+__global__ __launch_bounds__(64) void k_adj_sweep(const AdjShot* __restrict__ shots, int nstr,
+                                                  const float* __restrict__ lam_in,
+                                                  const float4* __restrict__ coef, float* __restrict__ lam_out,
+                                                  float* __restrict__ gc, float* xmsg) {
+  const AdjShot sh = shots[blockIdx.x];
+  const int lane = threadIdx.x;
+  const int NTP = sh.NTP, nxx = sh.sd.nxx;
+  for (int r = 0; r < nstr; ++r) {
+    const size_t base = sh.koff + (size_t)r * NTP * 64 + lane;
+    float* xin = xmsg + sh.xoff + (size_t)r * sh.xrs;
+    float* xout = xmsg + sh.xoff + (size_t)(r + 1) * sh.xrs;
+    float ql[ADJ_PF], qv[ADJ_PF], qe[ADJ_PF];
+    float4 qc[ADJ_PF];
+#pragma unroll
+    for (int u = 0; u < ADJ_PF; ++u) {
+      ql[u] = lam_in[base + (size_t)u * 64];
+      qc[u] = coef[base + (size_t)u * 64];
+      qv[u] = xin[u + 1];
+      qe[u] = xin[NTP + 1 + u];
+    }
+    float mv = 0.0f, me = 0.0f, mev = 0.0f, mevp = 0.0f;
+    for (int t0 = 0; t0 < NTP; t0 += ADJ_PF) {
+#pragma unroll
+      for (int u = 0; u < ADJ_PF; ++u) {
+        const int t = t0 + u;
+        const float lam = ql[u], xv = qv[u], xe = qe[u];
+        const float4 cf = qc[u];
+        const int tn = min(t + ADJ_PF, NTP - 1);
+        ql[u] = lam_in[base + (size_t)tn * 64];
+        qc[u] = coef[base + (size_t)tn * 64];
+        qv[u] = xin[tn + 1];
+        qe[u] = xin[NTP + 1 + tn];
+        // messages into (ip, jp): from (ip-1, jp) [lane k-1, step t-1],
+        // (ip, jp-1) [own lane, step t-1], (ip-1, jp-1) [lane k-1, step t-2]
+        const float in_v = dpp_shr1(xv, mv);
+        const float in_ev = dpp_shr1(xe, mevp);
+        const float mu = ((lam + in_v) + me) + in_ev;
+        mevp = mev;
+        const bool upd = cf.w >= 0.0f;
+        mv = upd ? mu * cf.x : 0.0f;
+        me = upd ? mu * cf.y : 0.0f;
+        mev = upd ? mu * cf.z : 0.0f;
+        lam_out[base + (size_t)t * 64] = upd ? 0.0f : mu;
+        gc[base + (size_t)t * 64] = upd ? mu * cf.w : 0.0f;
+        const int jp = t - 63;
+        if (lane == 63 && jp >= 0 && jp < nxx) {
+          xout[jp + 1] = mv;
+          xout[NTP + 1 + jp + 1] = mev;
+        }
+      }
+    }
+    // the next strip's lane 0 reads what lane 63 stored
+    __threadfence_block();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Slowness contributions of sweep s onto padded cell (i, j): the nodes
+// (i + di, j + dj) whose code marks offset (di, dj), in a fixed order.
+template <int SA, int SB>
+__global__ __launch_bounds__(256) void k_adj_fold(const AdjShot* __restrict__ shots, int nzz,
+                                                  const float* __restrict__ gc, const uint8_t* __restrict__ code,
+                                                  float* __restrict__ gshot) {
+  const AdjShot sh = shots[blockIdx.y];
+  const int nxx = sh.sd.nxx;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nzz * nxx) return;
+  const int i = idx % nzz, j = idx / nzz;
+  float acc = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int di = c & 1, dj = c >> 1;
+    const int ni = i + di, nj = j + dj;
+    if (ni >= 1 && ni <= nzz - 2 && nj >= 1 && nj <= nxx - 2) {
+      const size_t ko = sh.koff + adj_kidx(adj_ip(ni, SA, nzz), adj_jp(nj, SB, nxx), sh.NTP);
+      const int m = code[ko];
+      if (m & (1 << c)) acc += (m & (m - 1)) ? 0.5f * gc[ko] : gc[ko];
+    }
+  }
+  gshot[sh.toff + idx] += acc;
+}
+
+// T0 = S(node) * distance on the source stencil: dT0/dS(node) = distance.
+__global__ void k_adj_source(const AdjShot* __restrict__ shots, int nshots, int nzz, double dx, double dz,
+                             const float* __restrict__ lam0, float* __restrict__ gshot) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nshots * 9) return;
+  const AdjShot sh = shots[t / 9];
+  const int ii = (t % 9) % 3, jj = (t % 9) / 3;
+  const int sIdx = (int)(((double)sh.sd.sx + 0.5 * dx) / dx) + NB;
+  const int sIdz = (int)(((double)sh.sd.sz + 0.5 * dz) / dz) + NB;
+  const int zi = sIdz + (ii - 1), xi = sIdx + (jj - 1);
+  const double ddx = (double)(xi - NB) * dx - (double)sh.sd.sx;
+  const double ddz = (double)(zi - NB) * dz - (double)sh.sd.sz;
+  const float lam = lam0[sh.koff + adj_kidx(adj_ip(zi, 1, nzz), adj_jp(xi, 1, sh.sd.nxx), sh.NTP)];
+  gshot[sh.toff + zi + (size_t)xi * nzz] += lam * (float)sqrt(ddx * ddx + ddz * ddz);
+}
+
+// d/ds of the unpadded model: every padded node of every shot maps to the
+// model cell it replicates (rows clamped to the model, columns clamped to the
+// shot's window); sums in fp64, shots in order.
+__global__ __launch_bounds__(256) void k_adj_reduce(const AdjShot* __restrict__ shots, int nshots, int nz,
+                                                    int nx, int nzz, const float* __restrict__ gshot,
+                                                    double* __restrict__ grad) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nz * nx) return;
+  const int iz = idx % nz, ix = idx / nz;
+  const int plo = iz == 0 ? 0 : iz + NB;
+  const int phi = iz == nz - 1 ? nz - 1 + 2 * NB : iz + NB;
+  double acc = 0.0;
+  for (int s = 0; s < nshots; ++s) {
+    const int x0 = shots[s].sd.x0, w = shots[s].sd.w;
+    if (ix < x0 || ix >= x0 + w) continue;
+    const int lq = ix - x0;
+    const int qlo = lq == 0 ? 0 : lq + NB;
+    const int qhi = lq == w - 1 ? w - 1 + 2 * NB : lq + NB;
+    const float* __restrict__ g = gshot + shots[s].toff;
+    for (int q = qlo; q <= qhi; ++q)
+      for (int p = plo; p <= phi; ++p) acc += (double)g[p + (size_t)q * nzz];
+  }
+  grad[idx] = acc;
+}
+
+// ---------------------------------------------------------------------------
+void launch_fill_d(double* p, size_t n, double v, hipStream_t st) {
+  const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_fill_d, dim3(blocks), dim3(256), 0, st, p, n, v);
+}
+
+void launch_adj_seed(const size_t* node, const int* ptr, const int* rec, const float* wt, int nseed,
+                     const float* w, float* lam, hipStream_t st) {
+  if (nseed <= 0) return;
+  hipLaunchKernelGGL(k_adj_seed, dim3((nseed + 255) / 256), dim3(256), 0, st, node, ptr, rec, wt, nseed, w, lam);
+}
+
+static ConstsD consts_d(double dx, double dz) {
+  ConstsD k;
+  k.dx = dx;
+  k.dz = dz;
+  k.dz2i = 1.0 / (dz * dz);
+  k.dx2i = 1.0 / (dx * dx);
+  k.sum = k.dz2i + k.dx2i;
+  k.prod = k.dz2i * k.dx2i;
+  k.hyp = std::sqrt(dx * dx + dz * dz);
+  return k;
+}
+
+void launch_fwd64_init(const AdjShot* shots, int nshots, int nzz, double dx, double dz, double* T0, hipStream_t st) {
+  const int n = nshots * 9;
+  hipLaunchKernelGGL(k_fwd64_init, dim3((n + 255) / 256), dim3(256), 0, st, shots, nshots, nzz, dx, dz, T0);
+}
+
+void launch_fwd64_sweep(int sweep, const AdjShot* shots, int nshots, int nzz, int nstr, double dx, double dz,
+                        const double* Told, double* Tnew, hipStream_t st) {
+  const ConstsD k = consts_d(dx, dz);
+  switch (sweep) {
+    case 0: hipLaunchKernelGGL((k_fwd64_sweep<1, 1>), dim3(nshots), dim3(64), 0, st, shots, nzz, nstr, k, Told, Tnew); break;
+    case 1: hipLaunchKernelGGL((k_fwd64_sweep<-1, 1>), dim3(nshots), dim3(64), 0, st, shots, nzz, nstr, k, Told, Tnew); break;
+    case 2: hipLaunchKernelGGL((k_fwd64_sweep<1, -1>), dim3(nshots), dim3(64), 0, st, shots, nzz, nstr, k, Told, Tnew); break;
+    default: hipLaunchKernelGGL((k_fwd64_sweep<-1, -1>), dim3(nshots), dim3(64), 0, st, shots, nzz, nstr, k, Told, Tnew); break;
+  }
+}
+
+void launch_picks64(const AdjShot* shots, const int* rec_shot, const float* rx, const float* rz, int nrec, int nzz,
+                    double dx, double dz, const double* T4, double* out, hipStream_t st) {
+  if (nrec <= 0) return;
+  hipLaunchKernelGGL(k_picks64, dim3((nrec + 255) / 256), dim3(256), 0, st, shots, rec_shot, rx, rz, nrec, nzz, dx,
+                     dz, T4, out);
+}
+
+void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int max_nodes, int nzz, double dx, double dz,
+                     const double* Tnew, const double* Told, const float* lam_src, float* lam_dst, float4* coef,
+                     uint8_t* code, hipStream_t st) {
+  const dim3 grid((max_nodes + 255) / 256, nshots);
+  const int pa = adj_a(sweep + 1), pb = adj_b(sweep + 1);
+  const ConstsD k = consts_d(dx, dz);
+  switch (sweep) {
+    case 0: hipLaunchKernelGGL((k_adj_coef<1, 1>), grid, dim3(256), 0, st, shots, nzz, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
+    case 1: hipLaunchKernelGGL((k_adj_coef<-1, 1>), grid, dim3(256), 0, st, shots, nzz, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
+    case 2: hipLaunchKernelGGL((k_adj_coef<1, -1>), grid, dim3(256), 0, st, shots, nzz, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
+    default: hipLaunchKernelGGL((k_adj_coef<-1, -1>), grid, dim3(256), 0, st, shots, nzz, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
+  }
+}
+
+void launch_adj_sweep(const AdjShot* shots, int nshots, int nstr, const float* lam_in, const float4* coef,
+                      float* lam_out, float* gc, float* xmsg, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_sweep, dim3(nshots), dim3(64), 0, st, shots, nstr, lam_in, coef, lam_out, gc, xmsg);
+}
+
+void launch_adj_fold(int sweep, const AdjShot* shots, int nshots, int max_nodes, int nzz, const float* gc,
+                     const uint8_t* code, float* gshot, hipStream_t st) {
+  const dim3 grid((max_nodes + 255) / 256, nshots);
+  switch (sweep) {
+    case 0: hipLaunchKernelGGL((k_adj_fold<1, 1>), grid, dim3(256), 0, st, shots, nzz, gc, code, gshot); break;
+    case 1: hipLaunchKernelGGL((k_adj_fold<-1, 1>), grid, dim3(256), 0, st, shots, nzz, gc, code, gshot); break;
+    case 2: hipLaunchKernelGGL((k_adj_fold<1, -1>), grid, dim3(256), 0, st, shots, nzz, gc, code, gshot); break;
+    default: hipLaunchKernelGGL((k_adj_fold<-1, -1>), grid, dim3(256), 0, st, shots, nzz, gc, code, gshot); break;
+  }
+}
+
+void launch_adj_source(const AdjShot* shots, int nshots, int nzz, double dx, double dz, const float* lam0,
+                       float* gshot, hipStream_t st) {
+  const int n = nshots * 9;
+  hipLaunchKernelGGL(k_adj_source, dim3((n + 255) / 256), dim3(256), 0, st, shots, nshots, nzz, dx, dz, lam0,
+                     gshot);
+}
+
+void launch_adj_reduce(const AdjShot* shots, int nshots, int nz, int nx, int nzz, const float* gshot,
+                       double* grad, hipStream_t st) {
+  const int n = nz * nx;
+  hipLaunchKernelGGL(k_adj_reduce, dim3((n + 255) / 256), dim3(256), 0, st, shots, nshots, nz, nx, nzz, gshot,
+                     grad);
+}
+
+}  // namespace seisfat

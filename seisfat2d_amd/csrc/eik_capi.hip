@@ -8,6 +8,7 @@
 // solve.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "../../include/seisfat2d_eik.h"
+#include "eik_adjoint.h"
 #include "eik_kernels.h"
 
 using namespace seisfat;
@@ -68,6 +70,7 @@ struct seisfat_eik_s {
   hipStream_t stream = nullptr;
   float* d_szp = nullptr;  // row-padded slowness, nzz x nx
   bool has_s = false;
+  unsigned version = 0;  // incremented by every accepted slowness upload
   uint16_t* d_c[6] = {};
   float cmin[6] = {}, cmax[6] = {};
   bool has_c = false;
@@ -77,6 +80,33 @@ struct seisfat_eik_s {
   int KP = 0, NS = 0;
   SkewView skew[4] = {};
   seisfat_batch_s* single = nullptr;  // batch of the last seisfat_eik_solve
+};
+
+// Adjoint state of a batch, allocated on its first recorded run.
+struct AdjState {
+  bool ready = false;
+  bool recorded = false;  // a recorded run exists for model `version`
+  unsigned version = 0;
+  std::vector<AdjShot> tab;
+  AdjShot* d_tab = nullptr;
+  int nstr = 0, max_nodes = 0;
+  size_t ksize = 0, xsize = 0;
+  double* d_T64 = nullptr;  // fp64 trajectory: T0 (initial field) .. T4, tsize each
+  float* d_lamA = nullptr;
+  float* d_lamB = nullptr;
+  float4* d_coef = nullptr;
+  uint8_t* d_code = nullptr;
+  float* d_gc = nullptr;
+  float* d_gshot = nullptr;
+  float* d_xmsg = nullptr;
+  size_t* d_seed_node = nullptr;
+  int* d_seed_ptr = nullptr;
+  int* d_seed_rec = nullptr;
+  float* d_seed_w = nullptr;
+  int nseed = 0;
+  double* d_picks64 = nullptr;  // picks of the fp64 trajectory
+  float* d_w = nullptr;      // cotangents of the host-side misfit call
+  double* d_grad = nullptr;  // gradient of the host-side misfit call
 };
 
 struct seisfat_batch_s {
@@ -98,6 +128,10 @@ struct seisfat_batch_s {
   int4* d_tiles = nullptr;
   int ntiles = 0, nstrips = 0;
   std::vector<int> diag_off;
+  // receivers relative to their shot's grid origin, and their shots
+  std::vector<float> rxr, rzr;
+  std::vector<int> recshot;
+  AdjState adj;
   // persistent schedule state
   unsigned long long* d_xch = nullptr;  // exchange words of every shot
   size_t xch_words = 0;
@@ -159,6 +193,23 @@ void free_batch(seisfat_batch_s* b) {
   dfree(b->d_tiles);
   dfree(b->d_xch);
   dfree(b->d_ctl);
+  AdjState& a = b->adj;
+  dfree(a.d_tab);
+  dfree(a.d_T64);
+  dfree(a.d_lamA);
+  dfree(a.d_lamB);
+  dfree(a.d_coef);
+  dfree(a.d_code);
+  dfree(a.d_gc);
+  dfree(a.d_gshot);
+  dfree(a.d_xmsg);
+  dfree(a.d_seed_node);
+  dfree(a.d_seed_ptr);
+  dfree(a.d_seed_rec);
+  dfree(a.d_seed_w);
+  dfree(a.d_picks64);
+  dfree(a.d_w);
+  dfree(a.d_grad);
   delete b;
 }
 
@@ -293,6 +344,9 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   b->nshots = nshots;
   b->nrec = nrec_total;
   b->shots.assign(shots, shots + nshots);
+  b->rxr = rxr;
+  b->rzr = rzr;
+  b->recshot = recshot;
   b->iso.resize(nshots);
   b->toff.resize(nshots);
   const int nzz = h->nzz;
@@ -464,7 +518,6 @@ int batch_status(seisfat_batch_s* b) {
 
 int run_batch(seisfat_batch_s* b, hipStream_t st) {
   if (!b->h->has_s) return fail(SEISFAT_ESTATE, "slowness rejected or not set since the batch was created");
-  if (!st) st = b->h->stream;
   if (b->gexec[0]) {
     HIPCHK(hipGraphLaunch(b->gexec[0], st));
     HIPCHK(hipEventRecord(b->ev_sweep[0], st));
@@ -477,6 +530,169 @@ int run_batch(seisfat_batch_s* b, hipStream_t st) {
     HIPCHK(hipEventRecord(b->ev_sweep[i], st));
     return SEISFAT_OK;
   });
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint.  Layout of the per-batch state: see eik_adjoint.h.
+void cubic_weights(double d, double w[4]) {
+  // Catmull-Rom weights of cubic1d (src/admin/admin.cpp:121-127): the pick is
+  // linear in its 16 stencil values with weights wx[pIdx] * wz[pIdz].
+  w[0] = 0.5 * d * (-1.0 + d * (2.0 - d));
+  w[1] = 1.0 + 0.5 * d * d * (-5.0 + 3.0 * d);
+  w[2] = 0.5 * d * (1.0 + d * (4.0 - 3.0 * d));
+  w[3] = 0.5 * d * d * (-1.0 + d);
+}
+
+int adj_prepare(seisfat_batch_s* b) {
+  AdjState& A = b->adj;
+  if (A.ready) return SEISFAT_OK;
+  seisfat_eik_s* h = b->h;
+  const int nzz = h->nzz, nshots = b->nshots;
+  A.nstr = (nzz + 63) / 64;
+  A.tab.resize(nshots);
+  size_t koff = 0, xoff = 0;
+  A.max_nodes = 0;
+  for (int k = 0; k < nshots; ++k) {
+    AdjShot& s = A.tab[k];
+    s.sd = b->iso[k];
+    s.toff = b->toff[k];
+    const int nxx = s.sd.nxx;
+    s.NTP = (nxx + 63 + ADJ_PF - 1) / ADJ_PF * ADJ_PF;
+    s.xrs = (2 * (s.NTP + 1) + 63) / 64 * 64;
+    s.koff = koff;
+    s.xoff = xoff;
+    koff += (size_t)A.nstr * s.NTP * 64;
+    xoff += (size_t)(A.nstr + 1) * s.xrs;
+    A.max_nodes = std::max(A.max_nodes, nzz * nxx);
+  }
+  A.ksize = koff;
+  A.xsize = xoff;
+  // pick seeds: (skewed node of the last reverse sweep, receiver, weight),
+  // grouped by node with receivers in order
+  struct Seed {
+    size_t node;
+    int rec;
+    float w;
+  };
+  std::vector<Seed> es;
+  es.reserve((size_t)b->nrec * 16);
+  const float dx = h->dx, dz = h->dz;
+  for (int r = 0; r < b->nrec; ++r) {
+    const int k = b->recshot[r];
+    const AdjShot& s = A.tab[k];
+    const float x = b->rxr[r], z = b->rzr[r];
+    const float x0 = std::floor((x + 0.5f * dx) / dx) * dx;
+    const float z0 = std::floor((z + 0.5f * dz) / dz) * dz;
+    const float x1 = std::floor((x + 0.5f * dx) / dx) * dx + dx;
+    const float z1 = std::floor((z + 0.5f * dz) / dz) * dz + dz;
+    const float xd = (x - x0) / (x1 - x0);
+    const float zd = (z - z0) / (z1 - z0);
+    const int i = (int)((z + 0.5f * dz) / dz) + NB;
+    const int j = (int)((x + 0.5f * dx) / dx) + NB;
+    double wx[4], wz[4];
+    cubic_weights(xd, wx);
+    cubic_weights(zd, wz);
+    for (int pIdx = 0; pIdx < 4; ++pIdx)
+      for (int pIdz = 0; pIdz < 4; ++pIdz) {
+        const int ni = i + pIdz - 1, nj = j + pIdx - 1;
+        const size_t node =
+            s.koff + adj_kidx(adj_ip(ni, adj_a(3), nzz), adj_jp(nj, adj_b(3), s.sd.nxx), s.NTP);
+        es.push_back(Seed{node, r, (float)(wx[pIdx] * wz[pIdz])});
+      }
+  }
+  std::stable_sort(es.begin(), es.end(), [](const Seed& p, const Seed& q) { return p.node < q.node; });
+  std::vector<size_t> snode;
+  std::vector<int> sptr, srec;
+  std::vector<float> sw;
+  for (size_t e = 0; e < es.size(); ++e) {
+    if (e == 0 || es[e].node != es[e - 1].node) {
+      snode.push_back(es[e].node);
+      sptr.push_back((int)e);
+    }
+    srec.push_back(es[e].rec);
+    sw.push_back(es[e].w);
+  }
+  sptr.push_back((int)es.size());
+  A.nseed = (int)snode.size();
+
+  int rc;
+  const size_t tsize = b->tsize;
+  if ((rc = dalloc(&A.d_tab, (size_t)nshots)) || (rc = dalloc(&A.d_T64, 5 * tsize)) ||
+      (rc = dalloc(&A.d_lamA, A.ksize)) || (rc = dalloc(&A.d_lamB, A.ksize)) || (rc = dalloc(&A.d_coef, A.ksize)) ||
+      (rc = dalloc(&A.d_code, A.ksize)) || (rc = dalloc(&A.d_gc, A.ksize)) || (rc = dalloc(&A.d_gshot, tsize)) ||
+      (rc = dalloc(&A.d_xmsg, A.xsize)) || (rc = dalloc(&A.d_seed_node, snode.size())) ||
+      (rc = dalloc(&A.d_seed_ptr, sptr.size())) || (rc = dalloc(&A.d_seed_rec, srec.size())) ||
+      (rc = dalloc(&A.d_seed_w, sw.size())) || (rc = dalloc(&A.d_w, (size_t)b->nrec)) || (rc = dalloc(&A.d_picks64, (size_t)b->nrec)) ||
+      (rc = dalloc(&A.d_grad, (size_t)h->nz * h->nx)))
+    return rc;
+  hipStream_t st = h->stream;
+  HIPCHK(hipMemcpyAsync(A.d_tab, A.tab.data(), nshots * sizeof(AdjShot), hipMemcpyHostToDevice, st));
+  if (!snode.empty()) {
+    HIPCHK(hipMemcpyAsync(A.d_seed_node, snode.data(), snode.size() * sizeof(size_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(A.d_seed_ptr, sptr.data(), sptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(A.d_seed_rec, srec.data(), srec.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(A.d_seed_w, sw.data(), sw.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  }
+  // Entries of the skewed arrays outside every grid are never written and
+  // must read as zero; message slot 0 of every shot likewise.
+  HIPCHK(hipMemsetAsync(A.d_lamA, 0, A.ksize * sizeof(float), st));
+  HIPCHK(hipMemsetAsync(A.d_lamB, 0, A.ksize * sizeof(float), st));
+  HIPCHK(hipMemsetAsync(A.d_coef, 0, A.ksize * sizeof(float4), st));
+  HIPCHK(hipMemsetAsync(A.d_code, 0, A.ksize, st));
+  HIPCHK(hipMemsetAsync(A.d_gc, 0, A.ksize * sizeof(float), st));
+  HIPCHK(hipMemsetAsync(A.d_xmsg, 0, A.xsize * sizeof(float), st));
+  // ring nodes of every trajectory grid, and T0 off the source stencil, stay 1e6
+  launch_fill_d(A.d_T64, 5 * tsize, 1e6, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  A.ready = true;
+  return SEISFAT_OK;
+}
+
+// The batch's forward solve, then the fp64 trajectory the adjoint linearises
+// (T0 = initial field, T_{s+1} = sweep s of T_s).
+int run_record(seisfat_batch_s* b, hipStream_t st) {
+  if (b->type != SEISFAT_ISO) return fail(SEISFAT_EINVAL, "the adjoint covers isotropic batches");
+  if (int rc = adj_prepare(b)) return rc;
+  AdjState& A = b->adj;
+  A.recorded = false;
+  if (int rc = run_batch(b, st)) return rc;
+  seisfat_eik_s* h = b->h;
+  const size_t ts = b->tsize;
+  launch_fwd64_init(A.d_tab, b->nshots, h->nzz, h->dx, h->dz, A.d_T64, st);
+  for (int s = 0; s < 4; ++s)
+    launch_fwd64_sweep(s, A.d_tab, b->nshots, h->nzz, A.nstr, h->dx, h->dz, A.d_T64 + s * ts,
+                       A.d_T64 + (s + 1) * ts, st);
+  launch_picks64(A.d_tab, b->d_recshot, b->d_rx, b->d_rz, b->nrec, h->nzz, h->dx, h->dz, A.d_T64 + 4 * ts,
+                 A.d_picks64, st);
+  HIPCHK(hipGetLastError());
+  A.recorded = true;
+  A.version = h->version;
+  return SEISFAT_OK;
+}
+
+int enqueue_vjp(seisfat_batch_s* b, const float* d_w, double* d_grad, hipStream_t st) {
+  AdjState& A = b->adj;
+  seisfat_eik_s* h = b->h;
+  if (!A.recorded) return fail(SEISFAT_ESTATE, "no recorded run (seisfat_batch_run_record)");
+  if (A.version != h->version) return fail(SEISFAT_ESTATE, "the slowness changed since the recorded run");
+  if (!d_grad) return fail(SEISFAT_EINVAL, "null gradient pointer");
+  if (b->nrec > 0 && !d_w) return fail(SEISFAT_EINVAL, "null cotangent pointer");
+  const int nzz = h->nzz, ns = b->nshots;
+  launch_fill(A.d_lamB, A.ksize, 0.0f, st);
+  launch_adj_seed(A.d_seed_node, A.d_seed_ptr, A.d_seed_rec, A.d_seed_w, A.nseed, d_w, A.d_lamB, st);
+  launch_fill(A.d_gshot, b->tsize, 0.0f, st);
+  const size_t ts = b->tsize;
+  for (int s = 3; s >= 0; --s) {
+    launch_adj_coef(s, A.d_tab, ns, A.max_nodes, nzz, h->dx, h->dz, A.d_T64 + (s + 1) * ts, A.d_T64 + s * ts,
+                    s == 3 ? nullptr : A.d_lamA, A.d_lamB, A.d_coef, A.d_code, st);
+    launch_adj_sweep(A.d_tab, ns, A.nstr, A.d_lamB, A.d_coef, A.d_lamA, A.d_gc, A.d_xmsg, st);
+    launch_adj_fold(s, A.d_tab, ns, A.max_nodes, nzz, A.d_gc, A.d_code, A.d_gshot, st);
+  }
+  launch_adj_source(A.d_tab, ns, nzz, h->dx, h->dz, A.d_lamA, A.d_gshot, st);
+  launch_adj_reduce(A.d_tab, ns, h->nz, h->nx, nzz, A.d_gshot, d_grad, st);
+  HIPCHK(hipGetLastError());
+  return SEISFAT_OK;
 }
 
 }  // namespace
@@ -568,7 +784,8 @@ int seisfat_eik_destroy(seisfat_eik_t h) {
 int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* stream) {
   if (int rc = check_handle(h)) return rc;
   if (!d_s) return fail(SEISFAT_EINVAL, "null slowness pointer");
-  hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+  hipStream_t st = (hipStream_t)stream;
+  ++h->version;  // recorded adjoint runs of the previous model become stale
   HIPCHK(hipMemsetAsync(h->d_flag, 0, sizeof(int), st));
   launch_check_positive(d_s, (size_t)h->nz * h->nx, std::min(h->dx, h->dz), std::max(h->dx, h->dz), h->d_flag, st);
   launch_zpad(d_s, h->d_szp, h->nz, h->nx, st);
@@ -597,6 +814,8 @@ int seisfat_eik_set_slowness(seisfat_eik_t h, const float* s) {
       return fail(SEISFAT_EINVAL, "slowness must be positive and finite, with slowness*spacing and "
                                   "slowness/spacing in [1e-12, 1e12] (index " + std::to_string(k) + ")");
     }
+  // runs of this handle's batches still in flight on any stream read the model
+  HIPCHK(hipDeviceSynchronize());
   float* d_s = nullptr;
   if (int rc = dalloc(&d_s, n)) return rc;
   hipError_t e = hipMemcpyAsync(d_s, s, n * sizeof(float), hipMemcpyHostToDevice, h->stream);
@@ -604,7 +823,7 @@ int seisfat_eik_set_slowness(seisfat_eik_t h, const float* s) {
   if (e != hipSuccess)
     rc = fail(SEISFAT_EHIP, std::string("upload slowness: ") + hipGetErrorString(e));
   else
-    rc = seisfat_eik_set_slowness_device(h, d_s, nullptr);
+    rc = seisfat_eik_set_slowness_device(h, d_s, h->stream);
   (void)hipStreamSynchronize(h->stream);
   dfree(d_s);
   return rc;
@@ -736,6 +955,45 @@ int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms) {
 int seisfat_batch_status(seisfat_batch_t b) {
   if (!b) return fail(SEISFAT_EINVAL, "null batch");
   return batch_status(b);
+}
+
+int seisfat_batch_run_record(seisfat_batch_t b, void* stream) {
+  if (!b) return fail(SEISFAT_EINVAL, "null batch");
+  return run_record(b, (hipStream_t)stream);
+}
+
+int seisfat_batch_vjp_device(seisfat_batch_t b, const float* d_w, double* d_grad, void* stream) {
+  if (!b) return fail(SEISFAT_EINVAL, "null batch");
+  return enqueue_vjp(b, d_w, d_grad, (hipStream_t)stream);
+}
+
+int seisfat_batch_misfit_gradient(seisfat_batch_t b, const float* obs, double* J, double* grad, float* picks) {
+  if (!b) return fail(SEISFAT_EINVAL, "null batch");
+  if (!grad) return fail(SEISFAT_EINVAL, "null gradient buffer");
+  if (b->nrec > 0 && !obs) return fail(SEISFAT_EINVAL, "null observed picks");
+  seisfat_eik_s* h = b->h;
+  if (int rc = run_record(b, h->stream)) return rc;
+  if (int rc = batch_status(b)) return rc;
+  std::vector<float> pk(b->nrec), w(b->nrec);
+  std::vector<double> pk64(b->nrec);
+  if (b->nrec > 0) {
+    HIPCHK(hipMemcpy(pk.data(), b->d_picks, b->nrec * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pk64.data(), b->adj.d_picks64, b->nrec * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  double j = 0.0;
+  for (int r = 0; r < b->nrec; ++r) {
+    const double res = pk64[r] - (double)obs[r];
+    j += 0.5 * res * res;
+    w[r] = (float)res;
+  }
+  AdjState& A = b->adj;
+  if (b->nrec > 0) HIPCHK(hipMemcpyAsync(A.d_w, w.data(), b->nrec * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  if (int rc = enqueue_vjp(b, A.d_w, A.d_grad, h->stream)) return rc;
+  HIPCHK(hipMemcpyAsync(grad, A.d_grad, (size_t)h->nz * h->nx * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (J) *J = j;
+  if (picks && b->nrec > 0) std::memcpy(picks, pk.data(), b->nrec * sizeof(float));
+  return SEISFAT_OK;
 }
 
 int seisfat_batch_destroy(seisfat_batch_t b) {

@@ -72,6 +72,13 @@ __device__ __forceinline__ float vmin(float a, float b) {
   return r;
 }
 
+// wave_shr:1 (DPP control 0x138): lane k receives lane k-1's `src`; lane 0
+// keeps `old`.
+__device__ __forceinline__ float dpp_shr1(float old, float src) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x138,
+                                                    0xf, 0xf, false));
+}
+
 // A shot as the kernels see it.  T is the shot's padded traveltime grid
 // (nzz x nxx, column stride nzz).  Slowness of padded node/cell (i, j) is
 //   sbase[i + clamp(j - sofs, 0, smax) * nzz]

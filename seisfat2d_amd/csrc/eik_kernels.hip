@@ -132,12 +132,6 @@ __device__ __forceinline__ float cell_at(const ShotDev& sd, int nzz, int kap, in
   return s_at(sd, nzz, i, j);
 }
 
-__device__ __forceinline__ float dpp_shr1(float old, float src) {
-  // wave_shr:1 (DPP control 0x138): lane k receives lane k-1; lane 0 keeps `old`.
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x138,
-                                                    0xf, 0xf, false));
-}
-
 constexpr int RING_LD = 66;  // LDS row stride (words): 64 lanes + slowness row 64 + pad
 constexpr int PF = 8;        // slowness prefetch distance (steps) on the skewed path
 

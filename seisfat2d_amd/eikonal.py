@@ -149,6 +149,28 @@ class Batch:
         """Synchronise and raise if the device flagged a failure during the runs."""
         check(lib().seisfat_batch_status(self._b))
 
+    # -- adjoint (isotropic batches) ---------------------------------------
+    def run_record(self, stream=None):
+        """Forward solve that keeps what the adjoint needs (same picks as run)."""
+        check(lib().seisfat_batch_run_record(self._b, None if stream is None else C.c_void_p(stream)))
+
+    def vjp_device(self, w_ptr, grad_ptr, stream=None):
+        """grad (nz*nx float64, device) = sum_r w[r] dpick_r/ds for the recorded run."""
+        check(lib().seisfat_batch_vjp_device(self._b, C.c_void_p(w_ptr), C.c_void_p(grad_ptr),
+                                             None if stream is None else C.c_void_p(stream)))
+
+    def misfit_gradient(self, obs):
+        """(J, dJ/ds as (nx, nz) float64, picks) for J = 1/2 sum (pick - obs)^2."""
+        obs = np.ascontiguousarray(obs, np.float32)
+        if obs.size != self.nrec:
+            raise ValueError("obs must hold one value per receiver")
+        J = C.c_double()
+        g = np.empty((self.eik.nx, self.eik.nz), np.float64)
+        pk = np.empty(self.nrec, np.float32)
+        check(lib().seisfat_batch_misfit_gradient(self._b, capi._ptr(obs), C.byref(J), capi._ptr(g),
+                                                  capi._ptr(pk)))
+        return J.value, g, pk
+
     def close(self):
         if getattr(self, "_b", None):
             lib().seisfat_batch_destroy(self._b)

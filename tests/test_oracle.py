@@ -269,3 +269,34 @@ def test_adjoint_matches_central_differences(oracle, rng):
         h = 1e-6
         fd = (Jf(S + h * V) - Jf(S - h * V)) / (2 * h)
         assert abs(fd - np.sum(g * V)) <= 1e-6 * abs(fd)
+
+
+def test_batch_misfit_window_fold_matches_central_differences(oracle, rng):
+    # Window shots fold their padded gradient onto the window's model cells;
+    # FP64 central differences of the batch misfit pin that fold.
+    nz, nx, dh = 17, 40, 10.0
+    s = smooth_model(rng, nz, nx, dh, grad=2.0).astype(np.float64)
+    shots, rx, rz = [], [], []
+    for sx, sz, x0, w in ((200.0, 0.0, 0, 40), (120.0, 30.0, 5, 20), (300.0, 0.0, 18, 22)):
+        xs = np.arange(x0 + 1, x0 + w - 1, 4) * dh + 3.0
+        shots.append((sx, sz, x0, w, len(rx), len(xs)))
+        rx.extend(xs)
+        rz.extend([12.0] * len(xs))
+    rx, rz = np.array(rx), np.array(rz)
+    _, _, obs = oracle.batch_misfit(s * 1.03, dh, dh, shots, rx, rz, np.zeros(len(rx)), grad=False)
+    J, g, _ = oracle.batch_misfit(s, dh, dh, shots, rx, rz, obs)
+    assert np.isclose(J, oracle.batch_misfit(s, dh, dh, shots, rx, rz, obs, grad=False)[0], rtol=1e-14)
+    for _ in range(3):
+        V = rng.standard_normal(s.shape) * s
+        h = 1e-6
+        fd = (oracle.batch_misfit(s + h * V, dh, dh, shots, rx, rz, obs, grad=False)[0]
+              - oracle.batch_misfit(s - h * V, dh, dh, shots, rx, rz, obs, grad=False)[0]) / (2 * h)
+        assert abs(fd - np.sum(g * V)) <= 1e-6 * abs(fd)
+    # one full-model shot reproduces the single-shot adjoint folded by reduce_boundary_adjoint
+    s32 = s.astype(np.float32)
+    sel = [shots[0]]
+    n0 = shots[0][5]
+    _, g1, _ = oracle.batch_misfit(s32, dh, dh, sel, rx[:n0], rz[:n0], obs[:n0], nthreads=1)
+    _, gp, _ = oracle.adjoint(oracle.expand_boundary(s32), dh, dh, 200.0, 0.0, rx[:n0].astype(np.float32),
+                              rz[:n0].astype(np.float32), obs[:n0].astype(np.float32))
+    assert np.array_equal(g1, oracle.reduce_boundary_adjoint(gp))
