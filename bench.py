@@ -11,8 +11,10 @@ with the model resident in HBM.
 Reported beside it: the plain per-shot GPU solve (one launch per
 anti-diagonal, the reference's schedule) on a sample of the same windows, the
 same windows solved one shot at a time with the tiled schedule, the HBM
-roofline fraction of the sweep kernel, and the CPU oracle timed on a sample of
-the same windows.
+roofline fraction of the sweep kernel, the CPU oracle timed on a sample of
+the same windows, and two secondary workloads: the batched anisotropic (TTI
+qP) solve of 128 shots on 801 x 3001, and the adjoint-state misfit gradient
+of 128 shots x 401 receivers on 301 x 1201.
 
 Prints one JSON line.  `--gpus` above 1 is refused: this build runs on one GPU.
 """
@@ -31,6 +33,19 @@ BYTES_PER_CELL_UPDATE = 12  # S read + T read + T write, 4 B each
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_persist_c3_256shots_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per sweep launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes of this workload (calibrated against a device copy)."""
+    try:
+        kv = dict(l.split(":", 1) for l in open(TRAFFIC_FILE) if ":" in l)
+        return float(kv["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(w, batch, budget_s):
@@ -57,6 +72,76 @@ def cpu_baseline(w, batch, budget_s):
             "shots_per_s": done / t_cpu, "gpu_windows_bit_exact": f"{done - mismatches}/{done}"}
 
 
+def bench_ani(steps):
+    """128 full-model TTI qP shots on 801 x 3001 (dh 5 m): iso solve, qP
+    slowness, qP solve and picks per shot, one HIP graph per step."""
+    import torch
+    from seisfat2d_amd import ANI, SCHED_TILED, Eikonal, models
+    w = models.c4()
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    e.set_stiffness(w.stiffness)
+    b = e.batch(w.shots, w.rx, w.rz, type=ANI, schedule=SCHED_TILED)
+    b.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    b.status()
+    cu = 2 * models.cell_updates(w.nz, [s[3] for s in w.shots])
+    out = {"workload": f"batched TTI qP FSM, {len(w.shots)} full-model shots on {w.nz}x{w.nx} dh=5 m "
+                       "(eps, delta, tilt smooth seeded fields), two single-pass solves + qP slowness per shot",
+           "ms_per_step": dt * 1e3, "shots_per_s": len(w.shots) / dt, "cell_updates_per_s": cu / dt}
+    b.close()
+    e.close()
+    return out
+
+
+def bench_adjoint(steps):
+    """Misfit gradient of 128 shots x 401 receivers on 301 x 1201 (dh 10 m):
+    recorded forward (fp32 batch + fp64 trajectory) and the adjoint sweeps."""
+    import torch
+    from seisfat2d_amd import Eikonal, models
+    w = models.c5()
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.extra["s_true"])
+    b = e.batch(w.shots, w.rx, w.rz)
+    b.run()
+    obs = torch.from_numpy(b.picks()).cuda()
+    e.set_slowness(w.s)
+    g = torch.empty((w.nx, w.nz), dtype=torch.float64, device="cuda")
+    t_rec = t_vjp = 0.0
+    for it in range(steps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.run_record()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        res = torch.as_tensor(_DevPicks(b.device_picks_ptr(), len(w.rx)), device="cuda") - obs
+        b.vjp_device(res.data_ptr(), g.data_ptr())
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if it > 0:
+            t_rec += t1 - t0
+            t_vjp += t2 - t1
+    t_rec, t_vjp = t_rec / steps, t_vjp / steps
+    out = {"workload": f"misfit gradient, {len(w.shots)} shots x 401 receivers on {w.nz}x{w.nx} dh=10 m "
+                       "(v(z) start model, picks of v(z)+4 anomalies)",
+           "ms_record": t_rec * 1e3, "ms_vjp": t_vjp * 1e3, "ms_gradient": (t_rec + t_vjp) * 1e3,
+           "shots_per_s": len(w.shots) / (t_rec + t_vjp), "grad_norm": float(g.norm())}
+    b.close()
+    e.close()
+    return out
+
+
+class _DevPicks:
+    def __init__(self, ptr, n):
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": "<f4", "data": (int(ptr), False),
+                                         "version": 2}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +150,7 @@ def main():
     ap.add_argument("--shots", type=int, default=256)
     ap.add_argument("--plain-shots", type=int, default=3, help="windows timed with the per-diagonal schedule")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU oracle budget (0 = skip)")
+    ap.add_argument("--extras", type=int, default=1, help="also time the anisotropic and adjoint workloads")
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: the batched steps only, plus a 256 MiB device copy that calibrates "
                          "per-dispatch byte counters")
@@ -153,6 +239,14 @@ def main():
     single_shots_per_s = nps / t_single
     log(f"per-shot: plain {plain_shots_per_s:.1f} shots/s, tiled {single_shots_per_s:.1f} shots/s")
 
+    extras = {}
+    if args.extras:
+        extras["ani_c4"] = bench_ani(max(1, min(args.steps, 5)))
+        log(f"ani: {extras['ani_c4']['ms_per_step']:.1f} ms/step, {extras['ani_c4']['shots_per_s']:.0f} shots/s")
+        extras["adjoint_c5"] = bench_adjoint(max(1, min(args.steps, 5)))
+        a = extras["adjoint_c5"]
+        log(f"adjoint: record {a['ms_record']:.1f} ms + vjp {a['ms_vjp']:.1f} ms per gradient of 128 shots")
+
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(w, b, args.cpu_seconds)
@@ -182,11 +276,13 @@ def main():
         "rel_err_vs_oracle": 0.0 if (cpu and cpu["gpu_windows_bit_exact"].split("/")[0] ==
                                     cpu["gpu_windows_bit_exact"].split("/")[1]) else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                      "kernel": "k_sweep_persist", "launches": nlaunch,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": avg_launch_s * 1e6},
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(out))
     b.close()
