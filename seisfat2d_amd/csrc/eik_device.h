@@ -170,4 +170,69 @@ __device__ __forceinline__ float eik_local(float tv, float te, float tev, float 
   return vmin(t1D, vmin(t2D, TINF));
 }
 
+// Quantities of one slowness cell that the local solver derives from it,
+// computed once when the cell enters a tile's prefetch queue: the cell is the
+// z-other cell of one step and the reference cell of the next.  Each is the
+// same rounded expression eik_local evaluates.
+struct CellQ {
+  float dzs;   // dz * s
+  float dxs;   // dx * s
+  float s4;    // 4 * s * s * (dz2i + dx2i)
+  float thrB;  // s * dz * dz / hyp
+  float thrC;  // s * dx * dx / hyp
+  float s2;    // s * s
+};
+
+template <bool EQH>
+__device__ __forceinline__ CellQ cell_q(float s, const EikConsts& k) {
+  CellQ c;
+  c.dzs = s * k.dz;
+  c.dxs = s * k.dx;
+  c.s4 = 4.0f * s * s * k.sum;
+  c.thrB = rdiv(c.dzs * k.dz, k.hyp, k.r_hyp);
+  c.thrC = EQH ? c.thrB : rdiv(c.dxs * k.dx, k.hyp, k.r_hyp);
+  c.s2 = s * s;
+  return c;
+}
+
+// eik_local without branches, for the tile kernels: both candidate branches
+// are evaluated on every lane and selected, sharing one square root (branch A
+// and branches B/C each take exactly one, of different radicands).  With
+// h > 0, RN(h * min(a, b)) = min(RN(h * a), RN(h * b)), so the 1-D candidates
+// can use the cells' precomputed products.  Bit-identical to eik_local: the
+// selected lanes evaluate the same operations in the same order; the others
+// compute values that the selects discard.
+template <bool EQH>
+__device__ __forceinline__ float eik_local_sel(float tv, float te, float tev, const CellQ& r, float dzs_zo,
+                                               float dxs_xo, const EikConsts& k) {
+  const float t1d1 = tv + vmin(r.dzs, dzs_zo);
+  const float t1d2 = te + vmin(r.dxs, dxs_xo);
+  const float t1D = vmin(t1d1, t1d2);
+  const float de = te - tev;
+  const float dv = tv - tev;
+  // non-short-circuit conjunctions: plain mask arithmetic, no control flow
+  const bool cA = (tv <= te + r.dxs) & (te <= tv + r.dzs) & (de >= 0.0f) & (dv >= 0.0f);
+  const bool cB = (de <= r.thrB) & (de > 0.0f);
+  const bool cC = (dv <= r.thrC) & (dv > 0.0f);
+  // branch B precedes C (the reference's else-if order)
+  const float dd = cB ? de : dv;
+  const float h1 = cB ? k.dz : k.dx;
+  const float r1 = cB ? k.r_dz : k.r_dx;
+  const float h2 = cB ? k.dx : k.dz;
+  const float t0 = cB ? te : tv;
+  const float q = rdiv(dd, h1, r1);
+  const float ta = tev + te - tv;
+  const float tb = tev - te + tv;
+  const float dab = ta - tb;
+  // radicand a - b * c: A: 4 s^2 (dz2i + dx2i) - (dz2i dx2i dab) dab; B/C: s^2 - q q
+  const float ra = cA ? r.s4 : r.s2;
+  const float rb = cA ? k.prod * dab : q;
+  const float rc = cA ? dab : q;
+  const float sq = rsqrt_rn(ra - rb * rc);
+  const float tA = rdiv((tb * k.dz2i + ta * k.dx2i) + sq, k.sum, k.r_sum);
+  const float tBC = t0 + h2 * sq;
+  const float t2D = cA ? tA : ((cB | cC) ? tBC : TINF);
+  return vmin(t1D, vmin(t2D, TINF));
+}
+
 }  // namespace seisfat

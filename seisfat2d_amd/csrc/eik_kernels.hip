@@ -135,6 +135,15 @@ __device__ __forceinline__ float cell_at(const ShotDev& sd, int nzz, int kap, in
 constexpr int RING_LD = 66;  // LDS row stride (words): 64 lanes + slowness row 64 + pad
 constexpr int PF = 8;        // slowness prefetch distance (steps) on the skewed path
 
+// A tile's traveltimes in LDS are instruction-major: row v holds, for lane k,
+// the node of step (v + k) & 63 (the order in which 64 coalesced column loads
+// deliver them).  Lane k's node of step t is therefore at row (t - k) & 63;
+// with the 66-word stride a step's 64 accesses fall in distinct banks within
+// each half-wave.
+__device__ __forceinline__ int trow(int t, int lane) { return ((t - lane) & 63) * RING_LD + lane; }
+
+typedef __attribute__((address_space(3))) float lfloat;
+
 template <int SA>
 __device__ __forceinline__ int prow(int kk, int nzz) {
   return (SA > 0) ? 1 + kk : nzz - 2 - kk;
@@ -162,16 +171,18 @@ __device__ __forceinline__ int skew_addr(const ShotDev& sd, int nxx, int NC, int
 template <bool EQH>
 __device__ __forceinline__ void tile_step(float* __restrict__ sT, int t, int lane, bool row_ok, int c0, int NC,
                                           float told, float h, float szo, float sxo, float& te, float& tev,
-                                          float& sref, const EikConsts& k) {
+                                          CellQ& sref, const EikConsts& k) {
+  const CellQ zo = cell_q<EQH>(szo, k);  // this step's z-other cell, next step's reference cell
   const float tv = dpp_shr1(h, te);
   const int c = c0 + t - lane;
   const bool act = row_ok && (unsigned)c < (unsigned)NC;
-  const float cand = eik_local<EQH>(tv, te, tev, sref, szo, sxo, k);
-  const float res = act ? vmin(told, cand) : told;
-  sT[t * RING_LD + lane] = res;
+  const float cand = eik_local_sel<EQH>(tv, te, tev, sref, zo.dzs, k.dx * sxo, k);
+  const float m = vmin(told, cand);  // unconditional: a select, not a branch
+  const float res = act ? m : told;
+  sT[trow(t, lane)] = res;
   tev = tv;
   te = res;
-  sref = szo;
+  sref = zo;
 }
 
 // The 64 steps of a tile with slowness from the skewed global copy, loaded PF
@@ -183,7 +194,7 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int t, int lan
 template <int SB, bool EQH, bool FAST>
 __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewView& sv, float* __restrict__ sT,
                                                 const float* __restrict__ sH, int lane, int kap, bool row_ok,
-                                                int c0, int nxx, int NC, float& te, float& tev, float& sref,
+                                                int c0, int nxx, int NC, float& te, float& tev, CellQ& sref,
                                                 const EikConsts& k) {
   const gfloat* __restrict__ SK = gptr(sv.base);
   const int KP = sv.KP;
@@ -191,30 +202,45 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
   // (row kap+1 = next strip) sits one skew period (64 columns) away.
   const int sig0 = (SB > 0) ? sd.x0 + c0 - 2 : sd.x0 + sd.w + 63 - c0;
   const int colbase0 = (sig0 + SK_OFF) * KP;
-  const int offx = kap + 1 + ((lane == 63) ? -SB * 64 * KP : 0);
-  auto addr_z = [&](int t) -> int {
-    return FAST ? colbase0 + SB * t * KP + kap : skew_addr<SB>(sd, nxx, NC, KP, kap, c0 + t - lane + 1);
+  // fast path: a uniform base per step plus an unsigned per-lane offset (the
+  // x-other base sits one skew period lower so every lane offset is >= 0)
+  const unsigned offz = (unsigned)kap;
+  const unsigned offx = (unsigned)(kap + 1 + 64 * KP + ((lane == 63) ? -SB * 64 * KP : 0));
+  const gfloat* pz = SK + colbase0;  // fast path: step t's column, advanced one step per load
+  const gfloat* px = pz - 64 * KP;
+  auto load_z = [&](int t) -> float {
+    if (FAST) {
+      const float v = pz[offz];
+      pz += SB * KP;
+      return v;
+    }
+    return SK[skew_addr<SB>(sd, nxx, NC, KP, kap, c0 + t - lane + 1)];
   };
-  auto addr_x = [&](int t) -> int {
-    return FAST ? colbase0 + SB * t * KP + offx : skew_addr<SB>(sd, nxx, NC, KP, kap + 1, c0 + t - lane);
+  auto load_x = [&](int t) -> float {
+    if (FAST) {
+      const float v = px[offx];
+      px += SB * KP;
+      return v;
+    }
+    return SK[skew_addr<SB>(sd, nxx, NC, KP, kap + 1, c0 + t - lane)];
   };
   float zq[PF], xq[PF];
 #pragma unroll
   for (int u = 0; u < PF; ++u) {
-    zq[u] = SK[addr_z(u)];
-    xq[u] = SK[addr_x(u)];
+    zq[u] = load_z(u);
+    xq[u] = load_x(u);
   }
-  float told_n = sT[lane], h_n = sH[0];
+  float told_n = sT[trow(0, lane)], h_n = sH[0];
   for (int t0 = 0; t0 < 64; t0 += PF) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const int t = t0 + u;
       const float told = told_n, h = h_n;
-      told_n = sT[(t + 1) * RING_LD + lane];
+      told_n = sT[trow(t + 1, lane)];
       h_n = sH[t + 1];
       const float szo = zq[u], sxo = xq[u];
-      zq[u] = SK[addr_z(t + PF)];
-      xq[u] = SK[addr_x(t + PF)];
+      zq[u] = load_z(t + PF);
+      xq[u] = load_x(t + PF);
       tile_step<EQH>(sT, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
     }
   }
@@ -275,9 +301,9 @@ struct PersistCtl {
 // lane k's z-upwind neighbour is the value lane k-1 produced one step earlier
 // (a wave_shr:1 DPP move).  Tile (r, bb) needs (r, bb-1) and (r-1, bb+1).
 //
-// LDS holds the tile's old traveltimes step-major (sT[t][k] = node visited by
-// lane k at step t); results overwrite them in place and are written back
-// column-pair by column-pair, so every global access touches at most two
+// LDS holds the tile's old traveltimes instruction-major (see trow), loaded by
+// 64 column loads straight into LDS; results overwrite them in place and are
+// written back the same way, so every global access touches at most two
 // contiguous 256-byte segments.  SKEW: slowness comes from the skewed global
 // copy (one coalesced load per cell per step, prefetched PF steps ahead);
 // otherwise it is staged step-major in LDS from the natural layout (sS[t][kap]
@@ -303,6 +329,9 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   const bool row_ok = kg < NR;
   gfloat* __restrict__ T = gptr(sd.T);
   const int kgc = min(kg, NR);
+  // interior tile: every column it stages or writes lies in [0, NC)
+  const bool interior = c0 >= 64 && c0 + 64 <= NC;
+  const int jb = (SB > 0) ? 1 + c0 : nxx - 2 - c0;  // grid column of local column c0
 
   // ---- old traveltimes, step-major --------------------------------------
   if (FIRST) {
@@ -329,20 +358,30 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
           val = s_at(sd, nzz, i, j) * sqrtf(ddx * ddx + ddz * ddz);
         }
       }
-      sT[t * RING_LD + lane] = val;
+      sT[trow(t, lane)] = val;
     }
   } else {
-    // Pair loads: instruction v fetches, for lane k, column c = c0-k+((v+k)&63)
-    // (two contiguous column segments), i.e. the node of step (v+k)&63.
-    // Positions outside the interior clamp onto the boundary ring (always 1e6).
-    float tv_[64];
-#pragma unroll
-    for (int v = 0; v < 64; ++v) {
-      const int c = min(max(c0 - lane + ((v + lane) & 63), -1), NC);
-      tv_[v] = T[node_off<SA, SB>(kgc, c, nzz, nxx)];
+    // Column loads straight into LDS: instruction v fetches, for lane k, the
+    // node of step (v+k)&63, column c0-k+((v+k)&63) = c0+v or c0+v-64 (two
+    // contiguous column segments).  Edge tiles clamp positions outside the
+    // interior onto the boundary ring (always 1e6).
+    if (interior) {
+      const int irow = (SA > 0) ? 1 + kgc : nzz - 2 - kgc;
+      const unsigned o0 = (unsigned)(irow + (SB > 0 ? 64 * nzz : 0));
+      const unsigned o1 = (unsigned)(irow + (SB > 0 ? 0 : 64 * nzz));
+      const gfloat* Tb = T + (jb - (SB > 0 ? 64 : 0)) * nzz;
+      for (int v = 0; v < 64; ++v) {
+        __builtin_amdgcn_global_load_lds(Tb + (lane >= 64 - v ? o1 : o0), (lfloat*)(sT + v * RING_LD), 4, 0, 0);
+        Tb += SB * nzz;
+      }
+    } else {
+      for (int v = 0; v < 64; ++v) {
+        const int c = min(max(c0 - lane + ((v + lane) & 63), -1), NC);
+        __builtin_amdgcn_global_load_lds(T + (unsigned)node_off<SA, SB>(kgc, c, nzz, nxx),
+                                         (lfloat*)(sT + v * RING_LD), 4, 0, 0);
+      }
     }
-#pragma unroll
-    for (int v = 0; v < 64; ++v) sT[((v + lane) & 63) * RING_LD + lane] = tv_[v];
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0), lgkmcnt(0): the column loads have landed in LDS
   }
 
   // ---- upwind row above the strip (local row 64r-1) for lane 0, and the
@@ -371,7 +410,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
 
   if (SKEW) {
     const int kap = 64 * r + lane;
-    float sref = gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 - lane)];
+    CellQ sref = cell_q<EQH>(gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 - lane)], k);
     __syncthreads();
     // unreplicated cell columns: lam in [3, w+2] (SB > 0) or [2, w+1] (SB < 0)
     const int lam_lo = max(0, c0 - 63), lam_hi = min(NC, c0 + 64);
@@ -392,18 +431,18 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     }
     const int lam64 = min(max(c0 + 1 - 64 + ((lane + 63) & 63), 0), NC);
     const float s64 = cell_at<SA, SB>(sd, nzz, min(64 * r + 64, NR), lam64);
-    float sref = cell_at<SA, SB>(sd, nzz, kgc, min(max(c0 - lane, 0), NC));
+    CellQ sref = cell_q<EQH>(cell_at<SA, SB>(sd, nzz, kgc, min(max(c0 - lane, 0), NC)), k);
 #pragma unroll
     for (int v = 0; v < 64; ++v) sS[((v + lane - 1) & 63) * RING_LD + lane] = sv_[v];
     sS[((lane + 63) & 63) * RING_LD + 64] = s64;
     __syncthreads();
-    float told_n = sT[lane], h_n = sH[0], z_n = sS[lane], x_n = sS[lane + 1];
+    float told_n = sT[trow(0, lane)], h_n = sH[0], z_n = sS[lane], x_n = sS[lane + 1];
     for (int t0 = 0; t0 < 64; t0 += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int t = t0 + u;
         const float told = told_n, h = h_n, szo = z_n, sxo = x_n;
-        told_n = sT[(t + 1) * RING_LD + lane];
+        told_n = sT[trow(t + 1, lane)];
         h_n = sH[t + 1];
         z_n = sS[(t + 1) * RING_LD + lane];
         x_n = sS[(t + 1) * RING_LD + lane + 1];
@@ -418,19 +457,33 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     // each lane's last two results are block bb+1's carry-in.
     if (r + 1 < ctl.nstrips) {
       const int c = c0 + lane - 63;
-      if (c >= 0 && c < NC) xstore(sd.xh + (size_t)r * sd.ldh + c, sT[lane * RING_LD + 63], tagb);
+      if (c >= 0 && c < NC) xstore(sd.xh + (size_t)r * sd.ldh + c, sT[trow(lane, 63)], tagb);
     }
     unsigned long long* xc = sd.xc + (size_t)(2 * r + (bb & 1)) * 128;
     const unsigned ctag = tagb | ((unsigned)bb & 255u);
-    xstore(xc + 2 * lane, sT[63 * RING_LD + lane], ctag);
-    xstore(xc + 2 * lane + 1, sT[62 * RING_LD + lane], ctag);
+    xstore(xc + 2 * lane, sT[trow(63, lane)], ctag);
+    xstore(xc + 2 * lane + 1, sT[trow(62, lane)], ctag);
   }
 
-#pragma unroll 8
-  for (int v = 0; v < 64; ++v) {
-    const int t = (v + lane) & 63;
-    const int c = c0 + t - lane;
-    if (row_ok && c >= 0 && c < NC) T[node_off<SA, SB>(kg, c, nzz, nxx)] = sT[t * RING_LD + lane];
+  // write-back, row v of LDS = the nodes of column loads v
+  if (interior) {
+    const int irow = (SA > 0) ? 1 + kgc : nzz - 2 - kgc;
+    const unsigned o0 = (unsigned)(irow + (SB > 0 ? 64 * nzz : 0));
+    const unsigned o1 = (unsigned)(irow + (SB > 0 ? 0 : 64 * nzz));
+    gfloat* Tb = T + (jb - (SB > 0 ? 64 : 0)) * nzz;
+#pragma unroll 4
+    for (int v = 0; v < 64; ++v) {
+      const float val = sT[v * RING_LD + lane];
+      if (row_ok) Tb[lane >= 64 - v ? o1 : o0] = val;
+      Tb += SB * nzz;
+    }
+  } else {
+#pragma unroll 4
+    for (int v = 0; v < 64; ++v) {
+      const int c = c0 - lane + ((v + lane) & 63);
+      const float val = sT[v * RING_LD + lane];
+      if (row_ok && c >= 0 && c < NC) T[(unsigned)node_off<SA, SB>(kg, c, nzz, nxx)] = val;
+    }
   }
 }
 
@@ -439,7 +492,7 @@ template <int SA, int SB, bool EQH, bool SKEW, bool FIRST>
 __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ shots,
                                                     const int4* __restrict__ tiles, int nzz, EikConsts k,
                                                     SkewView sv) {
-  __shared__ float sT[65 * RING_LD];  // row 64: landing slot of the last step's read-ahead
+  __shared__ float sT[64 * RING_LD];
   __shared__ float sS[SKEW ? 1 : 65 * RING_LD];
   __shared__ float sH[65];
   const PersistCtl ctl{};
@@ -454,7 +507,7 @@ template <int SA, int SB, bool EQH, bool SKEW, bool FIRST>
 __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict__ shots,
                                                       const int4* __restrict__ tiles, int nzz, EikConsts k,
                                                       SkewView sv, PersistCtl ctl) {
-  __shared__ float sT[65 * RING_LD];
+  __shared__ float sT[64 * RING_LD];
   __shared__ float sS[SKEW ? 1 : 65 * RING_LD];
   __shared__ float sH[65];
   const unsigned tagb = (*ctl.epoch << 11) | (ctl.sweep_id << 8);
