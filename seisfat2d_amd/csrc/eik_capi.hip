@@ -285,6 +285,15 @@ void build_tiles(seisfat_batch_s* b, std::vector<int4>& tiles) {
     NBs[k] = (NC + 63 + 63) / 64;
     Dmax = std::max(Dmax, 2 * (NRS - 1) + NBs[k]);
   }
+  // Scheduling priority of a shot's tiles from its slack: a shot with fewer
+  // tile diagonals than the widest one can fall behind by that many tile
+  // times without delaying the sweep, so the widest shots' tiles (the
+  // critical path) issue first on a shared SIMD.
+  std::vector<int> prio(b->nshots);
+  for (int k = 0; k < b->nshots; ++k) {
+    const int slack = Dmax - (2 * (NRS - 1) + NBs[k]);
+    prio[k] = slack <= 6 ? 3 : slack <= 16 ? 2 : slack <= 30 ? 1 : 0;
+  }
   b->diag_off.assign(1, 0);
   tiles.clear();
   for (int d = 0; d < Dmax; ++d) {
@@ -292,7 +301,7 @@ void build_tiles(seisfat_batch_s* b, std::vector<int4>& tiles) {
       const int bb = d - 2 * r;
       if (bb < 0) break;
       for (int k = 0; k < b->nshots; ++k)
-        if (bb < NBs[k]) tiles.push_back(make_int4(k, r, bb, 0));
+        if (bb < NBs[k]) tiles.push_back(make_int4(k, r, bb, prio[k]));
     }
     b->diag_off.push_back((int)tiles.size());
   }

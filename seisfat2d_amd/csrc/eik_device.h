@@ -187,7 +187,7 @@ template <bool EQH>
 __device__ __forceinline__ CellQ cell_q(float s, const EikConsts& k) {
   CellQ c;
   c.dzs = s * k.dz;
-  c.dxs = s * k.dx;
+  c.dxs = EQH ? c.dzs : s * k.dx;  // EQH: dx == dz
   c.s4 = 4.0f * s * s * k.sum;
   c.thrB = rdiv(c.dzs * k.dz, k.hyp, k.r_hyp);
   c.thrC = EQH ? c.thrB : rdiv(c.dxs * k.dx, k.hyp, k.r_hyp);
@@ -216,9 +216,9 @@ __device__ __forceinline__ float eik_local_sel(float tv, float te, float tev, co
   const bool cC = (dv <= r.thrC) & (dv > 0.0f);
   // branch B precedes C (the reference's else-if order)
   const float dd = cB ? de : dv;
-  const float h1 = cB ? k.dz : k.dx;
-  const float r1 = cB ? k.r_dz : k.r_dx;
-  const float h2 = cB ? k.dx : k.dz;
+  const float h1 = EQH ? k.dz : (cB ? k.dz : k.dx);
+  const float r1 = EQH ? k.r_dz : (cB ? k.r_dz : k.r_dx);
+  const float h2 = EQH ? k.dz : (cB ? k.dx : k.dz);
   const float t0 = cB ? te : tv;
   const float q = rdiv(dd, h1, r1);
   const float ta = tev + te - tv;

@@ -132,17 +132,34 @@ __device__ __forceinline__ float cell_at(const ShotDev& sd, int nzz, int kap, in
   return s_at(sd, nzz, i, j);
 }
 
-constexpr int RING_LD = 66;  // LDS row stride (words): 64 lanes + slowness row 64 + pad
-constexpr int PF = 8;        // slowness prefetch distance (steps) on the skewed path
+constexpr int RING_LD = 64;  // LDS row stride (words) of a tile's traveltimes
+constexpr int SS_LD = 66;    // LDS row stride of staged slowness (64 lanes + cell row 64 + pad)
+constexpr int PF = 16;       // slowness prefetch distance (steps) on the skewed path
 
 // A tile's traveltimes in LDS are instruction-major: row v holds, for lane k,
 // the node of step (v + k) & 63 (the order in which 64 coalesced column loads
-// deliver them).  Lane k's node of step t is therefore at row (t - k) & 63;
-// with the 66-word stride a step's 64 accesses fall in distinct banks within
-// each half-wave.
+// deliver them).  Lane k's node of step t is therefore at row (t - k) & 63,
+// word k: a step's 64 accesses fall in distinct banks within each half-wave.
 __device__ __forceinline__ int trow(int t, int lane) { return ((t - lane) & 63) * RING_LD + lane; }
+// Word index of the same lane's node one step later (rows wrap mod 64).
+__device__ __forceinline__ int trow_next(int ta) { return (ta + RING_LD) & (64 * RING_LD - 1); }
+
+// wave_shl:1 (DPP control 0x130): lane k receives lane k+1's `src`; lane 63
+// keeps `old`.  Shifts the upwind row one step along: lane 0 holds step t's.
+__device__ __forceinline__ float dpp_shl1(float old, float src) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x130,
+                                                    0xf, 0xf, false));
+}
 
 typedef __attribute__((address_space(3))) float lfloat;
+
+// A tile is one wave, whose LDS operations execute in order: passing data
+// between its lanes through LDS needs no barrier, only that the compiler keep
+// the accesses in program order (and the LDS counter drained).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt untouched
+  asm volatile("" ::: "memory");
+}
 
 template <int SA>
 __device__ __forceinline__ int prow(int kk, int nzz) {
@@ -168,18 +185,21 @@ __device__ __forceinline__ int skew_addr(const ShotDev& sd, int nxx, int NC, int
 // One step of a tile: lane k updates sweep-local node (kg, c0 + t - k).  The
 // step's old traveltime, upwind-row value and slowness cells are loaded by the
 // caller ahead of time (software pipelining across the solver's branches).
-template <bool EQH>
-__device__ __forceinline__ void tile_step(float* __restrict__ sT, int t, int lane, bool row_ok, int c0, int NC,
+// ALLCOLS: every column the tile visits is an interior node column, so only
+// the row test remains.
+template <bool EQH, bool ALLCOLS>
+__device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t, int lane, bool row_ok, int c0,
+                                          int NC,
                                           float told, float h, float szo, float sxo, float& te, float& tev,
                                           CellQ& sref, const EikConsts& k) {
   const CellQ zo = cell_q<EQH>(szo, k);  // this step's z-other cell, next step's reference cell
   const float tv = dpp_shr1(h, te);
   const int c = c0 + t - lane;
-  const bool act = row_ok && (unsigned)c < (unsigned)NC;
+  const bool act = ALLCOLS ? row_ok : (row_ok && (unsigned)c < (unsigned)NC);
   const float cand = eik_local_sel<EQH>(tv, te, tev, sref, zo.dzs, k.dx * sxo, k);
   const float m = vmin(told, cand);  // unconditional: a select, not a branch
   const float res = act ? m : told;
-  sT[trow(t, lane)] = res;
+  sT[ta] = res;
   tev = tv;
   te = res;
   sref = zo;
@@ -187,13 +207,14 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int t, int lan
 
 // The 64 steps of a tile with slowness from the skewed global copy, loaded PF
 // steps ahead (loads past step 63 stay inside the copy's margin and are
-// unused).  FAST: the tile lies inside its window's unreplicated columns, so
+// unused).  FAST: the tile lies inside its window's unreplicated columns (so
+// also inside the interior node columns), so
 // at step t every lane's two cells share one skewed column and the addresses
 // advance by a uniform stride; otherwise each lane computes its clamped
 // address.
 template <int SB, bool EQH, bool FAST>
 __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewView& sv, float* __restrict__ sT,
-                                                const float* __restrict__ sH, int lane, int kap, bool row_ok,
+                                                float hrow, int lane, int kap, bool row_ok,
                                                 int c0, int nxx, int NC, float& te, float& tev, CellQ& sref,
                                                 const EikConsts& k) {
   const gfloat* __restrict__ SK = gptr(sv.base);
@@ -206,22 +227,17 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
   // x-other base sits one skew period lower so every lane offset is >= 0)
   const unsigned offz = (unsigned)kap;
   const unsigned offx = (unsigned)(kap + 1 + 64 * KP + ((lane == 63) ? -SB * 64 * KP : 0));
-  const gfloat* pz = SK + colbase0;  // fast path: step t's column, advanced one step per load
-  const gfloat* px = pz - 64 * KP;
+  // fast path: raw buffer loads, per-lane byte offset + per-step uniform
+  // byte offset (one scalar add per step instead of 64-bit address math)
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)sv.base, 0, 0x7fffffff, 0x00020000);
+  const int voz = (int)(offz * 4u), vox = (int)(offx * 4u);
   auto load_z = [&](int t) -> float {
-    if (FAST) {
-      const float v = pz[offz];
-      pz += SB * KP;
-      return v;
-    }
+    if (FAST) return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voz, (colbase0 + SB * t * KP) * 4, 0));
     return SK[skew_addr<SB>(sd, nxx, NC, KP, kap, c0 + t - lane + 1)];
   };
   auto load_x = [&](int t) -> float {
-    if (FAST) {
-      const float v = px[offx];
-      px += SB * KP;
-      return v;
-    }
+    if (FAST)
+      return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, vox, (colbase0 + SB * t * KP - 64 * KP) * 4, 0));
     return SK[skew_addr<SB>(sd, nxx, NC, KP, kap + 1, c0 + t - lane)];
   };
   float zq[PF], xq[PF];
@@ -230,18 +246,21 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
     zq[u] = load_z(u);
     xq[u] = load_x(u);
   }
-  float told_n = sT[trow(0, lane)], h_n = sH[0];
+  int ta = trow(0, lane);
+  float told_n = sT[ta];
   for (int t0 = 0; t0 < 64; t0 += PF) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const int t = t0 + u;
-      const float told = told_n, h = h_n;
-      told_n = sT[trow(t + 1, lane)];
-      h_n = sH[t + 1];
+      const float told = told_n, h = hrow;
+      const int tn = trow_next(ta);
+      told_n = sT[tn];
+      hrow = dpp_shl1(hrow, hrow);
       const float szo = zq[u], sxo = xq[u];
       zq[u] = load_z(t + PF);
       xq[u] = load_x(t + PF);
-      tile_step<EQH>(sT, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+      tile_step<EQH, FAST>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+      ta = tn;
     }
   }
 }
@@ -313,11 +332,11 @@ struct PersistCtl {
 // upwind row and the carry-in from block bb-1 arrive through the tagged
 // exchange buffers of the shot (and the tile publishes its own), instead of
 // being read from T after a launch boundary.
-template <int SA, int SB, bool EQH, bool SKEW, bool FIRST, bool PERSIST>
+template <int SA, int SB, bool EQH, bool SKEW, bool FIRST, bool PERSIST, class BeforeWriteBack>
 __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __restrict__ shots, int nzz,
                                              const EikConsts& k, const SkewView& sv, const PersistCtl& ctl,
                                              unsigned tagb, float* __restrict__ sT, float* __restrict__ sS,
-                                             float* __restrict__ sH) {
+                                             const BeforeWriteBack& before_wb) {
   const ShotDev sd = shots[tl.x];
   const int r = tl.y;
   const int bb = tl.z;
@@ -370,11 +389,13 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
       const unsigned o0 = (unsigned)(irow + (SB > 0 ? 64 * nzz : 0));
       const unsigned o1 = (unsigned)(irow + (SB > 0 ? 0 : 64 * nzz));
       const gfloat* Tb = T + (jb - (SB > 0 ? 64 : 0)) * nzz;
+#pragma unroll 2
       for (int v = 0; v < 64; ++v) {
         __builtin_amdgcn_global_load_lds(Tb + (lane >= 64 - v ? o1 : o0), (lfloat*)(sT + v * RING_LD), 4, 0, 0);
         Tb += SB * nzz;
       }
     } else {
+#pragma unroll 2
       for (int v = 0; v < 64; ++v) {
         const int c = min(max(c0 - lane + ((v + lane) & 63), -1), NC);
         __builtin_amdgcn_global_load_lds(T + (unsigned)node_off<SA, SB>(kgc, c, nzz, nxx),
@@ -384,14 +405,15 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0), lgkmcnt(0): the column loads have landed in LDS
   }
 
-  // ---- upwind row above the strip (local row 64r-1) for lane 0, and the
-  // carry-in from block bb-1: te = T(kg, c0-k-1), tev = T(kg-1, c0-k-1).
-  float te, tev;
+  // ---- upwind row above the strip (local row 64r-1; lane t holds column
+  // c0+t, the value lane 0 needs at step t), and the carry-in from block
+  // bb-1: te = T(kg, c0-k-1), tev = T(kg-1, c0-k-1).
+  float te, tev, hrow;
   if (PERSIST) {
     // halo: strip r-1's bottom row; lane t fetches column c0+t, lane 0 also c0-1
     const bool hneed = r > 0 && c0 + lane < NC;
     const unsigned long long* xh = sd.xh + (size_t)max(r - 1, 0) * sd.ldh;
-    sH[lane] = xwait(xh + min(c0 + lane, NC - 1), hneed, tagb, ctl.err);
+    hrow = xwait(xh + min(c0 + lane, NC - 1), hneed, tagb, ctl.err);
     const bool h1need = r > 0 && bb > 0 && lane == 0 && c0 - 1 < NC;
     const float hprev = xwait(xh + min(max(c0 - 1, 0), NC - 1), h1need, tagb, ctl.err);
     // carries of block bb-1: word 2k = lane k's step-63 result, 2k+1 = step 62
@@ -402,7 +424,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     const float tprev = xwait(xc + 2 * max(lane - 1, 0) + 1, cneed && lane > 0, ctag, ctl.err);
     tev = (lane == 0) ? hprev : tprev;
   } else {
-    sH[lane] = T[node_off<SA, SB>(64 * r - 1, min(c0 + lane, NC), nzz, nxx)];
+    hrow = T[node_off<SA, SB>(64 * r - 1, min(c0 + lane, NC), nzz, nxx)];
     const int cin = min(max(c0 - lane - 1, -1), NC);
     te = T[node_off<SA, SB>(kgc, cin, nzz, nxx)];
     tev = T[node_off<SA, SB>(min(kg - 1, NR), cin, nzz, nxx)];
@@ -411,14 +433,14 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   if (SKEW) {
     const int kap = 64 * r + lane;
     CellQ sref = cell_q<EQH>(gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 - lane)], k);
-    __syncthreads();
+    wave_lds_sync();
     // unreplicated cell columns: lam in [3, w+2] (SB > 0) or [2, w+1] (SB < 0)
     const int lam_lo = max(0, c0 - 63), lam_hi = min(NC, c0 + 64);
     const bool fast = (SB > 0) ? (lam_lo >= 3 && lam_hi <= sd.w + 2) : (lam_lo >= 2 && lam_hi <= sd.w + 1);
     if (fast)
-      tile_steps_skew<SB, EQH, true>(sd, sv, sT, sH, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, true>(sd, sv, sT, hrow, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
     else
-      tile_steps_skew<SB, EQH, false>(sd, sv, sT, sH, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, false>(sd, sv, sT, hrow, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
   } else {
     // Slowness cells step-major: sS[t][kap] = cell (64r+kap, c0+t-kap+1),
     // kap = 0..64; lane k reads its z-other at [t][k] and x-other at [t][k+1].
@@ -433,24 +455,27 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     const float s64 = cell_at<SA, SB>(sd, nzz, min(64 * r + 64, NR), lam64);
     CellQ sref = cell_q<EQH>(cell_at<SA, SB>(sd, nzz, kgc, min(max(c0 - lane, 0), NC)), k);
 #pragma unroll
-    for (int v = 0; v < 64; ++v) sS[((v + lane - 1) & 63) * RING_LD + lane] = sv_[v];
-    sS[((lane + 63) & 63) * RING_LD + 64] = s64;
-    __syncthreads();
-    float told_n = sT[trow(0, lane)], h_n = sH[0], z_n = sS[lane], x_n = sS[lane + 1];
+    for (int v = 0; v < 64; ++v) sS[((v + lane - 1) & 63) * SS_LD + lane] = sv_[v];
+    sS[((lane + 63) & 63) * SS_LD + 64] = s64;
+    wave_lds_sync();
+    int ta = trow(0, lane);
+    float told_n = sT[ta], z_n = sS[lane], x_n = sS[lane + 1];
     for (int t0 = 0; t0 < 64; t0 += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int t = t0 + u;
-        const float told = told_n, h = h_n, szo = z_n, sxo = x_n;
-        told_n = sT[trow(t + 1, lane)];
-        h_n = sH[t + 1];
-        z_n = sS[(t + 1) * RING_LD + lane];
-        x_n = sS[(t + 1) * RING_LD + lane + 1];
-        tile_step<EQH>(sT, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+        const float told = told_n, h = hrow, szo = z_n, sxo = x_n;
+        const int tn = trow_next(ta);
+        told_n = sT[tn];
+        hrow = dpp_shl1(hrow, hrow);
+        z_n = sS[(t + 1) * SS_LD + lane];
+        x_n = sS[(t + 1) * SS_LD + lane + 1];
+        tile_step<EQH, false>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+        ta = tn;
       }
     }
   }
-  __syncthreads();
+  wave_lds_sync();
 
   if (PERSIST) {
     // publish: lane 63's results are strip r+1's upwind row (columns c0+t-63),
@@ -465,6 +490,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     xstore(xc + 2 * lane + 1, sT[trow(62, lane)], ctag);
   }
 
+  before_wb();
   // write-back, row v of LDS = the nodes of column loads v
   if (interior) {
     const int irow = (SA > 0) ? 1 + kgc : nzz - 2 - kgc;
@@ -493,10 +519,9 @@ __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ 
                                                     const int4* __restrict__ tiles, int nzz, EikConsts k,
                                                     SkewView sv) {
   __shared__ float sT[64 * RING_LD];
-  __shared__ float sS[SKEW ? 1 : 65 * RING_LD];
-  __shared__ float sH[65];
+  __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
   const PersistCtl ctl{};
-  tile_process<SA, SB, EQH, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, sH);
+  tile_process<SA, SB, EQH, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, [] {});
 }
 
 // Persistent sweep: every workgroup claims tiles from a ticket counter in
@@ -508,16 +533,29 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
                                                       const int4* __restrict__ tiles, int nzz, EikConsts k,
                                                       SkewView sv, PersistCtl ctl) {
   __shared__ float sT[64 * RING_LD];
-  __shared__ float sS[SKEW ? 1 : 65 * RING_LD];
-  __shared__ float sH[65];
+  __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
   const unsigned tagb = (*ctl.epoch << 11) | (ctl.sweep_id << 8);
-  for (;;) {
-    int tk = 0;
-    if (threadIdx.x == 0) tk = atomicAdd(ctl.counter, 1);
-    tk = __builtin_amdgcn_readfirstlane(tk);
-    if (tk >= ctl.ntiles) break;
-    tile_process<SA, SB, EQH, SKEW, FIRST, true>(tiles[tk], shots, nzz, k, sv, ctl, tagb, sT, sS, sH);
-    __syncthreads();
+  auto claim = [&]() -> int {
+    int t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(ctl.counter, 1);
+    return __builtin_amdgcn_readfirstlane(t);
+  };
+  int tk = claim();
+  while (tk < ctl.ntiles) {
+    const int4 tl = tiles[tk];
+    // wave priority from the tile's scheduling slack (0 = most slack)
+    switch (tl.w) {
+      case 3: __builtin_amdgcn_s_setprio(3); break;
+      case 2: __builtin_amdgcn_s_setprio(2); break;
+      case 1: __builtin_amdgcn_s_setprio(1); break;
+      default: __builtin_amdgcn_s_setprio(0); break;
+    }
+    // the next ticket is claimed before this tile's write-back stores are
+    // issued, so waiting for it does not wait for them
+    int next = 0;
+    tile_process<SA, SB, EQH, SKEW, FIRST, true>(tl, shots, nzz, k, sv, ctl, tagb, sT, sS,
+                                                 [&] { next = claim(); });
+    tk = next;
   }
 }
 
