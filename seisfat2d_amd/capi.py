@@ -24,7 +24,7 @@ EXPORTS = [
     "seisfat_batch_device_picks", "seisfat_batch_device_T", "seisfat_batch_copy_T",
     "seisfat_batch_num_launches", "seisfat_batch_num_sweep_launches", "seisfat_batch_sweep_ms",
     "seisfat_batch_status", "seisfat_batch_destroy", "seisfat_batch_run_record", "seisfat_batch_vjp_device",
-    "seisfat_batch_misfit_gradient",
+    "seisfat_batch_misfit_gradient", "seisfat_eik_fast_division",
 ]
 
 
@@ -61,6 +61,7 @@ def lib():
         L.seisfat_eik_device_T.restype = vp
         L.seisfat_eik_device_T.argtypes = [vp]
         L.seisfat_eik_copy_T.argtypes = [vp, vp]
+        L.seisfat_eik_fast_division.argtypes = [vp]
         L.seisfat_batch_create.argtypes = [vp, i, i, i, C.POINTER(Shot), i, vp, vp, C.POINTER(vp)]
         L.seisfat_batch_run.argtypes = [vp, vp]
         L.seisfat_batch_picks.argtypes = [vp, vp]

@@ -171,6 +171,17 @@ EikConsts make_consts(float dx, float dz) {
   k.hyp = std::sqrt(dx * dx + dz * dz);
   k.sum = k.dz2i + k.dx2i;
   k.prod = k.dz2i * k.dx2i;
+  k.fastdiv = 0;
+  // zh = RN(1/b); 1 - b*zh is exact (one fma), so zl = RN((1 - b*zh) / b)
+  // = RN(1/b - zh)
+  auto two_word = [](float b, float& zh, float& zl) {
+    zh = 1.0f / b;
+    zl = std::fma(-b, zh, 1.0f) / b;
+  };
+  two_word(k.sum, k.zh_sum, k.zl_sum);
+  two_word(k.hyp, k.zh_hyp, k.zl_hyp);
+  two_word(k.dz, k.zh_dz, k.zl_dz);
+  two_word(k.dx, k.zh_dx, k.zl_dx);
   return k;
 }
 
@@ -753,17 +764,25 @@ int seisfat_eik_create(int nz, int nx, float dz, float dx, seisfat_eik_t* out) {
   }
   // refined reciprocals of the local solver's divisors, computed on the device
   {
-    const float divs[4] = {h->kc.sum, h->kc.hyp, h->kc.dz, h->kc.dx};
+    // divisors, their two-word reciprocals, refined reciprocals, check flags
+    const EikConsts& c = h->kc;
+    const float divs[12] = {c.sum,    c.hyp,    c.dz,    c.dx,    c.zh_sum, c.zh_hyp,
+                            c.zh_dz,  c.zh_dx,  c.zl_sum, c.zl_hyp, c.zl_dz,  c.zl_dx};
     float rec[4] = {};
+    int bad[4] = {1, 1, 1, 1};
     float* d = nullptr;
-    hipError_t e2 = hipMalloc((void**)&d, 8 * sizeof(float));
+    hipError_t e2 = hipMalloc((void**)&d, 20 * sizeof(float));
     if (e2 == hipSuccess) e2 = hipMemcpy(d, divs, sizeof(divs), hipMemcpyHostToDevice);
+    if (e2 == hipSuccess) e2 = hipMemset(d + 16, 0, 4 * sizeof(int));
     if (e2 == hipSuccess) {
-      launch_recip(d, d + 4, 4, h->stream);
+      launch_recip(d, d + 12, 4, h->stream);
+      launch_fastdiv_check(d, d + 4, d + 8, 4, (int*)(d + 16), h->stream);
       e2 = hipStreamSynchronize(h->stream);
     }
-    if (e2 == hipSuccess) e2 = hipMemcpy(rec, d + 4, sizeof(rec), hipMemcpyDeviceToHost);
+    if (e2 == hipSuccess) e2 = hipMemcpy(rec, d + 12, sizeof(rec), hipMemcpyDeviceToHost);
+    if (e2 == hipSuccess) e2 = hipMemcpy(bad, d + 16, sizeof(bad), hipMemcpyDeviceToHost);
     if (d) (void)hipFree(d);
+    h->kc.fastdiv = (bad[0] | bad[1] | bad[2] | bad[3]) == 0;
     if (e2 != hipSuccess) {
       seisfat_eik_destroy(h);
       return fail(SEISFAT_EHIP, std::string("reciprocal setup: ") + hipGetErrorString(e2));
@@ -892,6 +911,11 @@ int seisfat_eik_solve(seisfat_eik_t h, int type, int schedule, float sx, float s
   if (T_padded) HIPCHK(hipMemcpy(T_padded, b->d_T, b->tsize * sizeof(float), hipMemcpyDeviceToHost));
   if (nrec > 0) HIPCHK(hipMemcpy(picks, b->d_picks, nrec * sizeof(float), hipMemcpyDeviceToHost));
   return SEISFAT_OK;
+}
+
+int seisfat_eik_fast_division(seisfat_eik_t h) {
+  if (int rc = check_handle(h)) return rc;
+  return h->kc.fastdiv ? 1 : 0;
 }
 
 const float* seisfat_eik_device_T(seisfat_eik_t h) {

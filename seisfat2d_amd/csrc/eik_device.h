@@ -28,6 +28,12 @@ struct EikConsts {
   // the device by k_recip so they are exactly the values the hardware
   // division sequence would build
   float r_sum, r_hyp, r_dz, r_dx;
+  // two-word reciprocals zh = RN(1/b), zl = RN(1/b - zh) of the four
+  // divisors (see fdiv below)
+  float zh_sum, zl_sum, zh_hyp, zl_hyp, zh_dz, zl_dz, zh_dx, zl_dx;
+  // host-side: fdiv reproduces the correctly rounded quotient for all four
+  // divisors (checked exhaustively when the geometry is created)
+  int fastdiv;
 };
 
 // ---------------------------------------------------------------------------
@@ -52,6 +58,18 @@ __device__ __forceinline__ float rdiv(float a, float b, float r) {
   const float q1 = __builtin_fmaf(e1, r, q0);
   const float e2 = __builtin_fmaf(-b, q1, a);
   return __builtin_fmaf(e2, r, q1);
+}
+
+// Division by a known divisor b in two instructions (Brisebarre, Muller and
+// Raina, "Accelerating correctly rounded floating-point division when the
+// divisor is known in advance", IEEE Trans. Computers 53(8), 2004):
+//   q = RN(a * zh + RN(a * zl)),  zh = RN(1/b),  zl = RN(1/b - zh).
+// It returns RN(a/b) for every a of most divisors but not of all, so the
+// kernels use it only for a geometry whose four divisors passed an exhaustive
+// check against IEEE division over every dividend of magnitude in
+// [2^-100, 2^100) (k_fastdiv_check); otherwise they use rdiv.
+__device__ __forceinline__ float fdiv(float a, float zh, float zl) {
+  return __builtin_fmaf(a, zh, a * zl);
 }
 
 __device__ __forceinline__ float rsqrt_rn(float x) {
@@ -183,14 +201,14 @@ struct CellQ {
   float s2;    // s * s
 };
 
-template <bool EQH>
+template <bool EQH, bool FD>
 __device__ __forceinline__ CellQ cell_q(float s, const EikConsts& k) {
   CellQ c;
   c.dzs = s * k.dz;
   c.dxs = EQH ? c.dzs : s * k.dx;  // EQH: dx == dz
   c.s4 = 4.0f * s * s * k.sum;
-  c.thrB = rdiv(c.dzs * k.dz, k.hyp, k.r_hyp);
-  c.thrC = EQH ? c.thrB : rdiv(c.dxs * k.dx, k.hyp, k.r_hyp);
+  c.thrB = FD ? fdiv(c.dzs * k.dz, k.zh_hyp, k.zl_hyp) : rdiv(c.dzs * k.dz, k.hyp, k.r_hyp);
+  c.thrC = EQH ? c.thrB : (FD ? fdiv(c.dxs * k.dx, k.zh_hyp, k.zl_hyp) : rdiv(c.dxs * k.dx, k.hyp, k.r_hyp));
   c.s2 = s * s;
   return c;
 }
@@ -201,8 +219,8 @@ __device__ __forceinline__ CellQ cell_q(float s, const EikConsts& k) {
 // h > 0, RN(h * min(a, b)) = min(RN(h * a), RN(h * b)), so the 1-D candidates
 // can use the cells' precomputed products.  Bit-identical to eik_local: the
 // selected lanes evaluate the same operations in the same order; the others
-// compute values that the selects discard.
-template <bool EQH>
+// compute values that the selects discard.  FD: divisions by fdiv.
+template <bool EQH, bool FD>
 __device__ __forceinline__ float eik_local_sel(float tv, float te, float tev, const CellQ& r, float dzs_zo,
                                                float dxs_xo, const EikConsts& k) {
   const float t1d1 = tv + vmin(r.dzs, dzs_zo);
@@ -218,9 +236,11 @@ __device__ __forceinline__ float eik_local_sel(float tv, float te, float tev, co
   const float dd = cB ? de : dv;
   const float h1 = EQH ? k.dz : (cB ? k.dz : k.dx);
   const float r1 = EQH ? k.r_dz : (cB ? k.r_dz : k.r_dx);
+  const float zh1 = EQH ? k.zh_dz : (cB ? k.zh_dz : k.zh_dx);
+  const float zl1 = EQH ? k.zl_dz : (cB ? k.zl_dz : k.zl_dx);
   const float h2 = EQH ? k.dz : (cB ? k.dx : k.dz);
   const float t0 = cB ? te : tv;
-  const float q = rdiv(dd, h1, r1);
+  const float q = FD ? fdiv(dd, zh1, zl1) : rdiv(dd, h1, r1);
   const float ta = tev + te - tv;
   const float tb = tev - te + tv;
   const float dab = ta - tb;
@@ -229,7 +249,8 @@ __device__ __forceinline__ float eik_local_sel(float tv, float te, float tev, co
   const float rb = cA ? k.prod * dab : q;
   const float rc = cA ? dab : q;
   const float sq = rsqrt_rn(ra - rb * rc);
-  const float tA = rdiv((tb * k.dz2i + ta * k.dx2i) + sq, k.sum, k.r_sum);
+  const float num = (tb * k.dz2i + ta * k.dx2i) + sq;
+  const float tA = FD ? fdiv(num, k.zh_sum, k.zl_sum) : rdiv(num, k.sum, k.r_sum);
   const float tBC = t0 + h2 * sq;
   const float t2D = cA ? tA : ((cB | cC) ? tBC : TINF);
   return vmin(t1D, vmin(t2D, TINF));

@@ -19,6 +19,9 @@ struct StiffDev {
 
 void launch_fill(float* p, size_t n, float v, hipStream_t st);
 void launch_recip(const float* b, float* r, int n, hipStream_t st);
+// bad[d] |= 1 unless fdiv(a, zh[d], zl[d]) == a / b[d] for every dividend of
+// magnitude in [2^-100, 2^100)
+void launch_fastdiv_check(const float* b, const float* zh, const float* zl, int n, int* bad, hipStream_t st);
 void launch_zpad(const float* s, float* szp, int nz, int nx, hipStream_t st);
 void launch_check_positive(const float* s, size_t n, float hmin, float hmax, int* bad, hipStream_t st);
 void launch_init(const ShotDev* shots, int nshots, int nzz, float dx, float dz, hipStream_t st);

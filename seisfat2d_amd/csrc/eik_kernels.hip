@@ -187,16 +187,16 @@ __device__ __forceinline__ int skew_addr(const ShotDev& sd, int nxx, int NC, int
 // caller ahead of time (software pipelining across the solver's branches).
 // ALLCOLS: every column the tile visits is an interior node column, so only
 // the row test remains.
-template <bool EQH, bool ALLCOLS>
+template <bool EQH, bool FD, bool ALLCOLS>
 __device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t, int lane, bool row_ok, int c0,
                                           int NC,
                                           float told, float h, float szo, float sxo, float& te, float& tev,
                                           CellQ& sref, const EikConsts& k) {
-  const CellQ zo = cell_q<EQH>(szo, k);  // this step's z-other cell, next step's reference cell
+  const CellQ zo = cell_q<EQH, FD>(szo, k);  // this step's z-other cell, next step's reference cell
   const float tv = dpp_shr1(h, te);
   const int c = c0 + t - lane;
   const bool act = ALLCOLS ? row_ok : (row_ok && (unsigned)c < (unsigned)NC);
-  const float cand = eik_local_sel<EQH>(tv, te, tev, sref, zo.dzs, k.dx * sxo, k);
+  const float cand = eik_local_sel<EQH, FD>(tv, te, tev, sref, zo.dzs, k.dx * sxo, k);
   const float m = vmin(told, cand);  // unconditional: a select, not a branch
   const float res = act ? m : told;
   sT[ta] = res;
@@ -212,7 +212,7 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t,
 // at step t every lane's two cells share one skewed column and the addresses
 // advance by a uniform stride; otherwise each lane computes its clamped
 // address.
-template <int SB, bool EQH, bool FAST>
+template <int SB, bool EQH, bool FD, bool FAST>
 __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewView& sv, float* __restrict__ sT,
                                                 float hrow, int lane, int kap, bool row_ok,
                                                 int c0, int nxx, int NC, float& te, float& tev, CellQ& sref,
@@ -259,7 +259,7 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
       const float szo = zq[u], sxo = xq[u];
       zq[u] = load_z(t + PF);
       xq[u] = load_x(t + PF);
-      tile_step<EQH, FAST>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+      tile_step<EQH, FD, FAST>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
       ta = tn;
     }
   }
@@ -332,7 +332,7 @@ struct PersistCtl {
 // upwind row and the carry-in from block bb-1 arrive through the tagged
 // exchange buffers of the shot (and the tile publishes its own), instead of
 // being read from T after a launch boundary.
-template <int SA, int SB, bool EQH, bool SKEW, bool FIRST, bool PERSIST, class BeforeWriteBack>
+template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST, class BeforeWriteBack>
 __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __restrict__ shots, int nzz,
                                              const EikConsts& k, const SkewView& sv, const PersistCtl& ctl,
                                              unsigned tagb, float* __restrict__ sT, float* __restrict__ sS,
@@ -432,15 +432,15 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
 
   if (SKEW) {
     const int kap = 64 * r + lane;
-    CellQ sref = cell_q<EQH>(gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 - lane)], k);
+    CellQ sref = cell_q<EQH, FD>(gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 - lane)], k);
     wave_lds_sync();
     // unreplicated cell columns: lam in [3, w+2] (SB > 0) or [2, w+1] (SB < 0)
     const int lam_lo = max(0, c0 - 63), lam_hi = min(NC, c0 + 64);
     const bool fast = (SB > 0) ? (lam_lo >= 3 && lam_hi <= sd.w + 2) : (lam_lo >= 2 && lam_hi <= sd.w + 1);
     if (fast)
-      tile_steps_skew<SB, EQH, true>(sd, sv, sT, hrow, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, FD, true>(sd, sv, sT, hrow, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
     else
-      tile_steps_skew<SB, EQH, false>(sd, sv, sT, hrow, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, FD, false>(sd, sv, sT, hrow, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
   } else {
     // Slowness cells step-major: sS[t][kap] = cell (64r+kap, c0+t-kap+1),
     // kap = 0..64; lane k reads its z-other at [t][k] and x-other at [t][k+1].
@@ -453,7 +453,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     }
     const int lam64 = min(max(c0 + 1 - 64 + ((lane + 63) & 63), 0), NC);
     const float s64 = cell_at<SA, SB>(sd, nzz, min(64 * r + 64, NR), lam64);
-    CellQ sref = cell_q<EQH>(cell_at<SA, SB>(sd, nzz, kgc, min(max(c0 - lane, 0), NC)), k);
+    CellQ sref = cell_q<EQH, FD>(cell_at<SA, SB>(sd, nzz, kgc, min(max(c0 - lane, 0), NC)), k);
 #pragma unroll
     for (int v = 0; v < 64; ++v) sS[((v + lane - 1) & 63) * SS_LD + lane] = sv_[v];
     sS[((lane + 63) & 63) * SS_LD + 64] = s64;
@@ -470,7 +470,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
         hrow = dpp_shl1(hrow, hrow);
         z_n = sS[(t + 1) * SS_LD + lane];
         x_n = sS[(t + 1) * SS_LD + lane + 1];
-        tile_step<EQH, false>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+        tile_step<EQH, FD, false>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
         ta = tn;
       }
     }
@@ -521,14 +521,15 @@ __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ 
   __shared__ float sT[64 * RING_LD];
   __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
   const PersistCtl ctl{};
-  tile_process<SA, SB, EQH, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, [] {});
+  tile_process<SA, SB, EQH, false, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS,
+                                                       [] {});
 }
 
 // Persistent sweep: every workgroup claims tiles from a ticket counter in
 // tile-diagonal order until the sweep's tiles run out.  A tile only waits on
 // tiles with smaller tickets, which running workgroups hold, so the sweep
 // always drains.
-template <int SA, int SB, bool EQH, bool SKEW, bool FIRST>
+template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST>
 __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict__ shots,
                                                       const int4* __restrict__ tiles, int nzz, EikConsts k,
                                                       SkewView sv, PersistCtl ctl) {
@@ -553,8 +554,8 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
     // the next ticket is claimed before this tile's write-back stores are
     // issued, so waiting for it does not wait for them
     int next = 0;
-    tile_process<SA, SB, EQH, SKEW, FIRST, true>(tl, shots, nzz, k, sv, ctl, tagb, sT, sS,
-                                                 [&] { next = claim(); });
+    tile_process<SA, SB, EQH, FD, SKEW, FIRST, true>(tl, shots, nzz, k, sv, ctl, tagb, sT, sS,
+                                                     [&] { next = claim(); });
     tk = next;
   }
 }
@@ -668,10 +669,32 @@ __global__ void k_recip(const float* __restrict__ b, float* __restrict__ r, int 
   }
 }
 
+// Exhaustive check of fdiv for the divisors b[0..n): every positive dividend
+// a with 2^-100 <= a < 2^100 (biased exponents 27..226, all 2^23 fractions),
+// against IEEE division.  fdiv is odd in a, so negative dividends follow.
+// bad[d] is set when divisor d fails for some a.
+constexpr unsigned FDIV_LO = 27u << 23, FDIV_HI = 227u << 23;
+__global__ void k_fastdiv_check(const float* __restrict__ b, const float* __restrict__ zh,
+                                const float* __restrict__ zl, int* __restrict__ bad) {
+  const int d = blockIdx.y;  // one divisor per grid row
+  const float bd = b[d], h = zh[d], l = zl[d];
+  const unsigned stride = gridDim.x * blockDim.x;
+  int flag = 0;
+  for (unsigned u = FDIV_LO + blockIdx.x * blockDim.x + threadIdx.x; u < FDIV_HI; u += stride) {
+    const float a = __uint_as_float(u);
+    flag |= __float_as_uint(fdiv(a, h, l)) != __float_as_uint(a / bd);
+  }
+  if (flag) atomicOr(bad + d, 1);
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
 void launch_recip(const float* b, float* r, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_recip, dim3(1), dim3(64), 0, st, b, r, n);
+}
+
+void launch_fastdiv_check(const float* b, const float* zh, const float* zl, int n, int* bad, hipStream_t st) {
+  hipLaunchKernelGGL(k_fastdiv_check, dim3(4096, n), dim3(256), 0, st, b, zh, zl, bad);
 }
 
 void launch_fill(float* p, size_t n, float v, hipStream_t st) {
@@ -738,23 +761,33 @@ int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh
   return launches;
 }
 
+template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST>
+static void persist_launch(const ShotDev* shots, const int4* tiles, int nzz, const EikConsts& k,
+                           const SkewView& sv, const PersistCtl& ctl, hipStream_t st) {
+  static int grid = 0;
+  if (grid == 0) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persist<SA, SB, EQH, FD, SKEW, FIRST>, 64,
+                                                     0) != hipSuccess || per_cu <= 0)
+      per_cu = 4;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    grid = per_cu * cus;
+  }
+  hipLaunchKernelGGL((k_sweep_persist<SA, SB, EQH, FD, SKEW, FIRST>), dim3(std::min(grid, ctl.ntiles)), dim3(64), 0,
+                     st, shots, tiles, nzz, k, sv, ctl);
+}
+
 template <int SA, int SB, bool EQH, bool SKEW, bool FIRST>
 static void tiles_diag(const ShotDev* shots, const int4* tiles, int n, int nzz, const EikConsts& k,
                        const SkewView& sv, const PersistCtl* ctl, hipStream_t st) {
   if (ctl) {
-    static int grid = 0;
-    if (grid == 0) {
-      int per_cu = 0, dev = 0, cus = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persist<SA, SB, EQH, SKEW, FIRST>, 64,
-                                                       0) != hipSuccess || per_cu <= 0)
-        per_cu = 4;
-      (void)hipGetDevice(&dev);
-      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        cus = 256;
-      grid = per_cu * cus;
-    }
-    hipLaunchKernelGGL((k_sweep_persist<SA, SB, EQH, SKEW, FIRST>), dim3(std::min(grid, ctl->ntiles)), dim3(64),
-                       0, st, shots, tiles, nzz, k, sv, *ctl);
+    // the fdiv variant exists for the skewed-slowness (isotropic batch) path
+    if (SKEW && k.fastdiv)
+      persist_launch<SA, SB, EQH, SKEW, SKEW, FIRST>(shots, tiles, nzz, k, sv, *ctl, st);
+    else
+      persist_launch<SA, SB, EQH, false, SKEW, FIRST>(shots, tiles, nzz, k, sv, *ctl, st);
   } else {
     hipLaunchKernelGGL((k_sweep_tiles<SA, SB, EQH, SKEW, FIRST>), dim3(n), dim3(64), 0, st, shots, tiles, nzz, k,
                        sv);

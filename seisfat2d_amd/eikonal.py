@@ -73,6 +73,15 @@ class Eikonal:
     def device_T_ptr(self):
         return lib().seisfat_eik_device_T(self._h)
 
+    @property
+    def fast_division(self):
+        """True when the batched isotropic sweeps use the two-instruction
+        known-divisor division (results are bit-identical either way)."""
+        rc = lib().seisfat_eik_fast_division(self._h)
+        if rc < 0:
+            check(rc)
+        return rc == 1
+
     def batch(self, shots, rx=None, rz=None, type=ISO, schedule=SCHED_TILED):
         return Batch(self, shots, rx, rz, type, schedule)
 

@@ -149,6 +149,41 @@ def test_batch_rerun_after_new_slowness(oracle, rng):
     e.close()
 
 
+# dz = dx = 0x1.94f5a6p+2: the two-instruction division by dz rounds one
+# dividend per binade away from IEEE division, so this geometry must run the
+# five-instruction kernels
+FDIV_FALLBACK_H = float.fromhex("0x1.94f5a6p+2")
+
+
+@pytest.mark.parametrize("dz,dx,fast", [
+    (5.0, 5.0, True),
+    (10.0, 8.0, None),
+    (7.0, 10.0, None),
+    (FDIV_FALLBACK_H, FDIV_FALLBACK_H, False),
+    (FDIV_FALLBACK_H, 5.0, False),
+])
+def test_batch_division_variants_bit_exact(oracle, rng, dz, dx, fast):
+    """Batched isotropic sweeps divide with the two-instruction known-divisor
+    sequence only when all four divisors of the geometry passed the exhaustive
+    check at create; both variants reproduce the oracle bit for bit."""
+    nz, nx = 150, 420
+    s = smooth_slowness(rng, nz, nx, min(dz, dx))
+    shots = [(0.3 * (nx - 1) * dx, 0.0, 0, nx, 0, 0), (0.7 * (nx - 1) * dx, 0.6 * (nz - 1) * dz, 60, 300, 0, 0)]
+    e = Eikonal(nz, nx, dz, dx)
+    if fast is not None:
+        assert e.fast_division == fast
+    print(f"dz={dz!r} dx={dx!r}: fast division {e.fast_division}")
+    e.set_slowness(s)
+    b = e.batch(shots, schedule=SCHED_TILED)
+    b.run()
+    b.status()
+    for k, (sx, sz, x0, wd, _, _) in enumerate(shots):
+        Tr = oracle.solve_windows(s, dx, dz, [sx], [sz], [x0], [wd], nthreads=1)[0]
+        assert np.array_equal(b.T(k), Tr), k
+    b.close()
+    e.close()
+
+
 def ani_setup(nz, nx, dh, nshots, seed=7):
     from seisfat2d_amd.stiffness import compute_stiffness
     rng = np.random.default_rng(seed)
