@@ -272,31 +272,48 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
 __device__ __forceinline__ unsigned long long xpack(float v, unsigned tag) {
   return ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v);
 }
+// global address space: generic (flat) accesses may alias LDS, and the
+// compiler would order each one after the tile's LDS-destination loads
+typedef __attribute__((address_space(1))) unsigned long long gword;
 __device__ __forceinline__ void xstore(unsigned long long* p, float v, unsigned tag) {
-  __hip_atomic_store(p, xpack(v, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store((gword*)p, xpack(v, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 constexpr int SPIN_LIMIT = 1 << 16;  // ~0.1 s of polling (1000x a tile's run time); then flag and go on
 
-// Waits until every lane with `need` sees `tag` at *p; returns the value (1e6
-// for lanes without need).  The wait is bounded: past SPIN_LIMIT polls it sets
-// *err and returns whatever was read, so a broken dependency cannot hang the
-// device.
-__device__ __forceinline__ float xwait(const unsigned long long* p, bool need, unsigned tag, int* err) {
-  unsigned long long w = xpack(TINF, tag);
-  bool ok = !need;
+// Waits until every lane sees, for each of a tile's four dependency words
+// with `need`, the word's `tag`; returns the values (1e6 where not needed).
+// Every poll issues all outstanding loads before testing any tag, so a tile
+// whose predecessors are done pays one memory round trip, not four.  The wait
+// is bounded: past SPIN_LIMIT polls it sets *err and returns whatever was
+// read, so a broken dependency cannot hang the device.
+struct XDep {
+  const unsigned long long* p;
+  bool need;
+  unsigned tag;
+};
+__device__ __forceinline__ void xwait4(const XDep (&d)[4], float (&out)[4], int* err) {
+  unsigned long long w[4];
+  bool ok[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    w[q] = xpack(TINF, d[q].tag);
+    ok[q] = !d[q].need;
+  }
   for (int it = 0;; ++it) {
-    if (!ok) {
-      w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      ok = (unsigned)(w >> 32) == tag;
-    }
-    if (__all(ok)) break;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (!ok[q]) w[q] = __hip_atomic_load((const gword*)d[q].p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ok[q] = ok[q] || (unsigned)(w[q] >> 32) == d[q].tag;
+    if (__all(ok[0] && ok[1] && ok[2] && ok[3])) break;
     if (it > SPIN_LIMIT) {
       if (threadIdx.x == 0) atomicOr(err, 1);
       break;
     }
     __builtin_amdgcn_s_sleep(2);
   }
-  return need ? __uint_as_float((unsigned)w) : TINF;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) out[q] = d[q].need ? __uint_as_float((unsigned)w[q]) : TINF;
 }
 
 // Control block of a persistent sweep launch.
@@ -402,7 +419,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
                                          (lfloat*)(sT + v * RING_LD), 4, 0, 0);
       }
     }
-    __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0), lgkmcnt(0): the column loads have landed in LDS
+    // (waited for below, after the halo and carry loads: their latencies overlap)
   }
 
   // ---- upwind row above the strip (local row 64r-1; lane t holds column
@@ -413,22 +430,28 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     // halo: strip r-1's bottom row; lane t fetches column c0+t, lane 0 also c0-1
     const bool hneed = r > 0 && c0 + lane < NC;
     const unsigned long long* xh = sd.xh + (size_t)max(r - 1, 0) * sd.ldh;
-    hrow = xwait(xh + min(c0 + lane, NC - 1), hneed, tagb, ctl.err);
     const bool h1need = r > 0 && bb > 0 && lane == 0 && c0 - 1 < NC;
-    const float hprev = xwait(xh + min(max(c0 - 1, 0), NC - 1), h1need, tagb, ctl.err);
     // carries of block bb-1: word 2k = lane k's step-63 result, 2k+1 = step 62
     const bool cneed = bb > 0;
     const unsigned long long* xc = sd.xc + (size_t)(2 * r + ((bb - 1) & 1)) * 128;
     const unsigned ctag = tagb | ((unsigned)(bb - 1) & 255u);
-    te = xwait(xc + 2 * lane, cneed, ctag, ctl.err);
-    const float tprev = xwait(xc + 2 * max(lane - 1, 0) + 1, cneed && lane > 0, ctag, ctl.err);
-    tev = (lane == 0) ? hprev : tprev;
+    const XDep deps[4] = {{xh + min(c0 + lane, NC - 1), hneed, tagb},
+                          {xh + min(max(c0 - 1, 0), NC - 1), h1need, tagb},
+                          {xc + 2 * lane, cneed, ctag},
+                          {xc + 2 * max(lane - 1, 0) + 1, cneed && lane > 0, ctag}};
+    float dv[4];
+    xwait4(deps, dv, ctl.err);
+    hrow = dv[0];
+    te = dv[2];
+    tev = (lane == 0) ? dv[1] : dv[3];
   } else {
     hrow = T[node_off<SA, SB>(64 * r - 1, min(c0 + lane, NC), nzz, nxx)];
     const int cin = min(max(c0 - lane - 1, -1), NC);
     te = T[node_off<SA, SB>(kgc, cin, nzz, nxx)];
     tev = T[node_off<SA, SB>(min(kg - 1, NR), cin, nzz, nxx)];
   }
+
+  if (!FIRST) __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0), lgkmcnt(0): the column loads have landed in LDS
 
   if (SKEW) {
     const int kap = 64 * r + lane;
