@@ -12,6 +12,8 @@
 //   k_zpad            the row part of expand_boundary (src/modeling/modeling.cu:204-232)
 #include "eik_kernels.h"
 
+#include <type_traits>
+
 namespace seisfat {
 
 // ---------------------------------------------------------------------------
@@ -206,8 +208,9 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t,
 }
 
 // The 64 steps of a tile with slowness from the skewed global copy, loaded PF
-// steps ahead (loads past step 63 stay inside the copy's margin and are
-// unused).  FAST: the tile lies inside its window's unreplicated columns (so
+// steps ahead (none past step 63: every outstanding load holds one of the
+// wave's 63 memory-counter slots that the tile's write-back and the next
+// tile's staging need).  FAST: the tile lies inside its window's unreplicated columns (so
 // also inside the interior node columns), so
 // at step t every lane's two cells share one skewed column and the addresses
 // advance by a uniform stride; otherwise each lane computes its clamped
@@ -248,7 +251,9 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
   }
   int ta = trow(0, lane);
   float told_n = sT[ta];
-  for (int t0 = 0; t0 < 64; t0 += PF) {
+  // PF steps from t0; the last block is a separate copy without loads, so
+  // nothing is fetched for steps past 63
+  auto block = [&](int t0, auto fetch) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const int t = t0 + u;
@@ -257,12 +262,16 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
       told_n = sT[tn];
       hrow = dpp_shl1(hrow, hrow);
       const float szo = zq[u], sxo = xq[u];
-      zq[u] = load_z(t + PF);
-      xq[u] = load_x(t + PF);
+      if constexpr (decltype(fetch)::value) {
+        zq[u] = load_z(t + PF);
+        xq[u] = load_x(t + PF);
+      }
       tile_step<EQH, FD, FAST>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
       ta = tn;
     }
-  }
+  };
+  for (int t0 = 0; t0 < 64 - PF; t0 += PF) block(t0, std::true_type{});
+  block(64 - PF, std::false_type{});
 }
 
 // Tag words of the persistent schedule's boundary exchange: value bits in the
