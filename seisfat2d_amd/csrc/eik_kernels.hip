@@ -189,7 +189,7 @@ __device__ __forceinline__ int skew_addr(const ShotDev& sd, int nxx, int NC, int
 // caller ahead of time (software pipelining across the solver's branches).
 // ALLCOLS: every column the tile visits is an interior node column, so only
 // the row test remains.
-template <bool EQH, bool FD, bool ALLCOLS>
+template <bool EQH, bool FD, bool ALLCOLS, bool XDPP = false>
 __device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t, int lane, bool row_ok, int c0,
                                           int NC,
                                           float told, float h, float szo, float sxo, float& te, float& tev,
@@ -198,7 +198,10 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t,
   const float tv = dpp_shr1(h, te);
   const int c = c0 + t - lane;
   const bool act = ALLCOLS ? row_ok : (row_ok && (unsigned)c < (unsigned)NC);
-  const float cand = eik_local_sel<EQH, FD>(tv, te, tev, sref, zo.dzs, k.dx * sxo, k);
+  // XDPP: lane k's x-other cell is lane k+1's z-other cell of this step; sxo
+  // is then only lane 63's (times dx when EQH, where zo.dzs = dz*s = dx*s)
+  const float dxs_xo = !XDPP ? k.dx * sxo : (EQH ? dpp_shl1(sxo, zo.dzs) : k.dx * dpp_shl1(sxo, szo));
+  const float cand = eik_local_sel<EQH, FD>(tv, te, tev, sref, zo.dzs, dxs_xo, k);
   const float m = vmin(told, cand);  // unconditional: a select, not a branch
   const float res = act ? m : told;
   sT[ta] = res;
@@ -207,12 +210,15 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t,
   sref = zo;
 }
 
-// The 64 steps of a tile with slowness from the skewed global copy, loaded PF
-// steps ahead (none past step 63: every outstanding load holds one of the
-// wave's 63 memory-counter slots that the tile's write-back and the next
-// tile's staging need).  FAST: the tile lies inside its window's unreplicated columns (so
-// also inside the interior node columns), so
-// at step t every lane's two cells share one skewed column and the addresses
+// The 64 steps of a tile with slowness from the skewed global copy: each lane
+// loads its z-other cell PF steps ahead (none past step 63: every outstanding
+// load holds one of the wave's 63 memory-counter slots that the tile's
+// write-back and the next tile's staging need); its x-other cell is lane
+// k+1's z-other cell of the same step (one DPP move), except lane 63's, which
+// lies in the next strip: those 64 cells are gathered once (lane j holds step
+// 63-j's) and rotate one lane per step.  FAST: the tile lies inside its
+// window's unreplicated columns (so also inside the interior node columns),
+// so at step t every lane's cell shares one skewed column and the addresses
 // advance by a uniform stride; otherwise each lane computes its clamped
 // address.
 template <int SB, bool EQH, bool FD, bool FAST>
@@ -226,29 +232,21 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
   // (row kap+1 = next strip) sits one skew period (64 columns) away.
   const int sig0 = (SB > 0) ? sd.x0 + c0 - 2 : sd.x0 + sd.w + 63 - c0;
   const int colbase0 = (sig0 + SK_OFF) * KP;
-  // fast path: a uniform base per step plus an unsigned per-lane offset (the
-  // x-other base sits one skew period lower so every lane offset is >= 0)
-  const unsigned offz = (unsigned)kap;
-  const unsigned offx = (unsigned)(kap + 1 + 64 * KP + ((lane == 63) ? -SB * 64 * KP : 0));
   // fast path: raw buffer loads, per-lane byte offset + per-step uniform
   // byte offset (one scalar add per step instead of 64-bit address math)
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)sv.base, 0, 0x7fffffff, 0x00020000);
-  const int voz = (int)(offz * 4u), vox = (int)(offx * 4u);
+  const int voz = kap * 4;
   auto load_z = [&](int t) -> float {
     if (FAST) return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voz, (colbase0 + SB * t * KP) * 4, 0));
     return SK[skew_addr<SB>(sd, nxx, NC, KP, kap, c0 + t - lane + 1)];
   };
-  auto load_x = [&](int t) -> float {
-    if (FAST)
-      return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, vox, (colbase0 + SB * t * KP - 64 * KP) * 4, 0));
-    return SK[skew_addr<SB>(sd, nxx, NC, KP, kap + 1, c0 + t - lane)];
-  };
-  float zq[PF], xq[PF];
+  // lane 63's x-other cells (row kap63+1), lane j holding step 63-j's
+  const int kx = kap - lane + 64;
+  float xr = FAST ? SK[colbase0 + kx - SB * KP * (lane + 1)] : SK[skew_addr<SB>(sd, nxx, NC, KP, kx, c0 - lane)];
+  if (EQH) xr = k.dx * xr;
+  float zq[PF];
 #pragma unroll
-  for (int u = 0; u < PF; ++u) {
-    zq[u] = load_z(u);
-    xq[u] = load_x(u);
-  }
+  for (int u = 0; u < PF; ++u) zq[u] = load_z(u);
   int ta = trow(0, lane);
   float told_n = sT[ta];
   // PF steps from t0; the last block is a separate copy without loads, so
@@ -261,12 +259,10 @@ __device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewVie
       const int tn = trow_next(ta);
       told_n = sT[tn];
       hrow = dpp_shl1(hrow, hrow);
-      const float szo = zq[u], sxo = xq[u];
-      if constexpr (decltype(fetch)::value) {
-        zq[u] = load_z(t + PF);
-        xq[u] = load_x(t + PF);
-      }
-      tile_step<EQH, FD, FAST>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+      const float szo = zq[u];
+      if constexpr (decltype(fetch)::value) zq[u] = load_z(t + PF);
+      tile_step<EQH, FD, FAST, true>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, xr, te, tev, sref, k);
+      xr = dpp_shr1(xr, xr);
       ta = tn;
     }
   };
