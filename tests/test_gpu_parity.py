@@ -149,6 +149,33 @@ def test_batch_rerun_after_new_slowness(oracle, rng):
     e.close()
 
 
+def test_batch_many_tiles_per_workgroup_bit_exact(oracle, rng):
+    """More tiles than resident workgroups (every workgroup runs several, with
+    the next tile's traveltimes staged while the current one runs): windows
+    of a shallow model, every shot against the oracle."""
+    nz, nx, dh = 130, 1500, 5.0
+    s = smooth_slowness(rng, nz, nx, dh)
+    nshots = 160  # ~9k tiles, about 3.5 per resident workgroup
+    shots = []
+    for k in range(nshots):
+        wd = int(rng.integers(700, nx + 1))
+        x0 = int(rng.integers(0, nx - wd + 1))
+        sx = float((x0 + rng.uniform(0.0, wd - 1)) * dh)
+        shots.append((sx, float(rng.uniform(0.0, (nz - 1) * dh)), x0, wd, 0, 0))
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    b = e.batch(shots, schedule=SCHED_TILED)
+    for run in range(2):
+        b.run()
+        b.status()
+        Trefs = oracle.solve_windows(s, dh, dh, [x[0] for x in shots], [x[1] for x in shots],
+                                     [x[2] for x in shots], [x[3] for x in shots], nthreads=16)
+        for k, Tr in enumerate(Trefs):
+            assert np.array_equal(b.T(k), Tr), (run, k)
+    b.close()
+    e.close()
+
+
 # dz = dx = 0x1.94f5a6p+2: the two-instruction division by dz rounds one
 # dividend per binade away from IEEE division, so this geometry must run the
 # five-instruction kernels
