@@ -27,6 +27,10 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # atomics stay single-lane: the optimizer's wave-wide rewrite of the
+    # persistent sweeps' ticket claim would consume the result (and wait for
+    # every memory operation in flight) right where it is issued
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
     "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]

@@ -221,32 +221,60 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t,
 // so at step t every lane's cell shares one skewed column and the addresses
 // advance by a uniform stride; otherwise each lane computes its clamped
 // address.
-template <int SB, bool EQH, bool FD, bool FAST>
-__device__ __forceinline__ void tile_steps_skew(const ShotDev& sd, const SkewView& sv, float* __restrict__ sT,
-                                                float hrow, int lane, int kap, bool row_ok,
-                                                int c0, int nxx, int NC, float& te, float& tev, CellQ& sref,
-                                                const EikConsts& k) {
-  const gfloat* __restrict__ SK = gptr(sv.base);
-  const int KP = sv.KP;
-  // fast path: z-other of step t at column sig0 + SB*t; lane 63's x-other cell
-  // (row kap+1 = next strip) sits one skew period (64 columns) away.
-  const int sig0 = (SB > 0) ? sd.x0 + c0 - 2 : sd.x0 + sd.w + 63 - c0;
-  const int colbase0 = (sig0 + SK_OFF) * KP;
-  // fast path: raw buffer loads, per-lane byte offset + per-step uniform
-  // byte offset (one scalar add per step instead of 64-bit address math)
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)sv.base, 0, 0x7fffffff, 0x00020000);
-  const int voz = kap * 4;
-  auto load_z = [&](int t) -> float {
-    if (FAST) return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voz, (colbase0 + SB * t * KP) * 4, 0));
-    return SK[skew_addr<SB>(sd, nxx, NC, KP, kap, c0 + t - lane + 1)];
-  };
-  // lane 63's x-other cells (row kap63+1), lane j holding step 63-j's
-  const int kx = kap - lane + 64;
-  float xr = FAST ? SK[colbase0 + kx - SB * KP * (lane + 1)] : SK[skew_addr<SB>(sd, nxx, NC, KP, kx, c0 - lane)];
-  if (EQH) xr = k.dx * xr;
+// Per-step slowness addressing of a tile in the skewed copy.  FAST: the
+// z-other cell of step t sits at column sig0 + SB*t, a uniform base plus the
+// lane's row (raw buffer loads: per-lane byte offset + per-step uniform
+// offset); otherwise each lane computes its clamped address.
+template <int SB, bool FAST>
+struct SkewStream {
+  const ShotDev& sd;
+  const SkewView& sv;
+  int kap, lane, c0, nxx, NC, colbase0;
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ SkewStream(const ShotDev& sd_, const SkewView& sv_, int kap_, int lane_, int c0_,
+                                        int nxx_, int NC_)
+      : sd(sd_), sv(sv_), kap(kap_), lane(lane_), c0(c0_), nxx(nxx_), NC(NC_) {
+    const int sig0 = (SB > 0) ? sd.x0 + c0 - 2 : sd.x0 + sd.w + 63 - c0;
+    colbase0 = (sig0 + SK_OFF) * sv.KP;
+    rs = __builtin_amdgcn_make_buffer_rsrc((void*)sv.base, 0, 0x7fffffff, 0x00020000);
+  }
+  __device__ __forceinline__ float z(int t) const {
+    if (FAST)
+      return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, kap * 4, (colbase0 + SB * t * sv.KP) * 4, 0));
+    return gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 + t - lane + 1)];
+  }
+  // lane 63's x-other cells (row kap63+1 = next strip, one skew period away),
+  // lane j holding step 63-j's
+  __device__ __forceinline__ float x63() const {
+    const int kx = kap - lane + 64;
+    if (FAST) return gptr(sv.base)[colbase0 + kx - SB * sv.KP * (lane + 1)];
+    return gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kx, c0 - lane)];
+  }
+};
+
+// A tile's first slowness loads, issued before its dependency wait: the
+// z-other cells of steps 0..PF-1, lane 63's x-other cells and the reference
+// cell of step 0.
+struct SkewPre {
   float zq[PF];
+  float xr, s0;
+};
+template <int SB, bool FAST>
+__device__ __forceinline__ void skew_prefetch(SkewPre& p, const SkewStream<SB, FAST>& st) {
 #pragma unroll
-  for (int u = 0; u < PF; ++u) zq[u] = load_z(u);
+  for (int u = 0; u < PF; ++u) p.zq[u] = st.z(u);
+  p.xr = st.x63();
+  p.s0 = gptr(st.sv.base)[skew_addr<SB>(st.sd, st.nxx, st.NC, st.sv.KP, st.kap, st.c0 - st.lane)];
+}
+
+template <int SB, bool EQH, bool FD, bool FAST>
+__device__ __forceinline__ void tile_steps_skew(const SkewStream<SB, FAST>& st, SkewPre& pre, float* __restrict__ sT,
+                                                float hrow, int lane, bool row_ok, int c0, int NC, float& te,
+                                                float& tev, CellQ& sref, const EikConsts& k) {
+  float xr = pre.xr;
+  if (EQH) xr = k.dx * xr;
+  float* zq = pre.zq;
+  auto load_z = [&](int t) -> float { return st.z(t); };
   int ta = trow(0, lane);
   float told_n = sT[ta];
   // PF steps from t0; the last block is a separate copy without loads, so
@@ -354,11 +382,12 @@ struct PersistCtl {
 // upwind row and the carry-in from block bb-1 arrive through the tagged
 // exchange buffers of the shot (and the tile publishes its own), instead of
 // being read from T after a launch boundary.
-template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST, class BeforeWriteBack>
+template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST, class AfterStage,
+          class BeforeWriteBack>
 __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __restrict__ shots, int nzz,
                                              const EikConsts& k, const SkewView& sv, const PersistCtl& ctl,
                                              unsigned tagb, float* __restrict__ sT, float* __restrict__ sS,
-                                             const BeforeWriteBack& before_wb) {
+                                             const AfterStage& after_stage, const BeforeWriteBack& before_wb) {
   const ShotDev sd = shots[tl.x];
   const int r = tl.y;
   const int bb = tl.z;
@@ -426,6 +455,22 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     }
     // (waited for below, after the halo and carry loads: their latencies overlap)
   }
+  after_stage();
+
+  // skewed slowness: the tile's first loads go out before the dependency wait
+  const int kap = 64 * r + lane;
+  // unreplicated cell columns: lam in [3, w+2] (SB > 0) or [2, w+1] (SB < 0)
+  const int lam_lo = max(0, c0 - 63), lam_hi = min(NC, c0 + 64);
+  const bool fast = SKEW && ((SB > 0) ? (lam_lo >= 3 && lam_hi <= sd.w + 2) : (lam_lo >= 2 && lam_hi <= sd.w + 1));
+  const SkewStream<SB, true> st_fast(sd, sv, kap, lane, c0, nxx, NC);
+  const SkewStream<SB, false> st_edge(sd, sv, kap, lane, c0, nxx, NC);
+  SkewPre pre;
+  if (SKEW) {
+    if (fast)
+      skew_prefetch(pre, st_fast);
+    else
+      skew_prefetch(pre, st_edge);
+  }
 
   // ---- upwind row above the strip (local row 64r-1; lane t holds column
   // c0+t, the value lane 0 needs at step t), and the carry-in from block
@@ -459,16 +504,12 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   if (!FIRST) __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0), lgkmcnt(0): the column loads have landed in LDS
 
   if (SKEW) {
-    const int kap = 64 * r + lane;
-    CellQ sref = cell_q<EQH, FD>(gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 - lane)], k);
+    CellQ sref = cell_q<EQH, FD>(pre.s0, k);
     wave_lds_sync();
-    // unreplicated cell columns: lam in [3, w+2] (SB > 0) or [2, w+1] (SB < 0)
-    const int lam_lo = max(0, c0 - 63), lam_hi = min(NC, c0 + 64);
-    const bool fast = (SB > 0) ? (lam_lo >= 3 && lam_hi <= sd.w + 2) : (lam_lo >= 2 && lam_hi <= sd.w + 1);
     if (fast)
-      tile_steps_skew<SB, EQH, FD, true>(sd, sv, sT, hrow, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, FD, true>(st_fast, pre, sT, hrow, lane, row_ok, c0, NC, te, tev, sref, k);
     else
-      tile_steps_skew<SB, EQH, FD, false>(sd, sv, sT, hrow, lane, kap, row_ok, c0, nxx, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, FD, false>(st_edge, pre, sT, hrow, lane, row_ok, c0, NC, te, tev, sref, k);
   } else {
     // Slowness cells step-major: sS[t][kap] = cell (64r+kap, c0+t-kap+1),
     // kap = 0..64; lane k reads its z-other at [t][k] and x-other at [t][k+1].
@@ -550,7 +591,7 @@ __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ 
   __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
   const PersistCtl ctl{};
   tile_process<SA, SB, EQH, false, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS,
-                                                       [] {});
+                                                       [] {}, [] {});
 }
 
 // Persistent sweep: every workgroup claims tiles from a ticket counter in
@@ -564,12 +605,12 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
   __shared__ float sT[64 * RING_LD];
   __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
   const unsigned tagb = (*ctl.epoch << 11) | (ctl.sweep_id << 8);
-  auto claim = [&]() -> int {
+  auto claim_issue = [&]() -> int {
     int t = 0;
     if (threadIdx.x == 0) t = atomicAdd(ctl.counter, 1);
-    return __builtin_amdgcn_readfirstlane(t);
+    return t;
   };
-  int tk = claim();
+  int tk = __builtin_amdgcn_readfirstlane(claim_issue());
   while (tk < ctl.ntiles) {
     const int4 tl = tiles[tk];
     // wave priority from the tile's scheduling slack (0 = most slack)
@@ -579,11 +620,15 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
       case 1: __builtin_amdgcn_s_setprio(1); break;
       default: __builtin_amdgcn_s_setprio(0); break;
     }
-    // the next ticket is claimed before this tile's write-back stores are
-    // issued, so waiting for it does not wait for them
-    int next = 0;
-    tile_process<SA, SB, EQH, FD, SKEW, FIRST, true>(tl, shots, nzz, k, sv, ctl, tagb, sT, sS,
-                                                     [&] { next = claim(); });
+    // the next ticket is claimed as soon as this tile's staging is issued
+    // (its round trip overlaps the dependency wait) and read before the
+    // write-back.  Holding one ticket ahead keeps the sweep draining: each
+    // workgroup's tickets increase, so the smallest unfinished ticket is
+    // always one a workgroup is running, and its predecessors are done.
+    int raw = 0, next = 0;
+    tile_process<SA, SB, EQH, FD, SKEW, FIRST, true>(
+        tl, shots, nzz, k, sv, ctl, tagb, sT, sS, [&] { raw = claim_issue(); },
+        [&] { next = __builtin_amdgcn_readfirstlane(raw); });
     tk = next;
   }
 }
