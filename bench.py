@@ -38,14 +38,19 @@ def log(*a):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_persist_c3_256shots_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per sweep launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes of this workload (calibrated against a device copy)."""
+def profile_figure(key):
+    """A figure from the committed rocprofv3 summary of this workload: HBM
+    bytes per sweep launch (FETCH_SIZE / WRITE_SIZE passes calibrated against
+    a device copy) or the SQ pass's VALU-busy fraction."""
     try:
         kv = dict(l.split(":", 1) for l in open(TRAFFIC_FILE) if ":" in l)
-        return float(kv["traffic_bytes_per_launch"])
+        return float(kv[key])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def measured_traffic():
+    return profile_figure("traffic_bytes_per_launch")
 
 
 def cpu_baseline(w, batch, budget_s):
@@ -278,6 +283,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
+                     # the binding resource: SQ_ACTIVE_INST_VALU over the sweep kernels
+                     "valu_busy": profile_figure("valu_busy"),
                      "kernel": "k_sweep_persist", "launches": nlaunch,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": avg_launch_s * 1e6},
