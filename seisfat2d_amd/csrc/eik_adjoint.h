@@ -59,8 +59,12 @@ void launch_adj_seed(const size_t* node, const int* ptr, const int* rec, const f
 // fp64 trajectory: T0 stencil (T0 otherwise 1e6), and sweep s: Tnew = sweep(Told).
 void launch_fill_d(double* p, size_t n, double v, hipStream_t st);
 void launch_fwd64_init(const AdjShot* shots, int nshots, int nzz, double dx, double dz, double* T0, hipStream_t st);
+// walk_ctl: 2 ints (ticket counter, reset by the launch; wait-limit flag);
+// walk_x: nshots * nstr * ldx exchange words (ldx >= 2 * nxx); tag: a value
+// no earlier launch on the same walk_x used
 void launch_fwd64_sweep(int sweep, const AdjShot* shots, int nshots, int nzz, int nstr, double dx, double dz,
-                        const double* Told, double* Tnew, hipStream_t st);
+                        const double* Told, double* Tnew, int* walk_ctl, unsigned long long* walk_x, int ldx,
+                        unsigned tag, hipStream_t st);
 // Picks of the fp64 trajectory's final traveltimes (rx, rz relative to each
 // receiver's shot grid).
 void launch_picks64(const AdjShot* shots, const int* rec_shot, const float* rx, const float* rz, int nrec, int nzz,

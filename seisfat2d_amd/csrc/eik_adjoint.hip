@@ -90,32 +90,72 @@ __global__ void k_fwd64_init(const AdjShot* __restrict__ shots, int nshots, int 
 }
 
 // One fp64 sweep, Tnew = sweep(Told), out of place (ring nodes of Tnew hold
-// 1e6 for good).  One wave per shot walks the strips of 64 sweep-local rows;
+// 1e6 for good).  A wave walks one strip of 64 sweep-local rows of one shot;
 // lane k holds row 64r + k and visits column t - k at step t: its z-upwind
 // and diagonal neighbours are lane k-1's results one and two steps back (DPP
 // shift), its x-upwind neighbour its own previous result, and lane 0 reads
-// the previous strip's row from Tnew.  Inputs are loaded ADJ_PF steps ahead.
+// the previous strip's last row from Tnew.  Inputs are loaded ADJ_PF steps
+// ahead.
+//
+// The strips of a shot run concurrently, each trailing the one above: lane
+// 0's upwind inputs are the upper strip's lane-63 results (its last row), which
+// that strip publishes as it computes them, column by column, each double as
+// two 64-bit words (one 32-bit half + the launch's tag in the high half),
+// written and read as single agent-scope atomics: a reader that sees the tag
+// in both words has the value, with no fence on either side.  Lane 0 fetches
+// the words of step s's column ADJ_PF steps ahead and polls again at step s
+// while a tag is stale (bounded: past the limit it sets the flag and stops
+// waiting for the rest of the strip, so a broken dependency cannot stall the
+// device).  Its diagonal input is
+// its previous step's upwind value (the ring column 0 holds 1e6).  Strips are
+// claimed from a ticket counter strip-major (every shot's strip r before any
+// strip r+1), so a strip's upper neighbour always holds a smaller ticket and
+// is running: the walk drains for any number of resident waves.
+constexpr int WALK_SPIN = 1 << 16;
+
+__global__ void k_walk_begin(int* __restrict__ ctl) {
+  if (threadIdx.x == 0) ctl[0] = 0;
+}
+
+__device__ __forceinline__ unsigned long long walk_word(unsigned half, unsigned tag) {
+  return ((unsigned long long)tag << 32) | half;
+}
+
 template <int SA, int SB>
-__global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ shots, int nzz, int nstr,
-                                                    ConstsD K, const double* __restrict__ Told,
-                                                    double* __restrict__ Tnew) {
-  const AdjShot sh = shots[blockIdx.x];
-  const int lane = threadIdx.x, nxx = sh.sd.nxx;
-  const double* __restrict__ To = Told + sh.toff;
-  double* __restrict__ Tn = Tnew + sh.toff;
-  const int NT = (nxx + 63 + ADJ_PF - 1) / ADJ_PF * ADJ_PF;
-  for (int r = 0; r < nstr; ++r) {
+__global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ shots, int nshots, int nzz,
+                                                    int nstr, ConstsD K, const double* __restrict__ Told,
+                                                    double* __restrict__ Tnew, int* __restrict__ ctl,
+                                                    unsigned long long* __restrict__ xw, int ldx, unsigned tag) {
+  typedef __attribute__((address_space(1))) unsigned long long gword;
+  const int lane = threadIdx.x;
+  for (;;) {
+    int tk = 0;
+    if (lane == 0) tk = atomicAdd(ctl, 1);
+    tk = __builtin_amdgcn_readfirstlane(tk);
+    if (tk >= nshots * nstr) break;
+    const int r = tk / nshots, ks = tk - r * nshots;
+    const AdjShot sh = shots[ks];
+    const int nxx = sh.sd.nxx;
+    const double* __restrict__ To = Told + sh.toff;
+    double* __restrict__ Tn = Tnew + sh.toff;
+    const int NT = (nxx + 63 + ADJ_PF - 1) / ADJ_PF * ADJ_PF;
+    // this strip's published last row, and the upper strip's
+    gword* xme = (gword*)(xw + ((size_t)ks * nstr + r) * ldx);
+    const gword* xup = (const gword*)(xw + ((size_t)ks * nstr + max(r - 1, 0)) * ldx);
+    const bool pub = lane == 63 && r + 1 < nstr;
+    const bool gl = lane == 0 && r > 0;
     const int fi = 64 * r + lane;
     const bool row_int = fi >= 1 && fi <= nzz - 2;
     const int fic = min(max(fi, 1), nzz - 2);
     const int i = SA > 0 ? fic : nzz - 1 - fic;
     const int i1 = i - (SA + 1) / 2;
-    const bool gl = lane == 0 && r > 0;
-    double qto[ADJ_PF], qgv[ADJ_PF], qge[ADJ_PF];
+    double qto[ADJ_PF];
+    unsigned long long qhi[ADJ_PF], qlo[ADJ_PF];
     float qsr[ADJ_PF], qsz[ADJ_PF], qsx[ADJ_PF];
     // inputs of step t at clamped (always valid) coordinates; only interior
     // nodes use them
-    auto load = [&](int t, double& to, double& gv, double& ge, float& sr, float& sz, float& sx) {
+    auto load = [&](int t, double& to, unsigned long long& hi, unsigned long long& lo, float& sr, float& sz,
+                    float& sx) {
       const int fjc = min(max(t - lane, 1), nxx - 2);
       const int j = SB > 0 ? fjc : nxx - 1 - fjc;
       const int j1 = j - (SB + 1) / 2;
@@ -123,23 +163,41 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
       sr = s_at(sh.sd, nzz, i1, j1);
       sz = s_at(sh.sd, nzz, i1, j1 + SB);
       sx = s_at(sh.sd, nzz, i1 + SA, j1);
-      gv = 0.0;
-      ge = 0.0;
       if (gl) {
-        gv = Tn[(i - SA) + (size_t)j * nzz];
-        ge = Tn[(i - SA) + (size_t)(j - SB) * nzz];
+        hi = __hip_atomic_load(xup + 2 * fjc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        lo = __hip_atomic_load(xup + 2 * fjc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     };
 #pragma unroll
-    for (int u = 0; u < ADJ_PF; ++u) load(u, qto[u], qgv[u], qge[u], qsr[u], qsz[u], qsx[u]);
-    double prev = 1e6, prev2 = 1e6;
+    for (int u = 0; u < ADJ_PF; ++u) load(u, qto[u], qhi[u], qlo[u], qsr[u], qsz[u], qsx[u]);
+    double prev = 1e6, prev2 = 1e6, gv_prev = 1e6;
+    bool gave_up = false;
     for (int t0 = 0; t0 < NT; t0 += ADJ_PF) {
 #pragma unroll
       for (int u = 0; u < ADJ_PF; ++u) {
         const int t = t0 + u;
-        const double to = qto[u], gv = qgv[u], ge = qge[u];
+        const double to = qto[u];
         const float sr = qsr[u], sz = qsz[u], sx = qsx[u];
-        load(min(t + ADJ_PF, NT - 1), qto[u], qgv[u], qge[u], qsr[u], qsz[u], qsx[u]);
+        unsigned long long hi = qhi[u], lo = qlo[u];
+        load(min(t + ADJ_PF, NT - 1), qto[u], qhi[u], qlo[u], qsr[u], qsz[u], qsx[u]);
+        double gv = 0.0, ge = 0.0;
+        if (gl) {
+          // upper strip's value of column clamp(t, 1, nxx-2), published at its step t+63
+          const int fjc = min(max(t, 1), nxx - 2);
+          for (int it = 0; !gave_up && ((unsigned)(hi >> 32) != tag || (unsigned)(lo >> 32) != tag); ++it) {
+            if (it > WALK_SPIN) {
+              atomicOr(ctl + 1, 1);
+              gave_up = true;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            hi = __hip_atomic_load(xup + 2 * fjc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            lo = __hip_atomic_load(xup + 2 * fjc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          gv = __hiloint2double((int)(unsigned)hi, (int)(unsigned)lo);
+          ge = t >= 2 ? gv_prev : 1e6;  // column fjc-1: the previous step's, or the ring
+          gv_prev = gv;
+        }
         const double tv = dpp_shr1_d(gv, prev);
         const double tev = dpp_shr1_d(ge, prev2);
         const int fj = t - lane;
@@ -149,12 +207,16 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
           const int j = SB > 0 ? fj : nxx - 1 - fj;
           Tn[i + (size_t)j * nzz] = out;
         }
+        if (pub && fj >= 1 && fj <= nxx - 2) {
+          __hip_atomic_store(xme + 2 * fj, walk_word((unsigned)__double2hiint(out), tag), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(xme + 2 * fj + 1, walk_word((unsigned)__double2loint(out), tag), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        }
         prev2 = prev;
         prev = out;
       }
     }
-    // the next strip's lane 0 reads this strip's last row
-    __threadfence_block();
   }
 }
 
@@ -469,13 +531,18 @@ void launch_fwd64_init(const AdjShot* shots, int nshots, int nzz, double dx, dou
 }
 
 void launch_fwd64_sweep(int sweep, const AdjShot* shots, int nshots, int nzz, int nstr, double dx, double dz,
-                        const double* Told, double* Tnew, hipStream_t st) {
+                        const double* Told, double* Tnew, int* walk_ctl, unsigned long long* walk_x, int ldx,
+                        unsigned tag, hipStream_t st) {
   const ConstsD k = consts_d(dx, dz);
+  hipLaunchKernelGGL(k_walk_begin, dim3(1), dim3(64), 0, st, walk_ctl);
+  // one wave per strip, at most what is resident at once (the tickets keep
+  // any count correct)
+  const int grid = std::min(nshots * nstr, 4096);
   switch (sweep) {
-    case 0: hipLaunchKernelGGL((k_fwd64_sweep<1, 1>), dim3(nshots), dim3(64), 0, st, shots, nzz, nstr, k, Told, Tnew); break;
-    case 1: hipLaunchKernelGGL((k_fwd64_sweep<-1, 1>), dim3(nshots), dim3(64), 0, st, shots, nzz, nstr, k, Told, Tnew); break;
-    case 2: hipLaunchKernelGGL((k_fwd64_sweep<1, -1>), dim3(nshots), dim3(64), 0, st, shots, nzz, nstr, k, Told, Tnew); break;
-    default: hipLaunchKernelGGL((k_fwd64_sweep<-1, -1>), dim3(nshots), dim3(64), 0, st, shots, nzz, nstr, k, Told, Tnew); break;
+    case 0: hipLaunchKernelGGL((k_fwd64_sweep<1, 1>), dim3(grid), dim3(64), 0, st, shots, nshots, nzz, nstr, k, Told, Tnew, walk_ctl, walk_x, ldx, tag); break;
+    case 1: hipLaunchKernelGGL((k_fwd64_sweep<-1, 1>), dim3(grid), dim3(64), 0, st, shots, nshots, nzz, nstr, k, Told, Tnew, walk_ctl, walk_x, ldx, tag); break;
+    case 2: hipLaunchKernelGGL((k_fwd64_sweep<1, -1>), dim3(grid), dim3(64), 0, st, shots, nshots, nzz, nstr, k, Told, Tnew, walk_ctl, walk_x, ldx, tag); break;
+    default: hipLaunchKernelGGL((k_fwd64_sweep<-1, -1>), dim3(grid), dim3(64), 0, st, shots, nshots, nzz, nstr, k, Told, Tnew, walk_ctl, walk_x, ldx, tag); break;
   }
 }
 

@@ -105,6 +105,12 @@ struct AdjState {
   float* d_seed_w = nullptr;
   int nseed = 0;
   double* d_picks64 = nullptr;  // picks of the fp64 trajectory
+  // strip-parallel fp64 walk: ticket counter + wait-limit flag, each strip's
+  // published last row (tagged words, walk_ldx per strip), launch tag
+  int* d_walk_ctl = nullptr;
+  unsigned long long* d_walk_x = nullptr;
+  int walk_ldx = 0;
+  unsigned walk_tag = 0;
   float* d_w = nullptr;      // cotangents of the host-side misfit call
   double* d_grad = nullptr;  // gradient of the host-side misfit call
 };
@@ -219,6 +225,8 @@ void free_batch(seisfat_batch_s* b) {
   dfree(a.d_seed_rec);
   dfree(a.d_seed_w);
   dfree(a.d_picks64);
+  dfree(a.d_walk_ctl);
+  dfree(a.d_walk_x);
   dfree(a.d_w);
   dfree(a.d_grad);
   delete b;
@@ -533,6 +541,11 @@ int batch_status(seisfat_batch_s* b) {
     HIPCHK(hipMemcpy(&err, b->d_ctl + 9, sizeof(int), hipMemcpyDeviceToHost));
     if (err) return fail(SEISFAT_EHIP, "persistent sweep: a tile dependency did not arrive within the spin limit");
   }
+  if (b->adj.d_walk_ctl) {
+    int err = 0;
+    HIPCHK(hipMemcpy(&err, b->adj.d_walk_ctl + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(SEISFAT_EHIP, "fp64 trajectory: a strip's upper neighbour did not arrive within the spin limit");
+  }
   return SEISFAT_OK;
 }
 
@@ -643,8 +656,15 @@ int adj_prepare(seisfat_batch_s* b) {
       (rc = dalloc(&A.d_xmsg, A.xsize)) || (rc = dalloc(&A.d_seed_node, snode.size())) ||
       (rc = dalloc(&A.d_seed_ptr, sptr.size())) || (rc = dalloc(&A.d_seed_rec, srec.size())) ||
       (rc = dalloc(&A.d_seed_w, sw.size())) || (rc = dalloc(&A.d_w, (size_t)b->nrec)) || (rc = dalloc(&A.d_picks64, (size_t)b->nrec)) ||
-      (rc = dalloc(&A.d_grad, (size_t)h->nz * h->nx)))
+      (rc = dalloc(&A.d_grad, (size_t)h->nz * h->nx)) || (rc = dalloc(&A.d_walk_ctl, 2)))
     return rc;
+  int max_nxx = 0;
+  for (const ShotDev& sd : b->iso) max_nxx = std::max(max_nxx, sd.nxx);
+  A.walk_ldx = 2 * max_nxx;
+  if ((rc = dalloc(&A.d_walk_x, (size_t)nshots * A.nstr * A.walk_ldx))) return rc;
+  // exchange words start with tag 0, below every launch's tag (>= 1)
+  HIPCHK(hipMemsetAsync(A.d_walk_x, 0, (size_t)nshots * A.nstr * A.walk_ldx * sizeof(unsigned long long), h->stream));
+  HIPCHK(hipMemsetAsync(A.d_walk_ctl, 0, 2 * sizeof(int), h->stream));
   hipStream_t st = h->stream;
   HIPCHK(hipMemcpyAsync(A.d_tab, A.tab.data(), nshots * sizeof(AdjShot), hipMemcpyHostToDevice, st));
   if (!snode.empty()) {
@@ -682,7 +702,8 @@ int run_record(seisfat_batch_s* b, hipStream_t st) {
   launch_fwd64_init(A.d_tab, b->nshots, h->nzz, h->dx, h->dz, A.d_T64, st);
   for (int s = 0; s < 4; ++s)
     launch_fwd64_sweep(s, A.d_tab, b->nshots, h->nzz, A.nstr, h->dx, h->dz, A.d_T64 + s * ts,
-                       A.d_T64 + (s + 1) * ts, st);
+                       A.d_T64 + (s + 1) * ts, A.d_walk_ctl, A.d_walk_x, A.walk_ldx,
+                       A.walk_tag = A.walk_tag == 0xffffffffu ? 1 : A.walk_tag + 1, st);
   launch_picks64(A.d_tab, b->d_recshot, b->d_rx, b->d_rz, b->nrec, h->nzz, h->dx, h->dz, A.d_T64 + 4 * ts,
                  A.d_picks64, st);
   HIPCHK(hipGetLastError());
