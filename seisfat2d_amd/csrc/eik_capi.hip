@@ -73,6 +73,7 @@ struct seisfat_eik_s {
   unsigned version = 0;  // incremented by every accepted slowness upload
   uint16_t* d_c[6] = {};
   float cmin[6] = {}, cmax[6] = {};
+  float* d_anitab = nullptr;  // 7 planes of nzz*nxx: the qP slowness's shot-independent factors
   bool has_c = false;
   int* d_flag = nullptr;
   // skewed slowness copies, one per sweep orientation (tiled schedule)
@@ -278,8 +279,14 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
       sf.mn[c] = h->cmin[c];
       sf.mx[c] = h->cmax[c];
     }
-    launch_quasi_slowness(b->d_iso, b->d_qp, b->nshots, nzz, h->nxx, h->dx, h->dz, sf, st);
-    ++launches;
+    AniTab tb;
+    const size_t plane = (size_t)nzz * h->nxx;
+    for (int c = 0; c < 7; ++c) tb.v[c] = h->d_anitab + c * plane;
+    // the model as a full-width shot sees it (all anisotropic shots are)
+    ShotDev model = b->iso[0];
+    launch_ani_table(model, nzz, h->nxx, sf, tb, st);
+    launch_quasi_slowness(b->d_iso, b->d_qp, b->nshots, nzz, h->nxx, h->dx, h->dz, tb, st);
+    launches += 2;
     rc = one_solve(b->d_qp, b->qp, nullptr);  // per-shot qP slowness: staged in LDS
     if (rc) return rc;
   }
@@ -335,6 +342,8 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   if (!h->has_s) return fail(SEISFAT_ESTATE, "slowness not set (seisfat_eik_set_slowness)");
   if (type == SEISFAT_ANI && !h->has_c) return fail(SEISFAT_ESTATE, "stiffness not set (seisfat_eik_set_stiffness)");
   if (nshots <= 0 || !shots) return fail(SEISFAT_EINVAL, "nshots must be positive");
+  if (type == SEISFAT_ANI && (size_t)h->nzz * h->nxx >= ((size_t)1 << 31))
+    return fail(SEISFAT_EINVAL, "anisotropic batches take models of fewer than 2^31 padded nodes");
   if (nrec_total < 0 || (nrec_total > 0 && (!rx || !rz))) return fail(SEISFAT_EINVAL, "bad receiver arrays");
   const float dx = h->dx, dz = h->dz;
   std::vector<int> recshot(nrec_total, -1);
@@ -825,6 +834,7 @@ int seisfat_eik_destroy(seisfat_eik_t h) {
   dfree(h->d_flag);
   for (auto& p : h->d_sk) dfree(p);
   for (auto& c : h->d_c) dfree(c);
+  dfree(h->d_anitab);
   (void)hipStreamDestroy(h->stream);
   delete h;
   return SEISFAT_OK;
@@ -912,6 +922,8 @@ int seisfat_eik_set_stiffness(seisfat_eik_t h, const float* const c[6]) {
     HIPCHK(hipMemcpyAsync(h->d_c[e], q.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
   }
+  if (!h->d_anitab)
+    if (int rc = dalloc(&h->d_anitab, 7 * n)) return rc;
   h->has_c = true;
   return SEISFAT_OK;
 }

@@ -16,6 +16,11 @@ struct StiffDev {
   const uint16_t* c[6];  // C11, C13, C15, C33, C35, C55
   float mn[6], mx[6];
 };
+// Shot-independent factors of the qP slowness per padded node:
+// C11, C13, C15, C33, C35, C55 each divided twice by Ro, then Ro = C33 s^2.
+struct AniTab {
+  float* v[7];
+};
 
 void launch_fill(float* p, size_t n, float v, hipStream_t st);
 void launch_recip(const float* b, float* r, int n, hipStream_t st);
@@ -46,7 +51,9 @@ void launch_begin_run(const PersistRun& pr, hipStream_t st);
 void launch_skew_s(const float* szp, float* sk, int nzz, int nx, int KP, int NS, int sweep, hipStream_t st);
 void launch_picks(const ShotDev* shots, const int* rec_shot, const float* rx, const float* rz, float* out,
                   int nrec, int nzz, float dx, float dz, hipStream_t st);
+void launch_ani_table(const ShotDev& model, int nzz, int nxx, const StiffDev& stf, const AniTab& tb,
+                      hipStream_t st);
 void launch_quasi_slowness(const ShotDev* iso, ShotDev* qp, int nshots, int nzz, int nxx, float dx, float dz,
-                           const StiffDev& stf, hipStream_t st);
+                           const AniTab& tb, hipStream_t st);
 
 }  // namespace seisfat

@@ -684,50 +684,83 @@ __device__ __forceinline__ float decode_c(uint16_t u, float mn, float mx) {
   return (mn + (static_cast<float>(u) - 1.0f) * (mx - mn) / (65535 - 1));
 }
 
-__global__ void k_quasi_slowness(const ShotDev* __restrict__ iso, ShotDev* __restrict__ qp, int nzz,
-                                 float dx, float dz, StiffDev st) {
-  const ShotDev si = iso[blockIdx.y];
-  const ShotDev so = qp[blockIdx.y];
-  const int nxx = si.nxx;
-  const size_t n = (size_t)nzz * nxx;
-  const int sIdx = (int)((si.sx + 0.5f * dx) / dx) + NB;
-  const int sIdz = (int)((si.sz + 0.5f * dz) / dz) + NB;
-  gfloat* __restrict__ Sout = gptr(const_cast<float*>(so.sbase));
-  const gfloat* __restrict__ T = gptr(si.T);
-  size_t index = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; index < n; index += stride) {
-    const int i = (int)(index % nzz);
-    const int j = (int)(index / nzz);
-    const float s_iso = s_at(si, nzz, i, j);
-    float s_new = s_iso;
-    if ((i >= NB) && (i < nzz - NB) && (j >= NB) && (j < nxx - NB) && !((i == sIdz) && (j == sIdx))) {
-      const float dTz = 0.5f * (T[(i + 1) + (size_t)j * nzz] - T[(i - 1) + (size_t)j * nzz]) / dz;
-      const float dTx = 0.5f * (T[i + (size_t)(j + 1) * nzz] - T[i + (size_t)(j - 1) * nzz]) / dx;
-      const float norm = sqrtf(dTx * dTx + dTz * dTz);
-      const float p0 = dTx / norm;
-      const float p1 = dTz / norm;
-      const float c11 = decode_c(st.c[0][index], st.mn[0], st.mx[0]);
-      const float c13 = decode_c(st.c[1][index], st.mn[1], st.mx[1]);
-      const float c15 = decode_c(st.c[2][index], st.mn[2], st.mx[2]);
-      const float c33 = decode_c(st.c[3][index], st.mn[3], st.mx[3]);
-      const float c35 = decode_c(st.c[4][index], st.mn[4], st.mx[4]);
-      const float c55 = decode_c(st.c[5][index], st.mn[5], st.mx[5]);
-      const float Ro = c33 * s_iso * s_iso;
-      const float C00 = c11 / Ro / Ro, C01 = c13 / Ro / Ro, C02 = c15 / Ro / Ro;
-      const float C11 = c33 / Ro / Ro, C12 = c35 / Ro / Ro, C22 = c55 / Ro / Ro;
-      const float Gxx = C00 * p0 * p0 + C22 * p1 * p1 + 2.0f * C02 * p0 * p1;
-      const float Gzz = C22 * p0 * p0 + C11 * p1 * p1 + 2.0f * C12 * p0 * p1;
-      const float Gxz = C02 * p0 * p0 + C12 * p1 * p1 + (C01 + C22) * p0 * p1;
-      const float coeff1 = Gxx + Gzz;
-      const float coeff2 = Gxx - Gzz;
-      const float det = sqrtf((coeff2 * coeff2) / 4.0f + Gxz * Gxz);
-      float g0 = (float)((double)coeff1 / 2.0 + (double)det);
-      const float g1 = (float)((double)coeff1 / 2.0 - (double)det);
-      if (g0 < g1) g0 = g1;
-      s_new = 1.0f / sqrtf(g0 * Ro);
+// The shot-independent part of get_quasi_slowness: per padded node, the
+// decoded stiffness scaled by Ro = C33 * s^2 (Cij / Ro / Ro) and Ro itself,
+// evaluated with the same operations as the per-node expressions, once per
+// run instead of once per shot.
+__global__ void k_ani_table(ShotDev model, int nzz, int nxx, StiffDev st, AniTab tb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= nzz || j >= nxx) return;
+  const size_t index = i + (size_t)j * nzz;
+  const float s_iso = s_at(model, nzz, i, j);
+  const float c11 = decode_c(st.c[0][index], st.mn[0], st.mx[0]);
+  const float c13 = decode_c(st.c[1][index], st.mn[1], st.mx[1]);
+  const float c15 = decode_c(st.c[2][index], st.mn[2], st.mx[2]);
+  const float c33 = decode_c(st.c[3][index], st.mn[3], st.mx[3]);
+  const float c35 = decode_c(st.c[4][index], st.mn[4], st.mx[4]);
+  const float c55 = decode_c(st.c[5][index], st.mn[5], st.mx[5]);
+  const float Ro = c33 * s_iso * s_iso;
+  tb.v[0][index] = c11 / Ro / Ro;
+  tb.v[1][index] = c13 / Ro / Ro;
+  tb.v[2][index] = c15 / Ro / Ro;
+  tb.v[3][index] = c33 / Ro / Ro;
+  tb.v[4][index] = c35 / Ro / Ro;
+  tb.v[5][index] = c55 / Ro / Ro;
+  tb.v[6][index] = Ro;
+}
+
+// Per shot and padded node (src/modeling/eikonal_ani.cu:47-111): a thread
+// takes one node for QP_SHOTS shots, so the shot-independent factors are read
+// once for all of them.
+constexpr int QP_SHOTS = 4;
+__global__ void k_quasi_slowness(const ShotDev* __restrict__ iso, ShotDev* __restrict__ qp, int nshots, int nzz,
+                                 float dx, float dz, AniTab tb) {
+  const int nxx = iso[0].nxx;  // every anisotropic shot covers the whole model
+  const unsigned n = (unsigned)nzz * (unsigned)nxx;  // < 2^31 (checked when the batch is made)
+  const int k0 = blockIdx.y * QP_SHOTS, kn = min(nshots - k0, QP_SHOTS);
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned index = blockIdx.x * blockDim.x + threadIdx.x; index < n; index += stride) {
+    const int i = (int)(index % (unsigned)nzz);
+    const int j = (int)(index / (unsigned)nzz);
+    const bool inner = (i >= NB) && (i < nzz - NB) && (j >= NB) && (j < nxx - NB);
+    float C00 = 0.0f, C01 = 0.0f, C02 = 0.0f, C11 = 0.0f, C12 = 0.0f, C22 = 0.0f, Ro = 0.0f;
+    if (inner) {
+      C00 = tb.v[0][index];
+      C01 = tb.v[1][index];
+      C02 = tb.v[2][index];
+      C11 = tb.v[3][index];
+      C12 = tb.v[4][index];
+      C22 = tb.v[5][index];
+      Ro = tb.v[6][index];
     }
-    Sout[index] = s_new;
+    for (int q = 0; q < kn; ++q) {
+      const ShotDev& si = iso[k0 + q];
+      const ShotDev& so = qp[k0 + q];
+      const int sIdx = (int)((si.sx + 0.5f * dx) / dx) + NB;
+      const int sIdz = (int)((si.sz + 0.5f * dz) / dz) + NB;
+      const gfloat* __restrict__ T = gptr(si.T);
+      const float s_iso = s_at(si, nzz, i, j);
+      float s_new = s_iso;
+      if (inner && !((i == sIdz) && (j == sIdx))) {
+        const float dTz = 0.5f * (T[(i + 1) + (size_t)j * nzz] - T[(i - 1) + (size_t)j * nzz]) / dz;
+        const float dTx = 0.5f * (T[i + (size_t)(j + 1) * nzz] - T[i + (size_t)(j - 1) * nzz]) / dx;
+        const float norm = sqrtf(dTx * dTx + dTz * dTz);
+        const float p0 = dTx / norm;
+        const float p1 = dTz / norm;
+        const float Gxx = C00 * p0 * p0 + C22 * p1 * p1 + 2.0f * C02 * p0 * p1;
+        const float Gzz = C22 * p0 * p0 + C11 * p1 * p1 + 2.0f * C12 * p0 * p1;
+        const float Gxz = C02 * p0 * p0 + C12 * p1 * p1 + (C01 + C22) * p0 * p1;
+        const float coeff1 = Gxx + Gzz;
+        const float coeff2 = Gxx - Gzz;
+        const float det = sqrtf((coeff2 * coeff2) / 4.0f + Gxz * Gxz);
+        float g0 = (float)((double)coeff1 / 2.0 + (double)det);
+        const float g1 = (float)((double)coeff1 / 2.0 - (double)det);
+        if (g0 < g1) g0 = g1;
+        s_new = 1.0f / sqrtf(g0 * Ro);
+      }
+      gptr(const_cast<float*>(so.sbase))[index] = s_new;
+    }
   }
 }
 
@@ -962,11 +995,17 @@ void launch_picks(const ShotDev* shots, const int* rec_shot, const float* rx, co
                      nzz, dx, dz);
 }
 
+void launch_ani_table(const ShotDev& model, int nzz, int nxx, const StiffDev& stf, const AniTab& tb,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_ani_table, dim3((nzz + 63) / 64, nxx), dim3(64), 0, st, model, nzz, nxx, stf, tb);
+}
+
 void launch_quasi_slowness(const ShotDev* iso, ShotDev* qp, int nshots, int nzz, int nxx, float dx, float dz,
-                           const StiffDev& stf, hipStream_t st) {
+                           const AniTab& tb, hipStream_t st) {
   const size_t n = (size_t)nzz * nxx;
   const int bx = (int)std::min<size_t>((n + 255) / 256, 2048);
-  hipLaunchKernelGGL(k_quasi_slowness, dim3(bx, nshots), dim3(256), 0, st, iso, qp, nzz, dx, dz, stf);
+  hipLaunchKernelGGL(k_quasi_slowness, dim3(bx, (nshots + QP_SHOTS - 1) / QP_SHOTS), dim3(256), 0, st, iso, qp,
+                     nshots, nzz, dx, dz, tb);
 }
 
 }  // namespace seisfat
