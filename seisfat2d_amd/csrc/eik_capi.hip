@@ -178,6 +178,7 @@ EikConsts make_consts(float dx, float dz) {
   k.hyp = std::sqrt(dx * dx + dz * dz);
   k.sum = k.dz2i + k.dx2i;
   k.prod = k.dz2i * k.dx2i;
+  k.sum4 = 4.0f * k.sum;  // exact (power-of-two scaling)
   k.fastdiv = 0;
   // zh = RN(1/b); 1 - b*zh is exact (one fma), so zl = RN((1 - b*zh) / b)
   // = RN(1/b - zh)

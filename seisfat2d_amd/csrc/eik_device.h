@@ -24,6 +24,7 @@ struct EikConsts {
   float hyp;         // sqrtf(dx*dx + dz*dz)
   float sum;         // dz2i + dx2i
   float prod;        // dz2i * dx2i
+  float sum4;        // 4 * sum (exact)
   // refined reciprocals of the four divisors (see rdiv below), produced on
   // the device by k_recip so they are exactly the values the hardware
   // division sequence would build
@@ -87,6 +88,15 @@ __device__ __forceinline__ float rsqrt_rn(float x) {
 __device__ __forceinline__ float vmin(float a, float b) {
   float r;
   asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// Minimum of three non-NaN values (one instruction).  With no NaN operand
+// the minimum is exact and independent of grouping, so it equals any nesting
+// of vmin over the same three values.
+__device__ __forceinline__ float vmin3(float a, float b, float c) {
+  float r;
+  asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
   return r;
 }
 
@@ -206,10 +216,12 @@ __device__ __forceinline__ CellQ cell_q(float s, const EikConsts& k) {
   CellQ c;
   c.dzs = s * k.dz;
   c.dxs = EQH ? c.dzs : s * k.dx;  // EQH: dx == dz
-  c.s4 = 4.0f * s * s * k.sum;
+  // RN(RN(4s * s) * sum) with 4s exact and RN(4 s^2) = 4 RN(s^2) (s = 0 or
+  // s^2 normal), so it equals RN(RN(s^2) * 4sum): one product fewer
+  c.s2 = s * s;
+  c.s4 = c.s2 * k.sum4;
   c.thrB = FD ? fdiv(c.dzs * k.dz, k.zh_hyp, k.zl_hyp) : rdiv(c.dzs * k.dz, k.hyp, k.r_hyp);
   c.thrC = EQH ? c.thrB : (FD ? fdiv(c.dxs * k.dx, k.zh_hyp, k.zl_hyp) : rdiv(c.dxs * k.dx, k.hyp, k.r_hyp));
-  c.s2 = s * s;
   return c;
 }
 
@@ -219,13 +231,16 @@ __device__ __forceinline__ CellQ cell_q(float s, const EikConsts& k) {
 // h > 0, RN(h * min(a, b)) = min(RN(h * a), RN(h * b)), so the 1-D candidates
 // can use the cells' precomputed products.  Bit-identical to eik_local: the
 // selected lanes evaluate the same operations in the same order; the others
-// compute values that the selects discard.  FD: divisions by fdiv.
+// compute values that the selects discard.  FD: divisions by fdiv.  Returns
+// min(told, eik_local): the minimum of told, both 1-D candidates, the 2-D
+// candidate and TINF (no operand is a NaN, so two three-way minima give the
+// same value as the nested pairwise ones).
 template <bool EQH, bool FD>
 __device__ __forceinline__ float eik_local_sel(float tv, float te, float tev, const CellQ& r, float dzs_zo,
-                                               float dxs_xo, const EikConsts& k) {
+                                               float dxs_xo, const EikConsts& k, float told) {
   const float t1d1 = tv + vmin(r.dzs, dzs_zo);
   const float t1d2 = te + vmin(r.dxs, dxs_xo);
-  const float t1D = vmin(t1d1, t1d2);
+  const float t1D = vmin3(t1d1, t1d2, TINF);
   const float de = te - tev;
   const float dv = tv - tev;
   // non-short-circuit conjunctions: plain mask arithmetic, no control flow
@@ -253,7 +268,7 @@ __device__ __forceinline__ float eik_local_sel(float tv, float te, float tev, co
   const float tA = FD ? fdiv(num, k.zh_sum, k.zl_sum) : rdiv(num, k.sum, k.r_sum);
   const float tBC = t0 + h2 * sq;
   const float t2D = cA ? tA : ((cB | cC) ? tBC : TINF);
-  return vmin(t1D, vmin(t2D, TINF));
+  return vmin3(t1D, t2D, told);
 }
 
 }  // namespace seisfat

@@ -145,6 +145,16 @@ constexpr int PF = 16;       // slowness prefetch distance (steps) on the skewed
 __device__ __forceinline__ int trow(int t, int lane) { return ((t - lane) & 63) * RING_LD + lane; }
 // Word index of the same lane's node one step later (rows wrap mod 64).
 __device__ __forceinline__ int trow_next(int ta) { return (ta + RING_LD) & (64 * RING_LD - 1); }
+// The step loops walk the ring by byte offset (no index scaling per step).
+__device__ __forceinline__ int trow_next_b(int tab) { return (tab + 4 * RING_LD) & (4 * 64 * RING_LD - 1); }
+__device__ __forceinline__ float& ring_at(float* sT, int tab) {
+  return *reinterpret_cast<float*>(reinterpret_cast<char*>(sT) + tab);
+}
+// Per-step broadcast values of a tile in LDS: word 2t = the upwind-row value
+// lane 0 needs at step t, word 2t+1 = lane 63's x-other cell at step t (times
+// dx when dx == dz); one 8-byte read per step serves all lanes.  Entry 64 is
+// padding for the look-ahead read after the last step.
+constexpr int HX_WORDS = 2 * 65;
 
 // wave_shl:1 (DPP control 0x130): lane k receives lane k+1's `src`; lane 63
 // keeps `old`.  Shifts the upwind row one step along: lane 0 holds step t's.
@@ -190,7 +200,7 @@ __device__ __forceinline__ int skew_addr(const ShotDev& sd, int nxx, int NC, int
 // ALLCOLS: every column the tile visits is an interior node column, so only
 // the row test remains.
 template <bool EQH, bool FD, bool ALLCOLS, bool XDPP = false>
-__device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t, int lane, bool row_ok, int c0,
+__device__ __forceinline__ void tile_step(float* __restrict__ sT, int tab, int t, int lane, bool row_ok, int c0,
                                           int NC,
                                           float told, float h, float szo, float sxo, float& te, float& tev,
                                           CellQ& sref, const EikConsts& k) {
@@ -201,10 +211,9 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int ta, int t,
   // XDPP: lane k's x-other cell is lane k+1's z-other cell of this step; sxo
   // is then only lane 63's (times dx when EQH, where zo.dzs = dz*s = dx*s)
   const float dxs_xo = !XDPP ? k.dx * sxo : (EQH ? dpp_shl1(sxo, zo.dzs) : k.dx * dpp_shl1(sxo, szo));
-  const float cand = eik_local_sel<EQH, FD>(tv, te, tev, sref, zo.dzs, dxs_xo, k);
-  const float m = vmin(told, cand);  // unconditional: a select, not a branch
+  const float m = eik_local_sel<EQH, FD>(tv, te, tev, sref, zo.dzs, dxs_xo, k, told);  // min(told, candidates)
   const float res = act ? m : told;
-  sT[ta] = res;
+  ring_at(sT, tab) = res;
   tev = tv;
   te = res;
   sref = zo;
@@ -269,29 +278,28 @@ __device__ __forceinline__ void skew_prefetch(SkewPre& p, const SkewStream<SB, F
 
 template <int SB, bool EQH, bool FD, bool FAST>
 __device__ __forceinline__ void tile_steps_skew(const SkewStream<SB, FAST>& st, SkewPre& pre, float* __restrict__ sT,
-                                                float hrow, int lane, bool row_ok, int c0, int NC, float& te,
-                                                float& tev, CellQ& sref, const EikConsts& k) {
-  float xr = pre.xr;
-  if (EQH) xr = k.dx * xr;
+                                                const float* __restrict__ sHX, int lane, bool row_ok, int c0,
+                                                int NC, float& te, float& tev, CellQ& sref, const EikConsts& k) {
   float* zq = pre.zq;
   auto load_z = [&](int t) -> float { return st.z(t); };
-  int ta = trow(0, lane);
-  float told_n = sT[ta];
+  int tab = 4 * trow(0, lane);
+  float told_n = ring_at(sT, tab);
+  float2 hx_n = reinterpret_cast<const float2*>(sHX)[0];
   // PF steps from t0; the last block is a separate copy without loads, so
   // nothing is fetched for steps past 63
   auto block = [&](int t0, auto fetch) {
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const int t = t0 + u;
-      const float told = told_n, h = hrow;
-      const int tn = trow_next(ta);
-      told_n = sT[tn];
-      hrow = dpp_shl1(hrow, hrow);
+      const float told = told_n;
+      const float2 hx = hx_n;
+      const int tn = trow_next_b(tab);
+      told_n = ring_at(sT, tn);
+      hx_n = reinterpret_cast<const float2*>(sHX)[t + 1];
       const float szo = zq[u];
       if constexpr (decltype(fetch)::value) zq[u] = load_z(t + PF);
-      tile_step<EQH, FD, FAST, true>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, xr, te, tev, sref, k);
-      xr = dpp_shr1(xr, xr);
-      ta = tn;
+      tile_step<EQH, FD, FAST, true>(sT, tab, t, lane, row_ok, c0, NC, told, hx.x, szo, hx.y, te, tev, sref, k);
+      tab = tn;
     }
   };
   for (int t0 = 0; t0 < 64 - PF; t0 += PF) block(t0, std::true_type{});
@@ -387,7 +395,8 @@ template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST
 __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __restrict__ shots, int nzz,
                                              const EikConsts& k, const SkewView& sv, const PersistCtl& ctl,
                                              unsigned tagb, float* __restrict__ sT, float* __restrict__ sS,
-                                             const AfterStage& after_stage, const BeforeWriteBack& before_wb) {
+                                             float* __restrict__ sHX, const AfterStage& after_stage,
+                                             const BeforeWriteBack& before_wb) {
   const ShotDev sd = shots[tl.x];
   const int r = tl.y;
   const int bb = tl.z;
@@ -503,13 +512,17 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
 
   if (!FIRST) __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0), lgkmcnt(0): the column loads have landed in LDS
 
+  // per-step broadcasts: lane t's upwind-row value is step t's; lane j's
+  // x-other cell of lane 63 is step 63-j's
+  sHX[2 * lane] = hrow;
   if (SKEW) {
     CellQ sref = cell_q<EQH, FD>(pre.s0, k);
+    sHX[2 * (63 - lane) + 1] = EQH ? k.dx * pre.xr : pre.xr;
     wave_lds_sync();
     if (fast)
-      tile_steps_skew<SB, EQH, FD, true>(st_fast, pre, sT, hrow, lane, row_ok, c0, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, FD, true>(st_fast, pre, sT, sHX, lane, row_ok, c0, NC, te, tev, sref, k);
     else
-      tile_steps_skew<SB, EQH, FD, false>(st_edge, pre, sT, hrow, lane, row_ok, c0, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, FD, false>(st_edge, pre, sT, sHX, lane, row_ok, c0, NC, te, tev, sref, k);
   } else {
     // Slowness cells step-major: sS[t][kap] = cell (64r+kap, c0+t-kap+1),
     // kap = 0..64; lane k reads its z-other at [t][k] and x-other at [t][k+1].
@@ -527,20 +540,20 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     for (int v = 0; v < 64; ++v) sS[((v + lane - 1) & 63) * SS_LD + lane] = sv_[v];
     sS[((lane + 63) & 63) * SS_LD + 64] = s64;
     wave_lds_sync();
-    int ta = trow(0, lane);
-    float told_n = sT[ta], z_n = sS[lane], x_n = sS[lane + 1];
+    int tab = 4 * trow(0, lane);
+    float told_n = ring_at(sT, tab), z_n = sS[lane], x_n = sS[lane + 1], h_n = sHX[0];
     for (int t0 = 0; t0 < 64; t0 += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int t = t0 + u;
-        const float told = told_n, h = hrow, szo = z_n, sxo = x_n;
-        const int tn = trow_next(ta);
-        told_n = sT[tn];
-        hrow = dpp_shl1(hrow, hrow);
+        const float told = told_n, h = h_n, szo = z_n, sxo = x_n;
+        const int tn = trow_next_b(tab);
+        told_n = ring_at(sT, tn);
+        h_n = sHX[2 * (t + 1)];
         z_n = sS[(t + 1) * SS_LD + lane];
         x_n = sS[(t + 1) * SS_LD + lane + 1];
-        tile_step<EQH, FD, false>(sT, ta, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
-        ta = tn;
+        tile_step<EQH, FD, false>(sT, tab, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
+        tab = tn;
       }
     }
   }
@@ -589,8 +602,9 @@ __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ 
                                                     SkewView sv) {
   __shared__ float sT[64 * RING_LD];
   __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
+  __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
   const PersistCtl ctl{};
-  tile_process<SA, SB, EQH, false, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS,
+  tile_process<SA, SB, EQH, false, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, sHX,
                                                        [] {}, [] {});
 }
 
@@ -604,6 +618,7 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
                                                       SkewView sv, PersistCtl ctl) {
   __shared__ float sT[64 * RING_LD];
   __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
+  __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
   const unsigned tagb = (*ctl.epoch << 11) | (ctl.sweep_id << 8);
   auto claim_issue = [&]() -> int {
     int t = 0;
@@ -627,7 +642,7 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
     // always one a workgroup is running, and its predecessors are done.
     int raw = 0, next = 0;
     tile_process<SA, SB, EQH, FD, SKEW, FIRST, true>(
-        tl, shots, nzz, k, sv, ctl, tagb, sT, sS, [&] { raw = claim_issue(); },
+        tl, shots, nzz, k, sv, ctl, tagb, sT, sS, sHX, [&] { raw = claim_issue(); },
         [&] { next = __builtin_amdgcn_readfirstlane(raw); });
     tk = next;
   }
