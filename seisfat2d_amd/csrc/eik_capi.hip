@@ -80,6 +80,9 @@ struct seisfat_eik_s {
   float* d_sk[4] = {};
   int KP = 0, NS = 0;
   SkewView skew[4] = {};
+  // 1: workgroups go round robin over 8 XCCs (fused sweep pairs usable), 0:
+  // not, -1: not checked yet
+  int xcc_ok = -1;
   seisfat_batch_s* single = nullptr;  // batch of the last seisfat_eik_solve
 };
 
@@ -142,7 +145,15 @@ struct seisfat_batch_s {
   // persistent schedule state
   unsigned long long* d_xch = nullptr;  // exchange words of every shot
   size_t xch_words = 0;
-  int* d_ctl = nullptr;  // [0..7] ticket counters, [8] epoch, [9] error flag
+  int* d_ctl = nullptr;  // [0..NUM_COUNTERS) ticket counters, [CTL_EPOCH] epoch, [CTL_ERR] error flag
+  // fused sweep pairs (isotropic solve of the tiled schedule): per-XCD queues
+  bool pairs = false;
+  std::vector<ShotDev> iso_b;  // iso with the pairs' second-sweep exchange buffers
+  ShotDev* d_iso_b = nullptr;
+  int4* d_ptiles = nullptr;
+  int* d_qoff = nullptr;
+  std::vector<int4> pair_tiles;  // host copies of the queues' contents
+  std::vector<int> pair_qoff;
   PersistRun pr{};
   int launches = 0, sweep_launches = 0;
   hipEvent_t ev_sweep[2] = {nullptr, nullptr};
@@ -212,6 +223,9 @@ void free_batch(seisfat_batch_s* b) {
   dfree(b->d_tiles);
   dfree(b->d_xch);
   dfree(b->d_ctl);
+  dfree(b->d_iso_b);
+  dfree(b->d_ptiles);
+  dfree(b->d_qoff);
   AdjState& a = b->adj;
   dfree(a.d_tab);
   dfree(a.d_T64);
@@ -253,7 +267,10 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
     if (sweeps == 0)
       if (int rc = mark(0)) return rc;
     int n = 0;
-    if (b->schedule == SEISFAT_SCHED_TILED) {
+    if (b->schedule == SEISFAT_SCHED_TILED && b->pairs && skew) {
+      n = launch_sweeps_pairs(d_tab, b->d_iso_b, b->d_ptiles, b->d_qoff, b->nstrips, nzz, h->kc, h->eqh, skew, b->pr,
+                              st);
+    } else if (b->schedule == SEISFAT_SCHED_TILED) {
       n = launch_sweeps_persist(d_tab, b->d_tiles, b->ntiles, b->nstrips, nzz, h->kc, h->eqh, skew, true, b->pr,
                                 solve, st);
     } else if (b->schedule == SEISFAT_SCHED_TILED_DIAG) {
@@ -332,6 +349,99 @@ void build_tiles(seisfat_batch_s* b, std::vector<int4>& tiles) {
     }
     b->diag_off.push_back((int)tiles.size());
   }
+}
+
+// Tiles of both sweeps of a fused pair (rows down, then rows up; same column
+// direction), shots spread over the 8 XCDs by tile count, each XCD's tiles in
+// order of their level in the dependency graph (first sweep: diagonal 2r+bb;
+// second sweep: one more than the latest of its own predecessors and of the
+// first-sweep tiles under it), so every tile follows all tiles it waits for.
+// The first-sweep tiles under second-sweep tile (r, bb) follow the same
+// ranges as the kernel's wait (tile_process, XMODE 2).
+void build_pair_tiles(seisfat_batch_s* b, std::vector<int4>& pt, std::vector<int>& qoff) {
+  const int NR = b->h->nzz - 2;
+  const int NRS = (NR + 63) / 64;
+  const int n = b->nshots;
+  std::vector<int> NBs(n), prio(n), xcd(n), order(n);
+  int Dmax = 0;
+  for (int k = 0; k < n; ++k) {
+    NBs[k] = (b->iso[k].nxx - 2 + 63 + 63) / 64;
+    Dmax = std::max(Dmax, 2 * (NRS - 1) + NBs[k]);
+  }
+  for (int k = 0; k < n; ++k) {
+    const int slack = Dmax - (2 * (NRS - 1) + NBs[k]);
+    prio[k] = slack <= 6 ? 3 : slack <= 16 ? 2 : slack <= 30 ? 1 : 0;
+    order[k] = k;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return NBs[a] > NBs[c]; });
+  long long load[8] = {};
+  for (int k : order) {
+    const int x = (int)(std::min_element(load, load + 8) - load);
+    xcd[k] = x;
+    load[x] += NBs[k];
+  }
+  struct E {
+    int lvl, which, k, r, bb;
+  };
+  std::vector<std::vector<E>> per(8);
+  for (int k = 0; k < n; ++k) {
+    const int nb = NBs[k];
+    std::vector<int> Lb((size_t)NRS * nb, 0);
+    for (int r = 0; r < NRS; ++r)
+      for (int bb = 0; bb < nb; ++bb) {
+        int L = 0;
+        if (bb > 0) L = std::max(L, Lb[(size_t)r * nb + bb - 1] + 1);
+        if (r > 0) L = std::max(L, Lb[(size_t)(r - 1) * nb + std::min(bb + 1, nb - 1)] + 1);
+        const int c0 = 64 * bb;
+        const int kk1_hi = std::min(64 * r + 63, NR - 1);
+        const int kk0_lo = NR - 1 - kk1_hi, kk0_hi = NR - 1 - 64 * r;
+        for (int r0 = kk0_lo >> 6; r0 <= (kk0_hi >> 6); ++r0) {
+          const int klo = NR - 1 - 64 * r - std::min(kk0_hi, 64 * r0 + 63);
+          const int khi = NR - 1 - 64 * r - std::max(kk0_lo, 64 * r0);
+          const int C = NR - 1 - 64 * (r + r0);
+          const int blo = std::max((c0 - 2 * khi + C) >> 6, 0);
+          const int bhi = std::min((c0 + 63 - 2 * klo + C) >> 6, nb - 1);
+          if (blo <= bhi) L = std::max(L, 2 * r0 + bhi + 1);
+        }
+        Lb[(size_t)r * nb + bb] = L;
+        per[xcd[k]].push_back(E{2 * r + bb, 0, k, r, bb});
+        per[xcd[k]].push_back(E{L, 1, k, r, bb});
+      }
+  }
+  pt.clear();
+  qoff.assign(1, 0);
+  for (int x = 0; x < 8; ++x) {
+    std::sort(per[x].begin(), per[x].end(), [](const E& a, const E& c) {
+      if (a.lvl != c.lvl) return a.lvl < c.lvl;
+      if (a.which != c.which) return a.which < c.which;
+      if (a.k != c.k) return a.k < c.k;
+      if (a.r != c.r) return a.r < c.r;
+      return a.bb < c.bb;
+    });
+    for (const E& e : per[x]) pt.push_back(make_int4(e.k, e.r, e.bb, prio[e.k] | (e.which << 2)));
+    qoff.push_back((int)pt.size());
+  }
+}
+
+// Whether a launch's workgroups go round robin over 8 XCCs (checked once).
+int check_xcc(seisfat_eik_s* h) {
+  if (h->xcc_ok >= 0) return SEISFAT_OK;
+  const int n = 4096;
+  int* d = nullptr;
+  if (int rc = dalloc(&d, (size_t)n)) return rc;
+  launch_xcc_ids(d, n, h->stream);
+  std::vector<int> ids(n);
+  hipError_t e = hipMemcpyAsync(ids.data(), d, n * sizeof(int), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  dfree(d);
+  if (e != hipSuccess) return fail(SEISFAT_EHIP, hipGetErrorString(e));
+  // round robin over 8 XCCs from wherever the dispatcher stands: every XCC
+  // then receives workgroups of any launch of at least 8 (each workgroup
+  // reads its own XCC id, so the starting point does not matter)
+  h->xcc_ok = ids[0] >= 0 && ids[0] < 8;
+  for (int i = 0; i < n; ++i)
+    if (ids[i] != (ids[0] + i) % 8) h->xcc_ok = 0;
+  return SEISFAT_OK;
 }
 
 int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seisfat_shot* shots,
@@ -452,22 +562,53 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
     TRY(dalloc(&b->d_tiles, tiles.size()));
   }
   if (schedule == SEISFAT_SCHED_TILED) {
-    // exchange words per shot: bottom row of each strip + double-buffered carries
-    std::vector<size_t> xoff(nshots);
-    size_t words = 0;
+    TRY(check_xcc(h));
+    b->pairs = h->xcc_ok == 1;
+    // exchange words per shot: bottom row of each strip + double-buffered
+    // carries (a second set for the fused pairs' second sweeps), then the
+    // pairs' done words (one per tile)
+    std::vector<size_t> xoff(nshots), doff(nshots);
+    size_t words = 0, dwords = 0;
     for (int k = 0; k < nshots; ++k) {
       xoff[k] = words;
       const int ldh = ((b->iso[k].nxx - 2) + 63) / 64 * 64;
       words += (size_t)b->nstrips * ldh + (size_t)b->nstrips * 2 * 128;
+      doff[k] = dwords;
+      dwords += (size_t)b->nstrips * ((b->iso[k].nxx - 2 + 126) / 64);
     }
-    b->xch_words = words;
-    TRY(dalloc(&b->d_xch, words));
-    TRY(dalloc(&b->d_ctl, 16));
+    const size_t total = b->pairs ? 2 * words + dwords : words;
+    b->xch_words = total;
+    TRY(dalloc(&b->d_xch, total));
+    TRY(dalloc(&b->d_ctl, CTL_WORDS));
     for (int k = 0; k < nshots; ++k) {
       const int ldh = ((b->iso[k].nxx - 2) + 63) / 64 * 64;
       b->iso[k].xh = b->d_xch + xoff[k];
       b->iso[k].xc = b->d_xch + xoff[k] + (size_t)b->nstrips * ldh;
       b->iso[k].ldh = ldh;
+      if (b->pairs) {
+        b->iso[k].xd = b->d_xch + 2 * words + doff[k];
+        b->iso[k].nb = (b->iso[k].nxx - 2 + 126) / 64;
+      }
+    }
+    if (b->pairs) {
+      b->iso_b = b->iso;
+      for (int k = 0; k < nshots; ++k) {
+        b->iso_b[k].xh = b->iso[k].xh + words;
+        b->iso_b[k].xc = b->iso[k].xc + words;
+      }
+      std::vector<int4> pt;
+      std::vector<int> qoff;
+      build_pair_tiles(b, pt, qoff);
+      TRY(dalloc(&b->d_iso_b, nshots));
+      TRY(dalloc(&b->d_ptiles, pt.size()));
+      TRY(dalloc(&b->d_qoff, qoff.size()));
+      HIPCHK_B(hipMemcpyAsync(b->d_iso_b, b->iso_b.data(), nshots * sizeof(ShotDev), hipMemcpyHostToDevice,
+                              h->stream));
+      HIPCHK_B(hipMemcpyAsync(b->d_ptiles, pt.data(), pt.size() * sizeof(int4), hipMemcpyHostToDevice, h->stream));
+      HIPCHK_B(hipMemcpyAsync(b->d_qoff, qoff.data(), qoff.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+      // the host copies are read back only after the stream synchronises below
+      b->pair_tiles = std::move(pt);
+      b->pair_qoff = std::move(qoff);
     }
     for (int k = 0; k < (int)b->qp.size(); ++k) {
       b->qp[k].xh = b->iso[k].xh;
@@ -475,11 +616,11 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
       b->qp[k].ldh = b->iso[k].ldh;
     }
     b->pr.counters = b->d_ctl;
-    b->pr.epoch = (unsigned*)(b->d_ctl + 8);
-    b->pr.err = b->d_ctl + 9;
+    b->pr.epoch = (unsigned*)(b->d_ctl + CTL_EPOCH);
+    b->pr.err = b->d_ctl + CTL_ERR;
     // all-ones words carry a tag no run ever uses; epoch and flags start at 0
-    HIPCHK_B(hipMemsetAsync(b->d_xch, 0xff, words * sizeof(unsigned long long), h->stream));
-    HIPCHK_B(hipMemsetAsync(b->d_ctl, 0, 16 * sizeof(int), h->stream));
+    HIPCHK_B(hipMemsetAsync(b->d_xch, 0xff, total * sizeof(unsigned long long), h->stream));
+    HIPCHK_B(hipMemsetAsync(b->d_ctl, 0, CTL_WORDS * sizeof(int), h->stream));
   }
   auto up = [&](void* dst, const void* src, size_t bytes) -> int {
     if (bytes == 0) return SEISFAT_OK;
@@ -548,7 +689,7 @@ int batch_status(seisfat_batch_s* b) {
   HIPCHK(hipDeviceSynchronize());
   if (b->d_ctl) {
     int err = 0;
-    HIPCHK(hipMemcpy(&err, b->d_ctl + 9, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&err, b->d_ctl + CTL_ERR, sizeof(int), hipMemcpyDeviceToHost));
     if (err) return fail(SEISFAT_EHIP, "persistent sweep: a tile dependency did not arrive within the spin limit");
   }
   if (b->adj.d_walk_ctl) {

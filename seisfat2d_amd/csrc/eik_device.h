@@ -124,6 +124,10 @@ struct ShotDev {
   unsigned long long* xh;  // strip r's bottom row at [r * ldh + c]
   unsigned long long* xc;  // strip r's carries at [(2r + block parity) * 128 + 2*lane + {0,1}]
   int ldh;
+  // fused sweep pairs: done word of the pair's first-sweep tile (r, bb) at
+  // [r * nb + bb]; nb = blocks per strip
+  unsigned long long* xd;
+  int nb;
 };
 
 // Skewed copy of the row-padded model slowness for one sweep orientation

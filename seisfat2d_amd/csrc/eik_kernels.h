@@ -37,13 +37,26 @@ int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh
 // the initial traveltimes (no fill/init launches needed before it).
 int launch_sweeps_tiled(const ShotDev* shots, const int4* tiles, const std::vector<int>& diag_off, int nzz,
                         const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused, hipStream_t st);
-// Device state of the persistent schedule: 8 ticket counters (two solves x
-// four sweeps), the run epoch and the spin-limit error flag.
+// Device state of the persistent schedule: ticket counters (two solves x
+// four sweeps, then two fused sweep pairs x 8 XCD queues), the run epoch and
+// the spin-limit error flag.
+constexpr int PAIR_COUNTERS = 8;   // first fused-pair counter
+constexpr int NUM_COUNTERS = 24;   // counters zeroed at the start of a run
+constexpr int CTL_EPOCH = 32, CTL_ERR = 33, CTL_WORDS = 40;  // control block layout (ints)
 struct PersistRun {
   int* counters;
   unsigned* epoch;
   int* err;
 };
+// Both sweeps of each pair (0,1) and (2,3) in one launch each, shots
+// partitioned over the 8 XCDs: ptiles holds each XCD's tiles of both sweeps
+// (w bit 2 = second sweep, bits 0-1 = priority) at [qoff[x], qoff[x+1]);
+// shots_b differs from shots_a only in its exchange buffers.
+int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const int4* ptiles, const int* qoff,
+                        int nstrips, int nzz, const EikConsts& k, bool eqh, const SkewView* skew,
+                        const PersistRun& pr, hipStream_t st);
+// out[i] = XCC id of workgroup i (i < n)
+void launch_xcc_ids(int* out, int n, hipStream_t st);
 int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, int nstrips, int nzz,
                           const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused,
                           const PersistRun& pr, int solve, hipStream_t st);

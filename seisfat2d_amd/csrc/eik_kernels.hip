@@ -365,6 +365,7 @@ struct PersistCtl {
   int ntiles;
   int nstrips;             // strips per shot (NRS)
   unsigned sweep_id;       // 0..7 within a run
+  unsigned xtag;           // fused pairs: tag of the partner sweep's done words
 };
 
 // One 64x64 skewed tile of a sweep.
@@ -390,7 +391,14 @@ struct PersistCtl {
 // upwind row and the carry-in from block bb-1 arrive through the tagged
 // exchange buffers of the shot (and the tile publishes its own), instead of
 // being read from T after a launch boundary.
-template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST, class AfterStage,
+//
+// XMODE (fused sweep pairs, whose two sweeps differ only in the row
+// direction): 1 = the pair's first sweep, which marks each tile done once its
+// write-back has reached L2; 2 = the second sweep, which before staging waits
+// for the done words of the first-sweep tiles holding its nodes and drops the
+// CU's L1 copies.  Both sweeps of a shot run on one XCD, whose L2 is shared,
+// so completion at L2 is all a reader on another CU needs.
+template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST, int XMODE, class AfterStage,
           class BeforeWriteBack>
 __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __restrict__ shots, int nzz,
                                              const EikConsts& k, const SkewView& sv, const PersistCtl& ctl,
@@ -411,6 +419,34 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   // interior tile: every column it stages or writes lies in [0, NC)
   const bool interior = c0 >= 64 && c0 + 64 <= NC;
   const int jb = (SB > 0) ? 1 + c0 : nxx - 2 - c0;  // grid column of local column c0
+
+  if (XMODE == 2) {
+    // The partner sweep runs the rows the other way, columns the same: this
+    // tile's local rows kk1 are its rows NR-1-kk1 (strips r0, at most two),
+    // and its node (kk0, cc) lies in block floor((cc + kk0 mod 64) / 64).
+    // Over this tile, cc + kk0 - 64 r0 = c0 + t - 2k + C (t, k in 0..63), so
+    // each strip needs at most four consecutive blocks: lane j < 8 waits for
+    // strip r0lo + j/4, block blo + j%4.
+    const int kk1_hi = min(64 * r + 63, NR - 1);
+    const int kk0_lo = NR - 1 - kk1_hi, kk0_hi = NR - 1 - 64 * r;
+    const int r0 = (kk0_lo >> 6) + (lane >> 2);
+    bool need = false;
+    const unsigned long long* fp = sd.xd;
+    if (lane < 8 && r0 <= (kk0_hi >> 6)) {
+      const int klo = NR - 1 - 64 * r - min(kk0_hi, 64 * r0 + 63);
+      const int khi = NR - 1 - 64 * r - max(kk0_lo, 64 * r0);
+      const int C = NR - 1 - 64 * (r + r0);
+      const int blo = max((c0 - 2 * khi + C) >> 6, 0);
+      const int bhi = min((c0 + 63 - 2 * klo + C) >> 6, sd.nb - 1);
+      const int b0 = blo + (lane & 3);
+      need = b0 <= bhi;
+      fp = sd.xd + (size_t)r0 * sd.nb + min(b0, sd.nb - 1);
+    }
+    const XDep deps[4] = {{fp, need, ctl.xtag}, {fp, false, 0u}, {fp, false, 0u}, {fp, false, 0u}};
+    float dv[4];
+    xwait4(deps, dv, ctl.err);
+    asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1 may hold the old values
+  }
 
   // ---- old traveltimes, step-major --------------------------------------
   if (FIRST) {
@@ -593,6 +629,10 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
       if (row_ok && c >= 0 && c < NC) T[(unsigned)node_off<SA, SB>(kg, c, nzz, nxx)] = val;
     }
   }
+  if (XMODE == 1) {
+    __builtin_amdgcn_s_waitcnt(0);  // the write-back stores have reached L2
+    if (lane == 0) xstore(sd.xd + (size_t)r * sd.nb + bb, 0.0f, tagb);
+  }
 }
 
 // One tile per workgroup; launch d runs the tiles of tile diagonal d = 2r+bb.
@@ -604,7 +644,7 @@ __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ 
   __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
   __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
   const PersistCtl ctl{};
-  tile_process<SA, SB, EQH, false, SKEW, FIRST, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, sHX,
+  tile_process<SA, SB, EQH, false, SKEW, FIRST, false, 0>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, sHX,
                                                        [] {}, [] {});
 }
 
@@ -641,11 +681,67 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
     // workgroup's tickets increase, so the smallest unfinished ticket is
     // always one a workgroup is running, and its predecessors are done.
     int raw = 0, next = 0;
-    tile_process<SA, SB, EQH, FD, SKEW, FIRST, true>(
+    tile_process<SA, SB, EQH, FD, SKEW, FIRST, true, 0>(
         tl, shots, nzz, k, sv, ctl, tagb, sT, sS, sHX, [&] { raw = claim_issue(); },
         [&] { next = __builtin_amdgcn_readfirstlane(raw); });
     tk = next;
   }
+}
+
+// Fused sweep pair: the two sweeps (1, SB) and (-1, SB) of a solve in one
+// persistent launch.  Each shot belongs to one XCD (the workgroup's XCC id,
+// read from the hardware register, picks its queue): its tiles of both sweeps
+// are claimed there in one ticket order in which every tile follows the tiles
+// it waits for, so a second-sweep tile starts as soon as the first-sweep
+// tiles under it are written back, not when the whole first sweep is done.
+// Per-XCD queues drain for the same reason a single queue does.
+template <int SB, bool EQH, bool FD, bool FIRST_A>
+__global__ __launch_bounds__(64) void k_sweep_pair(const ShotDev* __restrict__ shots_a,
+                                                   const ShotDev* __restrict__ shots_b,
+                                                   const int4* __restrict__ tiles, const int* __restrict__ qoff,
+                                                   int nzz, EikConsts k, SkewView sv_a, SkewView sv_b,
+                                                   PersistCtl ctl_a, PersistCtl ctl_b) {
+  __shared__ float sT[64 * RING_LD];
+  __shared__ float sS[1];
+  __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
+  const int xcd = __builtin_amdgcn_s_getreg((15 << 11) | 20) & 7;  // hwreg(HW_REG_XCC_ID, 0, 16)
+  const unsigned tag_a = (*ctl_a.epoch << 11) | (ctl_a.sweep_id << 8);
+  const unsigned tag_b = (*ctl_b.epoch << 11) | (ctl_b.sweep_id << 8);
+  PersistCtl cb = ctl_b;
+  cb.xtag = tag_a;
+  int* const counter = ctl_a.counter + xcd;
+  const int base = qoff[xcd], n = qoff[xcd + 1] - base;
+  auto claim_issue = [&]() -> int {
+    int t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(counter, 1);
+    return t;
+  };
+  int tk = __builtin_amdgcn_readfirstlane(claim_issue());
+  while (tk < n) {
+    const int4 tl = tiles[base + tk];
+    switch (tl.w & 3) {
+      case 3: __builtin_amdgcn_s_setprio(3); break;
+      case 2: __builtin_amdgcn_s_setprio(2); break;
+      case 1: __builtin_amdgcn_s_setprio(1); break;
+      default: __builtin_amdgcn_s_setprio(0); break;
+    }
+    int raw = 0, next = 0;
+    if (tl.w & 4)
+      tile_process<-1, SB, EQH, FD, true, false, true, 2>(
+          tl, shots_b, nzz, k, sv_b, cb, tag_b, sT, sS, sHX, [&] { raw = claim_issue(); },
+          [&] { next = __builtin_amdgcn_readfirstlane(raw); });
+    else
+      tile_process<1, SB, EQH, FD, true, FIRST_A, true, 1>(
+          tl, shots_a, nzz, k, sv_a, ctl_a, tag_a, sT, sS, sHX, [&] { raw = claim_issue(); },
+          [&] { next = __builtin_amdgcn_readfirstlane(raw); });
+    tk = next;
+  }
+}
+
+// XCC id of each of n workgroups (the fused pairs rely on workgroups going
+// round robin over 8 XCCs; checked once per geometry).
+__global__ void k_xcc_ids(int* __restrict__ out) {
+  if (threadIdx.x == 0) out[blockIdx.x] = (int)(__builtin_amdgcn_s_getreg((15 << 11) | 20) & 0xffff);
 }
 
 // Start of a run: advance the epoch (tags of the previous run become stale)
@@ -976,8 +1072,69 @@ int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, i
   return 4;
 }
 
+template <int SB, bool EQH, bool FD, bool FIRST_A>
+static void pair_launch(const ShotDev* shots_a, const ShotDev* shots_b, const int4* tiles, const int* qoff,
+                        int nzz, const EikConsts& k, const SkewView& sv_a, const SkewView& sv_b,
+                        const PersistCtl& ca, const PersistCtl& cb, hipStream_t st) {
+  static int grid = 0;
+  if (grid == 0) {
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_pair<SB, EQH, FD, FIRST_A>, 64, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+      per_cu = 4;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    grid = per_cu * cus;
+  }
+  hipLaunchKernelGGL((k_sweep_pair<SB, EQH, FD, FIRST_A>), dim3(grid), dim3(64), 0, st, shots_a, shots_b, tiles,
+                     qoff, nzz, k, sv_a, sv_b, ca, cb);
+}
+
+template <int SB, bool FIRST_A>
+static void pair_dispatch(bool eqh, bool fd, const ShotDev* shots_a, const ShotDev* shots_b, const int4* tiles,
+                          const int* qoff, int nzz, const EikConsts& k, const SkewView& sv_a,
+                          const SkewView& sv_b, const PersistCtl& ca, const PersistCtl& cb, hipStream_t st) {
+  if (eqh) {
+    if (fd) pair_launch<SB, true, true, FIRST_A>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st);
+    else pair_launch<SB, true, false, FIRST_A>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st);
+  } else {
+    if (fd) pair_launch<SB, false, true, FIRST_A>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st);
+    else pair_launch<SB, false, false, FIRST_A>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st);
+  }
+}
+
+int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const int4* ptiles, const int* qoff,
+                        int nstrips, int nzz, const EikConsts& k, bool eqh, const SkewView* skew,
+                        const PersistRun& pr, hipStream_t st) {
+  for (int p = 0; p < 2; ++p) {
+    PersistCtl ca, cb;
+    ca.counter = pr.counters + PAIR_COUNTERS + 8 * p;
+    ca.epoch = cb.epoch = pr.epoch;
+    ca.err = cb.err = pr.err;
+    ca.ntiles = cb.ntiles = 0;
+    ca.nstrips = cb.nstrips = nstrips;
+    ca.sweep_id = (unsigned)(2 * p);
+    cb.sweep_id = (unsigned)(2 * p + 1);
+    ca.xtag = cb.xtag = 0u;
+    cb.counter = ca.counter;
+    if (p == 0)
+      pair_dispatch<1, true>(eqh, k.fastdiv != 0, shots_a, shots_b, ptiles, qoff, nzz, k, skew[0], skew[1], ca, cb,
+                             st);
+    else
+      pair_dispatch<-1, false>(eqh, k.fastdiv != 0, shots_a, shots_b, ptiles, qoff, nzz, k, skew[2], skew[3], ca,
+                               cb, st);
+  }
+  return 2;
+}
+
+void launch_xcc_ids(int* out, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_xcc_ids, dim3(n), dim3(64), 0, st, out);
+}
+
 void launch_begin_run(const PersistRun& pr, hipStream_t st) {
-  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(64), 0, st, pr.counters, 8, pr.epoch);
+  hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(64), 0, st, pr.counters, NUM_COUNTERS, pr.epoch);
 }
 
 // Skewed slowness copy for sweep orientation (SA, SB): one thread per entry.
