@@ -176,6 +176,35 @@ def test_batch_many_tiles_per_workgroup_bit_exact(oracle, rng):
     e.close()
 
 
+def test_batch_fused_sweep_pairs_bit_exact(oracle, rng):
+    """The tiled isotropic schedule runs the sweeps as two fused pairs with the
+    shots spread over the XCDs (a second-sweep tile starts once the first-sweep
+    tiles under it are written back): two sweep launches per solve, and every
+    shot bit-exact against the oracle on three consecutive runs, with odd
+    strip counts and windows of 1 to 3 columns among them."""
+    nz, nx, dh = 97, 900, 5.0
+    s = smooth_slowness(rng, nz, nx, dh)
+    shots = []
+    for k in range(75):
+        wd = [1, 2, 3][k] if k < 3 else int(rng.integers(4, nx + 1))
+        x0 = int(rng.integers(0, nx - wd + 1))
+        sx = float((x0 + rng.uniform(0.0, wd - 1)) * dh)
+        shots.append((sx, float(rng.uniform(0.0, (nz - 1) * dh)), x0, wd, 0, 0))
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    b = e.batch(shots, schedule=SCHED_TILED)
+    assert b.num_sweep_launches == 2
+    Trefs = oracle.solve_windows(s, dh, dh, [x[0] for x in shots], [x[1] for x in shots],
+                                 [x[2] for x in shots], [x[3] for x in shots], nthreads=16)
+    for run in range(3):
+        b.run()
+        b.status()
+        for k, Tr in enumerate(Trefs):
+            assert np.array_equal(b.T(k), Tr), (run, k)
+    b.close()
+    e.close()
+
+
 # dz = dx = 0x1.94f5a6p+2: the two-instruction division by dz rounds one
 # dividend per binade away from IEEE division, so this geometry must run the
 # five-instruction kernels
