@@ -395,9 +395,12 @@ struct PersistCtl {
 // XMODE (fused sweep pairs, whose two sweeps differ only in the row
 // direction): 1 = the pair's first sweep, which marks each tile done once its
 // write-back has reached L2; 2 = the second sweep, which before staging waits
-// for the done words of the first-sweep tiles holding its nodes and drops the
-// CU's L1 copies.  Both sweeps of a shot run on one XCD, whose L2 is shared,
-// so completion at L2 is all a reader on another CU needs.
+// for the done words of the first-sweep tiles holding its nodes and then
+// stages with agent-scope loads (cache policy sc1, as the compiler issues
+// agent-scope atomic loads), which do not hit L1 lines this CU may hold from
+// earlier tiles.  Both sweeps of a shot run on one XCD, whose L2 is shared,
+// so completion at L2 is all such a reader needs, and no L1 invalidation
+// (which would also drop the other waves' lines) is.
 template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST, int XMODE, class AfterStage,
           class BeforeWriteBack>
 __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __restrict__ shots, int nzz,
@@ -419,6 +422,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   // interior tile: every column it stages or writes lies in [0, NC)
   const bool interior = c0 >= 64 && c0 + 64 <= NC;
   const int jb = (SB > 0) ? 1 + c0 : nxx - 2 - c0;  // grid column of local column c0
+  constexpr int STAGE_CPOL = XMODE == 2 ? 16 : 0;   // cache policy of the staging loads: sc1 (agent scope)
 
   if (XMODE == 2) {
     // The partner sweep runs the rows the other way, columns the same: this
@@ -445,7 +449,6 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     const XDep deps[4] = {{fp, need, ctl.xtag}, {fp, false, 0u}, {fp, false, 0u}, {fp, false, 0u}};
     float dv[4];
     xwait4(deps, dv, ctl.err);
-    asm volatile("buffer_inv sc0" ::: "memory");  // this CU's L1 may hold the old values
   }
 
   // ---- old traveltimes, step-major --------------------------------------
@@ -487,7 +490,8 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
       const gfloat* Tb = T + (jb - (SB > 0 ? 64 : 0)) * nzz;
 #pragma unroll 2
       for (int v = 0; v < 64; ++v) {
-        __builtin_amdgcn_global_load_lds(Tb + (lane >= 64 - v ? o1 : o0), (lfloat*)(sT + v * RING_LD), 4, 0, 0);
+        __builtin_amdgcn_global_load_lds(Tb + (lane >= 64 - v ? o1 : o0), (lfloat*)(sT + v * RING_LD), 4, 0,
+                                         STAGE_CPOL);
         Tb += SB * nzz;
       }
     } else {
@@ -495,7 +499,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
       for (int v = 0; v < 64; ++v) {
         const int c = min(max(c0 - lane + ((v + lane) & 63), -1), NC);
         __builtin_amdgcn_global_load_lds(T + (unsigned)node_off<SA, SB>(kgc, c, nzz, nxx),
-                                         (lfloat*)(sT + v * RING_LD), 4, 0, 0);
+                                         (lfloat*)(sT + v * RING_LD), 4, 0, STAGE_CPOL);
       }
     }
     // (waited for below, after the halo and carry loads: their latencies overlap)
