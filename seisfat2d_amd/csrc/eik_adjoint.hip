@@ -345,14 +345,21 @@ __global__ __launch_bounds__(256) void k_adj_coef(const AdjShot* __restrict__ sh
                                                   int pa, int pb, const float* __restrict__ lam_src,
                                                   float* __restrict__ lam_dst, float4* __restrict__ coef,
                                                   uint8_t* __restrict__ code) {
+  // one thread per entry of the strip-skewed layout (a wave writes 64
+  // consecutive entries: strip rs, step t, lanes 0..63), the node it holds
+  // read where it lies; entries outside the grid are left untouched
   const AdjShot sh = shots[blockIdx.y];
-  const int nxx = sh.sd.nxx;
-  const int n = nzz * nxx;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  const int i = idx % nzz, j = idx / nzz;
-  const size_t ko = sh.koff + adj_kidx(adj_ip(i, SA, nzz), adj_jp(j, SB, nxx), sh.NTP);
-  if (lam_src) lam_dst[ko] = lam_src[sh.koff + adj_kidx(adj_ip(i, pa, nzz), adj_jp(j, pb, nxx), sh.NTP)];
+  const int nxx = sh.sd.nxx, NTP = sh.NTP;
+  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(o & 63);
+  const size_t row = o >> 6;
+  const int rs = (int)(row / NTP), t = (int)(row - (size_t)rs * NTP);
+  const int ip = 64 * rs + lane, jp = t - lane;
+  if (ip >= nzz || jp < 0 || jp >= nxx) return;
+  const int i = adj_ip(ip, SA, nzz), j = adj_jp(jp, SB, nxx);  // (both maps are involutions)
+  const size_t idx = i + (size_t)j * nzz;
+  const size_t ko = sh.koff + o;
+  if (lam_src) lam_dst[ko] = lam_src[sh.koff + adj_kidx(adj_ip(i, pa, nzz), adj_jp(j, pb, nxx), NTP)];
   AdjCoef c{0.0f, 0.0f, 0.0f, -1.0f, 0};
   if (i >= 1 && i <= nzz - 2 && j >= 1 && j <= nxx - 2) {
     const double* __restrict__ Tn = Tnew + sh.toff;
@@ -553,10 +560,10 @@ void launch_picks64(const AdjShot* shots, const int* rec_shot, const float* rx, 
                      dz, T4, out);
 }
 
-void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int max_nodes, int nzz, double dx, double dz,
+void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int max_entries, int nzz, double dx, double dz,
                      const double* Tnew, const double* Told, const float* lam_src, float* lam_dst, float4* coef,
                      uint8_t* code, hipStream_t st) {
-  const dim3 grid((max_nodes + 255) / 256, nshots);
+  const dim3 grid((max_entries + 255) / 256, nshots);
   const int pa = adj_a(sweep + 1), pb = adj_b(sweep + 1);
   const ConstsD k = consts_d(dx, dz);
   switch (sweep) {

@@ -93,7 +93,7 @@ struct AdjState {
   unsigned version = 0;
   std::vector<AdjShot> tab;
   AdjShot* d_tab = nullptr;
-  int nstr = 0, max_nodes = 0;
+  int nstr = 0, max_nodes = 0, max_kent = 0;
   size_t ksize = 0, xsize = 0;
   double* d_T64 = nullptr;  // fp64 trajectory: T0 (initial field) .. T4, tsize each
   float* d_lamA = nullptr;
@@ -736,6 +736,7 @@ int adj_prepare(seisfat_batch_s* b) {
   A.tab.resize(nshots);
   size_t koff = 0, xoff = 0;
   A.max_nodes = 0;
+  A.max_kent = 0;
   for (int k = 0; k < nshots; ++k) {
     AdjShot& s = A.tab[k];
     s.sd = b->iso[k];
@@ -748,6 +749,7 @@ int adj_prepare(seisfat_batch_s* b) {
     koff += (size_t)A.nstr * s.NTP * 64;
     xoff += (size_t)(A.nstr + 1) * s.xrs;
     A.max_nodes = std::max(A.max_nodes, nzz * nxx);
+    A.max_kent = std::max(A.max_kent, A.nstr * s.NTP * 64);
   }
   A.ksize = koff;
   A.xsize = xoff;
@@ -876,7 +878,7 @@ int enqueue_vjp(seisfat_batch_s* b, const float* d_w, double* d_grad, hipStream_
   launch_fill(A.d_gshot, b->tsize, 0.0f, st);
   const size_t ts = b->tsize;
   for (int s = 3; s >= 0; --s) {
-    launch_adj_coef(s, A.d_tab, ns, A.max_nodes, nzz, h->dx, h->dz, A.d_T64 + (s + 1) * ts, A.d_T64 + s * ts,
+    launch_adj_coef(s, A.d_tab, ns, A.max_kent, nzz, h->dx, h->dz, A.d_T64 + (s + 1) * ts, A.d_T64 + s * ts,
                     s == 3 ? nullptr : A.d_lamA, A.d_lamB, A.d_coef, A.d_code, st);
     launch_adj_sweep(A.d_tab, ns, A.nstr, A.d_lamB, A.d_coef, A.d_lamA, A.d_gc, A.d_xmsg, st);
     launch_adj_fold(s, A.d_tab, ns, A.max_nodes, nzz, A.d_gc, A.d_code, A.d_gshot, st);
