@@ -148,8 +148,9 @@ struct seisfat_batch_s {
   int* d_ctl = nullptr;  // [0..NUM_COUNTERS) ticket counters, [CTL_EPOCH] epoch, [CTL_ERR] error flag
   // fused sweep pairs (isotropic solve of the tiled schedule): per-XCD queues
   bool pairs = false;
-  std::vector<ShotDev> iso_b;  // iso with the pairs' second-sweep exchange buffers
+  std::vector<ShotDev> iso_b, qp_b;  // iso / qp with the pairs' second-sweep exchange buffers
   ShotDev* d_iso_b = nullptr;
+  ShotDev* d_qp_b = nullptr;
   int4* d_ptiles = nullptr;
   int* d_qoff = nullptr;
   std::vector<int4> pair_tiles;  // host copies of the queues' contents
@@ -224,6 +225,7 @@ void free_batch(seisfat_batch_s* b) {
   dfree(b->d_xch);
   dfree(b->d_ctl);
   dfree(b->d_iso_b);
+  dfree(b->d_qp_b);
   dfree(b->d_ptiles);
   dfree(b->d_qoff);
   AdjState& a = b->adj;
@@ -256,7 +258,8 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
   const int nzz = h->nzz;
   int launches = 0, sweeps = 0;
   int solve = 0;
-  auto one_solve = [&](ShotDev* d_tab, const std::vector<ShotDev>& tab, const SkewView* skew) -> int {
+  auto one_solve = [&](ShotDev* d_tab, ShotDev* d_tab_b, const std::vector<ShotDev>& tab,
+                       const SkewView* skew) -> int {
     if (b->schedule == SEISFAT_SCHED_PLAIN) {
       // time_set + time_init as separate launches, as the reference does; the
       // tiled sweeps generate the initial field inside their first sweep.
@@ -267,9 +270,9 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
     if (sweeps == 0)
       if (int rc = mark(0)) return rc;
     int n = 0;
-    if (b->schedule == SEISFAT_SCHED_TILED && b->pairs && skew) {
-      n = launch_sweeps_pairs(d_tab, b->d_iso_b, b->d_ptiles, b->d_qoff, b->nstrips, nzz, h->kc, h->eqh, skew, b->pr,
-                              st);
+    if (b->schedule == SEISFAT_SCHED_TILED && b->pairs && d_tab_b) {
+      n = launch_sweeps_pairs(d_tab, d_tab_b, b->d_ptiles, b->d_qoff, b->nstrips, nzz, h->kc, h->eqh, skew, b->pr,
+                              solve, st);
     } else if (b->schedule == SEISFAT_SCHED_TILED) {
       n = launch_sweeps_persist(d_tab, b->d_tiles, b->ntiles, b->nstrips, nzz, h->kc, h->eqh, skew, true, b->pr,
                                 solve, st);
@@ -288,7 +291,7 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
     launch_begin_run(b->pr, st);
     ++launches;
   }
-  int rc = one_solve(b->d_iso, b->iso, h->skew);
+  int rc = one_solve(b->d_iso, b->d_iso_b, b->iso, h->skew);
   if (rc) return rc;
   if (b->type == SEISFAT_ANI) {
     StiffDev sf;
@@ -305,7 +308,7 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
     launch_ani_table(model, nzz, h->nxx, sf, tb, st);
     launch_quasi_slowness(b->d_iso, b->d_qp, b->nshots, nzz, h->nxx, h->dx, h->dz, tb, st);
     launches += 2;
-    rc = one_solve(b->d_qp, b->qp, nullptr);  // per-shot qP slowness: staged in LDS
+    rc = one_solve(b->d_qp, b->d_qp_b, b->qp, nullptr);  // per-shot qP slowness: staged in LDS
     if (rc) return rc;
   }
   if ((rc = mark(1))) return rc;
@@ -610,10 +613,21 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
       b->pair_tiles = std::move(pt);
       b->pair_qoff = std::move(qoff);
     }
+    // the qP solve (anisotropic batches) runs after the isotropic one and
+    // reuses its exchange and done words under its own sweep ids
     for (int k = 0; k < (int)b->qp.size(); ++k) {
       b->qp[k].xh = b->iso[k].xh;
       b->qp[k].xc = b->iso[k].xc;
       b->qp[k].ldh = b->iso[k].ldh;
+      b->qp[k].xd = b->iso[k].xd;
+      b->qp[k].nb = b->iso[k].nb;
+    }
+    if (b->pairs && !b->qp.empty()) {
+      b->qp_b = b->qp;
+      for (int k = 0; k < nshots; ++k) {
+        b->qp_b[k].xh = b->iso_b[k].xh;
+        b->qp_b[k].xc = b->iso_b[k].xc;
+      }
     }
     b->pr.counters = b->d_ctl;
     b->pr.epoch = (unsigned*)(b->d_ctl + CTL_EPOCH);
@@ -629,6 +643,10 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   };
   TRY(up(b->d_iso, b->iso.data(), nshots * sizeof(ShotDev)));
   if (type == SEISFAT_ANI) TRY(up(b->d_qp, b->qp.data(), nshots * sizeof(ShotDev)));
+  if (!b->qp_b.empty()) {
+    TRY(dalloc(&b->d_qp_b, nshots));
+    TRY(up(b->d_qp_b, b->qp_b.data(), nshots * sizeof(ShotDev)));
+  }
   TRY(up(b->d_rx, rxr.data(), nrec_total * sizeof(float)));
   TRY(up(b->d_rz, rzr.data(), nrec_total * sizeof(float)));
   TRY(up(b->d_recshot, recshot.data(), nrec_total * sizeof(int)));

@@ -38,11 +38,11 @@ int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh
 int launch_sweeps_tiled(const ShotDev* shots, const int4* tiles, const std::vector<int>& diag_off, int nzz,
                         const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused, hipStream_t st);
 // Device state of the persistent schedule: ticket counters (two solves x
-// four sweeps, then two fused sweep pairs x 8 XCD queues), the run epoch and
-// the spin-limit error flag.
+// four sweeps, then two solves x two fused sweep pairs x 8 XCD queues), the
+// run epoch and the spin-limit error flag.
 constexpr int PAIR_COUNTERS = 8;   // first fused-pair counter
-constexpr int NUM_COUNTERS = 24;   // counters zeroed at the start of a run
-constexpr int CTL_EPOCH = 32, CTL_ERR = 33, CTL_WORDS = 40;  // control block layout (ints)
+constexpr int NUM_COUNTERS = 40;   // counters zeroed at the start of a run
+constexpr int CTL_EPOCH = 40, CTL_ERR = 41, CTL_WORDS = 48;  // control block layout (ints)
 struct PersistRun {
   int* counters;
   unsigned* epoch;
@@ -51,10 +51,12 @@ struct PersistRun {
 // Both sweeps of each pair (0,1) and (2,3) in one launch each, shots
 // partitioned over the 8 XCDs: ptiles holds each XCD's tiles of both sweeps
 // (w bit 2 = second sweep, bits 0-1 = priority) at [qoff[x], qoff[x+1]);
-// shots_b differs from shots_a only in its exchange buffers.
+// shots_b differs from shots_a only in its exchange buffers.  skew: the
+// skewed slowness copies, or null for slowness staged from the shots' own
+// grids (the qP solve); solve (0 or 1) selects sweep ids and queue counters.
 int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const int4* ptiles, const int* qoff,
                         int nstrips, int nzz, const EikConsts& k, bool eqh, const SkewView* skew,
-                        const PersistRun& pr, hipStream_t st);
+                        const PersistRun& pr, int solve, hipStream_t st);
 // out[i] = XCC id of workgroup i (i < n)
 void launch_xcc_ids(int* out, int n, hipStream_t st);
 int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, int nstrips, int nzz,

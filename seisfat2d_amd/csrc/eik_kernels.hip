@@ -699,14 +699,14 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
 // it waits for, so a second-sweep tile starts as soon as the first-sweep
 // tiles under it are written back, not when the whole first sweep is done.
 // Per-XCD queues drain for the same reason a single queue does.
-template <int SB, bool EQH, bool FD, bool FIRST_A>
+template <int SB, bool EQH, bool FD, bool FIRST_A, bool SKEW>
 __global__ __launch_bounds__(64) void k_sweep_pair(const ShotDev* __restrict__ shots_a,
                                                    const ShotDev* __restrict__ shots_b,
                                                    const int4* __restrict__ tiles, const int* __restrict__ qoff,
                                                    int nzz, EikConsts k, SkewView sv_a, SkewView sv_b,
                                                    PersistCtl ctl_a, PersistCtl ctl_b) {
   __shared__ float sT[64 * RING_LD];
-  __shared__ float sS[1];
+  __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
   __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
   const int xcd = __builtin_amdgcn_s_getreg((15 << 11) | 20) & 7;  // hwreg(HW_REG_XCC_ID, 0, 16)
   const unsigned tag_a = (*ctl_a.epoch << 11) | (ctl_a.sweep_id << 8);
@@ -731,11 +731,11 @@ __global__ __launch_bounds__(64) void k_sweep_pair(const ShotDev* __restrict__ s
     }
     int raw = 0, next = 0;
     if (tl.w & 4)
-      tile_process<-1, SB, EQH, FD, true, false, true, 2>(
+      tile_process<-1, SB, EQH, FD, SKEW, false, true, 2>(
           tl, shots_b, nzz, k, sv_b, cb, tag_b, sT, sS, sHX, [&] { raw = claim_issue(); },
           [&] { next = __builtin_amdgcn_readfirstlane(raw); });
     else
-      tile_process<1, SB, EQH, FD, true, FIRST_A, true, 1>(
+      tile_process<1, SB, EQH, FD, SKEW, FIRST_A, true, 1>(
           tl, shots_a, nzz, k, sv_a, ctl_a, tag_a, sT, sS, sHX, [&] { raw = claim_issue(); },
           [&] { next = __builtin_amdgcn_readfirstlane(raw); });
     tk = next;
@@ -1076,14 +1076,14 @@ int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, i
   return 4;
 }
 
-template <int SB, bool EQH, bool FD, bool FIRST_A>
+template <int SB, bool EQH, bool FD, bool FIRST_A, bool SKEW>
 static void pair_launch(const ShotDev* shots_a, const ShotDev* shots_b, const int4* tiles, const int* qoff,
                         int nzz, const EikConsts& k, const SkewView& sv_a, const SkewView& sv_b,
                         const PersistCtl& ca, const PersistCtl& cb, hipStream_t st) {
   static int grid = 0;
   if (grid == 0) {
     int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_pair<SB, EQH, FD, FIRST_A>, 64, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_pair<SB, EQH, FD, FIRST_A, SKEW>, 64, 0) !=
             hipSuccess ||
         per_cu <= 0)
       per_cu = 4;
@@ -1092,43 +1092,54 @@ static void pair_launch(const ShotDev* shots_a, const ShotDev* shots_b, const in
       cus = 256;
     grid = per_cu * cus;
   }
-  hipLaunchKernelGGL((k_sweep_pair<SB, EQH, FD, FIRST_A>), dim3(grid), dim3(64), 0, st, shots_a, shots_b, tiles,
+  hipLaunchKernelGGL((k_sweep_pair<SB, EQH, FD, FIRST_A, SKEW>), dim3(grid), dim3(64), 0, st, shots_a, shots_b, tiles,
                      qoff, nzz, k, sv_a, sv_b, ca, cb);
 }
 
 template <int SB, bool FIRST_A>
-static void pair_dispatch(bool eqh, bool fd, const ShotDev* shots_a, const ShotDev* shots_b, const int4* tiles,
-                          const int* qoff, int nzz, const EikConsts& k, const SkewView& sv_a,
+static void pair_dispatch(bool eqh, bool fd, bool skew, const ShotDev* shots_a, const ShotDev* shots_b,
+                          const int4* tiles, const int* qoff, int nzz, const EikConsts& k, const SkewView& sv_a,
                           const SkewView& sv_b, const PersistCtl& ca, const PersistCtl& cb, hipStream_t st) {
-  if (eqh) {
-    if (fd) pair_launch<SB, true, true, FIRST_A>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st);
-    else pair_launch<SB, true, false, FIRST_A>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st);
+#define SEIS_PAIR(E, F, S) pair_launch<SB, E, F, FIRST_A, S>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st)
+  if (skew) {
+    // skewed slowness: the known-divisor division where it is exact
+    if (eqh) {
+      if (fd) SEIS_PAIR(true, true, true);
+      else SEIS_PAIR(true, false, true);
+    } else {
+      if (fd) SEIS_PAIR(false, true, true);
+      else SEIS_PAIR(false, false, true);
+    }
   } else {
-    if (fd) pair_launch<SB, false, true, FIRST_A>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st);
-    else pair_launch<SB, false, false, FIRST_A>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st);
+    if (eqh) SEIS_PAIR(true, false, false);
+    else SEIS_PAIR(false, false, false);
   }
+#undef SEIS_PAIR
 }
 
 int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const int4* ptiles, const int* qoff,
                         int nstrips, int nzz, const EikConsts& k, bool eqh, const SkewView* skew,
-                        const PersistRun& pr, hipStream_t st) {
+                        const PersistRun& pr, int solve, hipStream_t st) {
+  const SkewView none{nullptr, 0};
   for (int p = 0; p < 2; ++p) {
     PersistCtl ca, cb;
-    ca.counter = pr.counters + PAIR_COUNTERS + 8 * p;
+    ca.counter = pr.counters + PAIR_COUNTERS + 16 * solve + 8 * p;
     ca.epoch = cb.epoch = pr.epoch;
     ca.err = cb.err = pr.err;
     ca.ntiles = cb.ntiles = 0;
     ca.nstrips = cb.nstrips = nstrips;
-    ca.sweep_id = (unsigned)(2 * p);
-    cb.sweep_id = (unsigned)(2 * p + 1);
+    ca.sweep_id = (unsigned)(4 * solve + 2 * p);
+    cb.sweep_id = (unsigned)(4 * solve + 2 * p + 1);
     ca.xtag = cb.xtag = 0u;
     cb.counter = ca.counter;
+    const SkewView& sa = skew ? skew[2 * p] : none;
+    const SkewView& sb = skew ? skew[2 * p + 1] : none;
     if (p == 0)
-      pair_dispatch<1, true>(eqh, k.fastdiv != 0, shots_a, shots_b, ptiles, qoff, nzz, k, skew[0], skew[1], ca, cb,
-                             st);
+      pair_dispatch<1, true>(eqh, k.fastdiv != 0, skew != nullptr, shots_a, shots_b, ptiles, qoff, nzz, k, sa, sb,
+                             ca, cb, st);
     else
-      pair_dispatch<-1, false>(eqh, k.fastdiv != 0, shots_a, shots_b, ptiles, qoff, nzz, k, skew[2], skew[3], ca,
-                               cb, st);
+      pair_dispatch<-1, false>(eqh, k.fastdiv != 0, skew != nullptr, shots_a, shots_b, ptiles, qoff, nzz, k, sa, sb,
+                               ca, cb, st);
   }
   return 2;
 }
