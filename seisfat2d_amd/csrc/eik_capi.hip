@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <new>
@@ -429,6 +430,14 @@ void build_pair_tiles(seisfat_batch_s* b, std::vector<int4>& pt, std::vector<int
 // Whether a launch's workgroups go round robin over 8 XCCs (checked once).
 int check_xcc(seisfat_eik_s* h) {
   if (h->xcc_ok >= 0) return SEISFAT_OK;
+  // SEISFAT_SWEEP_PAIRS=0 (read when a handle's first tiled batch is
+  // created): one persistent launch per sweep instead of the fused pairs
+  if (const char* env = std::getenv("SEISFAT_SWEEP_PAIRS")) {
+    if (env[0] == '0') {
+      h->xcc_ok = 0;
+      return SEISFAT_OK;
+    }
+  }
   const int n = 4096;
   int* d = nullptr;
   if (int rc = dalloc(&d, (size_t)n)) return rc;

@@ -205,6 +205,47 @@ def test_batch_fused_sweep_pairs_bit_exact(oracle, rng):
     e.close()
 
 
+def test_batch_per_sweep_launches_bit_exact(oracle, rng, monkeypatch):
+    """The schedule used where workgroups do not go round robin over 8 XCDs
+    (SEISFAT_SWEEP_PAIRS=0 forces it): one persistent launch per sweep, every
+    shot bit-exact against the oracle, isotropic and anisotropic."""
+    monkeypatch.setenv("SEISFAT_SWEEP_PAIRS", "0")
+    nz, nx, dh = 97, 700, 5.0
+    s = smooth_slowness(rng, nz, nx, dh)
+    shots = []
+    for k in range(24):
+        wd = int(rng.integers(1, nx + 1))
+        x0 = int(rng.integers(0, nx - wd + 1))
+        sx = float((x0 + rng.uniform(0.0, wd - 1)) * dh)
+        shots.append((sx, float(rng.uniform(0.0, (nz - 1) * dh)), x0, wd, 0, 0))
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    b = e.batch(shots, schedule=SCHED_TILED)
+    assert b.num_sweep_launches == 4
+    Trefs = oracle.solve_windows(s, dh, dh, [x[0] for x in shots], [x[1] for x in shots],
+                                 [x[2] for x in shots], [x[3] for x in shots], nthreads=16)
+    for run in range(2):
+        b.run()
+        b.status()
+        for k, Tr in enumerate(Trefs):
+            assert np.array_equal(b.T(k), Tr), (run, k)
+    b.close()
+    e.close()
+    nz, nx = 161, 257
+    s, cs, sxs = ani_setup(nz, nx, dh, 3)
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    e.set_stiffness(cs)
+    b = e.batch([(float(x), 0.0, 0, nx, 0, 0) for x in sxs], type=ANI, schedule=SCHED_TILED)
+    assert b.num_sweep_launches == 8
+    b.run()
+    b.status()
+    for k, x in enumerate(sxs):
+        assert np.array_equal(b.T(k), oracle_ani(oracle, s, cs, dh, float(x), 0.0)), k
+    b.close()
+    e.close()
+
+
 # dz = dx = 0x1.94f5a6p+2: the two-instruction division by dz rounds one
 # dividend per binade away from IEEE division, so this geometry must run the
 # five-instruction kernels
