@@ -135,7 +135,7 @@ __device__ __forceinline__ float cell_at(const ShotDev& sd, int nzz, int kap, in
 }
 
 constexpr int RING_LD = 64;  // LDS row stride (words) of a tile's traveltimes
-constexpr int SS_LD = 66;    // LDS row stride of staged slowness (64 lanes + cell row 64 + pad)
+constexpr int SS_STEPS = 16; // steps per slowness staging phase of the non-skewed path
 constexpr int PF = 16;       // slowness prefetch distance (steps) on the skewed path
 
 // A tile's traveltimes in LDS are instruction-major: row v holds, for lane k,
@@ -171,6 +171,25 @@ typedef __attribute__((address_space(3))) float lfloat;
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt untouched
   asm volatile("" ::: "memory");
+}
+
+// Slowness staging of the non-skewed path, SS_STEPS steps per phase: phase p's
+// instruction u fetches, for lane k, its z-other cell of step
+// t = p*SS_STEPS + ((u + k) mod SS_STEPS), cell (kap, c0 + 1 - k + t) (at most
+// five contiguous column runs per instruction); stored step-major,
+// sS[(t mod SS_STEPS)*64 + k], one bank per lane on both sides.
+template <int SA, int SB>
+__device__ __forceinline__ void ss_load(float (&v)[SS_STEPS], const ShotDev& sd, int nzz, int kap, int c0, int lane,
+                                        int NC, int p) {
+#pragma unroll
+  for (int u = 0; u < SS_STEPS; ++u) {
+    const int t = p * SS_STEPS + ((u + lane) & (SS_STEPS - 1));
+    v[u] = cell_at<SA, SB>(sd, nzz, kap, min(max(c0 + 1 - lane + t, 0), NC));
+  }
+}
+__device__ __forceinline__ void ss_store(const float (&v)[SS_STEPS], float* __restrict__ sS, int lane) {
+#pragma unroll
+  for (int u = 0; u < SS_STEPS; ++u) sS[((u + lane) & (SS_STEPS - 1)) * 64 + lane] = v[u];
 }
 
 template <int SA>
@@ -384,8 +403,9 @@ struct PersistCtl {
 // written back the same way, so every global access touches at most two
 // contiguous 256-byte segments.  SKEW: slowness comes from the skewed global
 // copy (one coalesced load per cell per step, prefetched PF steps ahead);
-// otherwise it is staged step-major in LDS from the natural layout (sS[t][kap]
-// = cell (64r+kap, c0+t-kap+1)).  FIRST: the launch is the first sweep of a
+// otherwise it is staged step-major in LDS from the natural layout,
+// SS_STEPS steps at a time (sS[t mod SS_STEPS][kap] = cell
+// (64r+kap, c0+t-kap+1)).  FIRST: the launch is the first sweep of a
 // solve, whose old traveltimes are time_set + time_init (1e6 except the 3x3
 // source stencil) and are generated here instead of read.  PERSIST: the
 // upwind row and the carry-in from block bb-1 arrive through the tagged
@@ -514,11 +534,18 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   const SkewStream<SB, true> st_fast(sd, sv, kap, lane, c0, nxx, NC);
   const SkewStream<SB, false> st_edge(sd, sv, kap, lane, c0, nxx, NC);
   SkewPre pre;
+  // natural-layout slowness: phase 0's cells and lane 63's x-other cells
+  // (lane j: step j's, cell (64r+64, c0-63+j))
+  float sv_[SS_STEPS];
+  float sx63 = 0.0f;
   if (SKEW) {
     if (fast)
       skew_prefetch(pre, st_fast);
     else
       skew_prefetch(pre, st_edge);
+  } else {
+    ss_load<SA, SB>(sv_, sd, nzz, kgc, c0, lane, NC, 0);
+    sx63 = cell_at<SA, SB>(sd, nzz, min(64 * r + 64, NR), min(max(c0 - 63 + lane, 0), NC));
   }
 
   // ---- upwind row above the strip (local row 64r-1; lane t holds column
@@ -564,37 +591,37 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     else
       tile_steps_skew<SB, EQH, FD, false>(st_edge, pre, sT, sHX, lane, row_ok, c0, NC, te, tev, sref, k);
   } else {
-    // Slowness cells step-major: sS[t][kap] = cell (64r+kap, c0+t-kap+1),
-    // kap = 0..64; lane k reads its z-other at [t][k] and x-other at [t][k+1].
-    const int kapc = min(64 * r + lane, NR);
-    float sv_[64];
-#pragma unroll
-    for (int v = 0; v < 64; ++v) {
-      const int lam = min(max(c0 + 1 - lane + ((v + lane - 1) & 63), 0), NC);
-      sv_[v] = cell_at<SA, SB>(sd, nzz, kapc, lam);
-    }
-    const int lam64 = min(max(c0 + 1 - 64 + ((lane + 63) & 63), 0), NC);
-    const float s64 = cell_at<SA, SB>(sd, nzz, min(64 * r + 64, NR), lam64);
+    // Slowness in SS_STEPS-step phases (the next phase's loads in flight
+    // while this one runs); lane k's x-other cell is lane k+1's z-other cell
+    // of the same step (one DPP move), lane 63's comes from the broadcasts.
+    constexpr int NPH = 64 / SS_STEPS;
     CellQ sref = cell_q<EQH, FD>(cell_at<SA, SB>(sd, nzz, kgc, min(max(c0 - lane, 0), NC)), k);
-#pragma unroll
-    for (int v = 0; v < 64; ++v) sS[((v + lane - 1) & 63) * SS_LD + lane] = sv_[v];
-    sS[((lane + 63) & 63) * SS_LD + 64] = s64;
-    wave_lds_sync();
+    ss_store(sv_, sS, lane);
+    sHX[2 * lane + 1] = EQH ? k.dx * sx63 : sx63;
     int tab = 4 * trow(0, lane);
-    float told_n = ring_at(sT, tab), z_n = sS[lane], x_n = sS[lane + 1], h_n = sHX[0];
-    for (int t0 = 0; t0 < 64; t0 += 4) {
+    float told_n = ring_at(sT, tab);
+    float2 hx_n = make_float2(0.0f, 0.0f);
+    for (int ph = 0; ph < NPH; ++ph) {
+      if (ph + 1 < NPH) ss_load<SA, SB>(sv_, sd, nzz, kgc, c0, lane, NC, ph + 1);
+      wave_lds_sync();
+      float z_n = sS[lane];
+      if (ph == 0) hx_n = reinterpret_cast<const float2*>(sHX)[0];
+      for (int t0 = ph * SS_STEPS; t0 < (ph + 1) * SS_STEPS; t0 += 4) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int t = t0 + u;
-        const float told = told_n, h = h_n, szo = z_n, sxo = x_n;
-        const int tn = trow_next_b(tab);
-        told_n = ring_at(sT, tn);
-        h_n = sHX[2 * (t + 1)];
-        z_n = sS[(t + 1) * SS_LD + lane];
-        x_n = sS[(t + 1) * SS_LD + lane + 1];
-        tile_step<EQH, FD, false>(sT, tab, t, lane, row_ok, c0, NC, told, h, szo, sxo, te, tev, sref, k);
-        tab = tn;
+        for (int u = 0; u < 4; ++u) {
+          const int t = t0 + u;
+          const float told = told_n, szo = z_n;
+          const float2 hx = hx_n;
+          const int tn = trow_next_b(tab);
+          told_n = ring_at(sT, tn);
+          hx_n = reinterpret_cast<const float2*>(sHX)[t + 1];
+          z_n = sS[((t + 1) & (SS_STEPS - 1)) * 64 + lane];
+          tile_step<EQH, FD, false, true>(sT, tab, t, lane, row_ok, c0, NC, told, hx.x, szo, hx.y, te, tev, sref,
+                                          k);
+          tab = tn;
+        }
       }
+      if (ph + 1 < NPH) ss_store(sv_, sS, lane);
     }
   }
   wave_lds_sync();
@@ -645,7 +672,7 @@ __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ 
                                                     const int4* __restrict__ tiles, int nzz, EikConsts k,
                                                     SkewView sv) {
   __shared__ float sT[64 * RING_LD];
-  __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
+  __shared__ float sS[SKEW ? 1 : SS_STEPS * 64];
   __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
   const PersistCtl ctl{};
   tile_process<SA, SB, EQH, false, SKEW, FIRST, false, 0>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, sHX,
@@ -661,7 +688,7 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
                                                       const int4* __restrict__ tiles, int nzz, EikConsts k,
                                                       SkewView sv, PersistCtl ctl) {
   __shared__ float sT[64 * RING_LD];
-  __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
+  __shared__ float sS[SKEW ? 1 : SS_STEPS * 64];
   __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
   const unsigned tagb = (*ctl.epoch << 11) | (ctl.sweep_id << 8);
   auto claim_issue = [&]() -> int {
@@ -706,7 +733,7 @@ __global__ __launch_bounds__(64) void k_sweep_pair(const ShotDev* __restrict__ s
                                                    int nzz, EikConsts k, SkewView sv_a, SkewView sv_b,
                                                    PersistCtl ctl_a, PersistCtl ctl_b) {
   __shared__ float sT[64 * RING_LD];
-  __shared__ float sS[SKEW ? 1 : 65 * SS_LD];
+  __shared__ float sS[SKEW ? 1 : SS_STEPS * 64];
   __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
   const int xcd = __builtin_amdgcn_s_getreg((15 << 11) | 20) & 7;  // hwreg(HW_REG_XCC_ID, 0, 16)
   const unsigned tag_a = (*ctl_a.epoch << 11) | (ctl_a.sweep_id << 8);
