@@ -159,6 +159,8 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: the batched steps only, plus a 256 MiB device copy that calibrates "
                          "per-dispatch byte counters")
+    ap.add_argument("--only", choices=["ani", "adjoint"],
+                    help="profiler run of one secondary workload: its steps only, printed as one JSON line")
     args = ap.parse_args()
     if args.profile:
         args.plain_shots, args.cpu_seconds = 0, 0.0
@@ -172,6 +174,10 @@ def main():
     from seisfat2d_amd import SCHED_PLAIN, SCHED_TILED, Eikonal, models
 
     torch.cuda.init()
+    if args.only:
+        run = bench_ani if args.only == "ani" else bench_adjoint
+        print(json.dumps(run(max(1, args.steps))))
+        return
     dev = torch.cuda.get_device_properties(0)
     log(f"device {dev.name} CUs {dev.multi_processor_count}")
     w = models.c3(nshots=args.shots)
