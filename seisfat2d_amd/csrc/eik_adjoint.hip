@@ -102,11 +102,12 @@ __global__ void k_fwd64_init(const AdjShot* __restrict__ shots, int nshots, int 
 // that strip publishes as it computes them, column by column, each double as
 // two 64-bit words (one 32-bit half + the launch's tag in the high half),
 // written and read as single agent-scope atomics: a reader that sees the tag
-// in both words has the value, with no fence on either side.  Lane 0 fetches
-// the words of step s's column ADJ_PF steps ahead and polls again at step s
-// while a tag is stale (bounded: past the limit it sets the flag and stops
-// waiting for the rest of the strip, so a broken dependency cannot stall the
-// device).  Its diagonal input is
+// in both words has the value, with no fence on either side.  The wave
+// fetches the words of a block of ADJ_PF steps in one load (lane u holds step
+// u's), one block ahead, takes step s's from its lane, and, while a tag is
+// stale, polls that column again (bounded: past the limit it sets the flag and
+// stops waiting for the rest of the strip, so a broken dependency cannot stall
+// the device).  Lane 0's diagonal input is
 // its previous step's upwind value (the ring column 0 holds 1e6).  Strips are
 // claimed from a ticket counter strip-major (every shot's strip r before any
 // strip r+1), so a strip's upper neighbour always holds a smaller ticket and
@@ -138,24 +139,24 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
     const int nxx = sh.sd.nxx;
     const double* __restrict__ To = Told + sh.toff;
     double* __restrict__ Tn = Tnew + sh.toff;
-    const int NT = (nxx + 63 + ADJ_PF - 1) / ADJ_PF * ADJ_PF;
+    const int NT = (nxx + 63 + 2 * ADJ_PF - 1) / (2 * ADJ_PF) * (2 * ADJ_PF);  // whole block pairs
     // this strip's published last row, and the upper strip's
     gword* xme = (gword*)(xw + ((size_t)ks * nstr + r) * ldx);
     const gword* xup = (const gword*)(xw + ((size_t)ks * nstr + max(r - 1, 0)) * ldx);
     const bool pub = lane == 63 && r + 1 < nstr;
-    const bool gl = lane == 0 && r > 0;
+    // the upper strip's row (lane 0's upwind values): the loads, the tag test
+    // and the wait are wave-uniform
+    const bool up = r > 0;
     const int fi = 64 * r + lane;
     const bool row_int = fi >= 1 && fi <= nzz - 2;
     const int fic = min(max(fi, 1), nzz - 2);
     const int i = SA > 0 ? fic : nzz - 1 - fic;
     const int i1 = i - (SA + 1) / 2;
     double qto[ADJ_PF];
-    unsigned long long qhi[ADJ_PF], qlo[ADJ_PF];
     float qsr[ADJ_PF], qsz[ADJ_PF], qsx[ADJ_PF];
     // inputs of step t at clamped (always valid) coordinates; only interior
     // nodes use them
-    auto load = [&](int t, double& to, unsigned long long& hi, unsigned long long& lo, float& sr, float& sz,
-                    float& sx) {
+    auto load = [&](int t, double& to, float& sr, float& sz, float& sx) {
       const int fjc = min(max(t - lane, 1), nxx - 2);
       const int j = SB > 0 ? fjc : nxx - 1 - fjc;
       const int j1 = j - (SB + 1) / 2;
@@ -163,38 +164,53 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
       sr = s_at(sh.sd, nzz, i1, j1);
       sz = s_at(sh.sd, nzz, i1, j1 + SB);
       sx = s_at(sh.sd, nzz, i1 + SA, j1);
-      if (gl) {
-        hi = __hip_atomic_load(xup + 2 * fjc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        lo = __hip_atomic_load(xup + 2 * fjc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+    };
+    // the upper strip's words of steps tb..tb+ADJ_PF-1, one block load: lane
+    // u (mod ADJ_PF) holds step tb+u's (column clamp(tb+u, 1, nxx-2)).
+    // Unconditional (strip 0 reads its own words and ignores them), so no
+    // branch forces a wait for it.
+    auto xload = [&](int tb, unsigned long long& h, unsigned long long& l) {
+      const int fjx = min(max(tb + (lane & (ADJ_PF - 1)), 1), nxx - 2);
+      h = __hip_atomic_load(xup + 2 * fjx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      l = __hip_atomic_load(xup + 2 * fjx + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
 #pragma unroll
-    for (int u = 0; u < ADJ_PF; ++u) load(u, qto[u], qhi[u], qlo[u], qsr[u], qsz[u], qsx[u]);
+    for (int u = 0; u < ADJ_PF; ++u) load(u, qto[u], qsr[u], qsz[u], qsx[u]);
     double prev = 1e6, prev2 = 1e6, gv_prev = 1e6;
     bool gave_up = false;
-    for (int t0 = 0; t0 < NT; t0 += ADJ_PF) {
+    // ADJ_PF steps from t0 with the upper strip's words in (xh, xl).  Each
+    // step reloads its queue slot after its last use of the slot (and the
+    // caller each word block after its block), so a slot keeps its register
+    // across the loop and no copy waits for a load in flight.
+    auto block = [&](int t0, unsigned long long xh, unsigned long long xl) {
 #pragma unroll
       for (int u = 0; u < ADJ_PF; ++u) {
         const int t = t0 + u;
-        const double to = qto[u];
-        const float sr = qsr[u], sz = qsz[u], sx = qsx[u];
-        unsigned long long hi = qhi[u], lo = qlo[u];
-        load(min(t + ADJ_PF, NT - 1), qto[u], qhi[u], qlo[u], qsr[u], qsz[u], qsx[u]);
         double gv = 0.0, ge = 0.0;
-        if (gl) {
+        if (up) {
           // upper strip's value of column clamp(t, 1, nxx-2), published at its step t+63
           const int fjc = min(max(t, 1), nxx - 2);
-          for (int it = 0; !gave_up && ((unsigned)(hi >> 32) != tag || (unsigned)(lo >> 32) != tag); ++it) {
+          unsigned hw = __builtin_amdgcn_readlane((unsigned)xh, u);
+          unsigned ht = __builtin_amdgcn_readlane((unsigned)(xh >> 32), u);
+          unsigned lw = __builtin_amdgcn_readlane((unsigned)xl, u);
+          unsigned lt = __builtin_amdgcn_readlane((unsigned)(xl >> 32), u);
+          for (int it = 0; !gave_up && (ht != tag || lt != tag); ++it) {
             if (it > WALK_SPIN) {
-              atomicOr(ctl + 1, 1);
+              if (lane == 0) atomicOr(ctl + 1, 1);
               gave_up = true;
               break;
             }
             __builtin_amdgcn_s_sleep(1);
-            hi = __hip_atomic_load(xup + 2 * fjc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            lo = __hip_atomic_load(xup + 2 * fjc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long hi =
+                __hip_atomic_load(xup + 2 * fjc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long lo =
+                __hip_atomic_load(xup + 2 * fjc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            hw = __builtin_amdgcn_readfirstlane((unsigned)hi);
+            ht = __builtin_amdgcn_readfirstlane((unsigned)(hi >> 32));
+            lw = __builtin_amdgcn_readfirstlane((unsigned)lo);
+            lt = __builtin_amdgcn_readfirstlane((unsigned)(lo >> 32));
           }
-          gv = __hiloint2double((int)(unsigned)hi, (int)(unsigned)lo);
+          gv = __hiloint2double((int)hw, (int)lw);
           ge = t >= 2 ? gv_prev : 1e6;  // column fjc-1: the previous step's, or the ring
           gv_prev = gv;
         }
@@ -203,7 +219,7 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
         const int fj = t - lane;
         double out = 1e6;
         if (row_int && fj >= 1 && fj <= nxx - 2) {
-          out = fmin(to, local_d(tv, prev, tev, sr, sz, sx, K));
+          out = fmin(qto[u], local_d(tv, prev, tev, qsr[u], qsz[u], qsx[u], K));
           const int j = SB > 0 ? fj : nxx - 1 - fj;
           Tn[i + (size_t)j * nzz] = out;
         }
@@ -215,7 +231,24 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
         }
         prev2 = prev;
         prev = out;
+        load(min(t + ADJ_PF, NT - 1), qto[u], qsr[u], qsz[u], qsx[u]);
       }
+    };
+    // word blocks ping-pong between (ah, al) and (bh, bl), each loaded one
+    // block before its use
+    unsigned long long ah, al, bh, bl;
+    xload(0, ah, al);
+    // Drain once per strip: inside the loop every word block and queue slot
+    // was loaded a full block (over 63 memory operations) before its use,
+    // which needs no wait; the compiler's count merges the loop-entry path,
+    // where the first loads are recent, and would otherwise wait there at
+    // every block.
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int t0 = 0; t0 < NT; t0 += 2 * ADJ_PF) {
+      xload(t0 + ADJ_PF, bh, bl);
+      block(t0, ah, al);
+      xload(min(t0 + 2 * ADJ_PF, NT - ADJ_PF), ah, al);
+      block(t0 + ADJ_PF, bh, bl);
     }
   }
 }
