@@ -11,7 +11,7 @@
 //   2. runs the linear recurrence
 //        mu(n) = lam_in(n) + sum over the three nodes m that read n of
 //                mu(m) * d(new_m)/d(n)
-//      in reverse dependency order — k_adj_sweep, one wave per shot;
+//      in reverse dependency order — k_adj_sweep, one wave per 64-row strip;
 //   3. folds mu(n) * d(new_n)/dS into the per-shot slowness gradient — k_adj_fold.
 // The source initialisation T0 = S(node) * distance and the FP64 cross-shot
 // reduction onto the unpadded model close the chain.
@@ -31,9 +31,7 @@ struct AdjShot {
   ShotDev sd;   // final traveltimes (sd.T), slowness view, width, source, window
   size_t toff;  // offset of the shot in the natural-layout arrays (snapshots, gshot)
   size_t koff;  // offset in the strip-skewed arrays
-  size_t xoff;  // offset in the strip-boundary message array
   int NTP;      // steps per strip: nxx + 63 rounded up to ADJ_PF
-  int xrs;      // floats per message slot (multiple of 64)
 };
 
 // Forward sweep s runs rows in direction a and columns in direction b
@@ -77,7 +75,8 @@ void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int max_entrie
                      const double* Tnew, const double* Told, const float* lam_src, float* lam_dst, float4* coef,
                      uint8_t* code, hipStream_t st);
 void launch_adj_sweep(const AdjShot* shots, int nshots, int nstr, const float* lam_in, const float4* coef,
-                      float* lam_out, float* gc, float* xmsg, hipStream_t st);
+                      float* lam_out, float* gc, int* walk_ctl, unsigned long long* walk_x, int ldx, unsigned tag,
+                      hipStream_t st);
 void launch_adj_fold(int sweep, const AdjShot* shots, int nshots, int max_nodes, int nzz, const float* gc,
                      const uint8_t* code, float* gshot, hipStream_t st);
 void launch_adj_source(const AdjShot* shots, int nshots, int nzz, double dx, double dz, const float* lam0,

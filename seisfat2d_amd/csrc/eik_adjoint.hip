@@ -411,51 +411,93 @@ __global__ __launch_bounds__(256) void k_adj_coef(const AdjShot* __restrict__ sh
 }
 
 // ---------------------------------------------------------------------------
-// Reverse sweep: one wave per shot walks the strips of 64 reverse-local rows
-// one after another; lane k holds row 64r + k and visits column t - k at step
-// t.  The z-upwind and diagonal messages come from lane k-1 (DPP shift), the
-// x message from the lane's own previous step, and lane 0 reads the previous
-// strip's lane-63 messages from xmsg (slot r; slot 0 stays zero).  Inputs of
-// ADJ_PF steps ahead are kept in flight.  Nodes outside the grid hold zero
+// Reverse sweep: one wave per strip of 64 reverse-local rows; lane k holds
+// row 64r + k and visits column t - k at step t.  The z-upwind and diagonal
+// messages come from lane k-1 (DPP shift), the x message from the lane's own
+// previous step.  Lane 0's messages are the upper strip's lane-63 ones, which
+// that strip publishes column by column as tagged words (word 2jp = the launch's
+// tag and mv of column jp, word 2jp+1 the same for mev), written at its step
+// jp+63 and read with the forward walker's protocol: one block load per
+// ADJ_PF steps (lane u holds step u's words), one block ahead, and a bounded
+// poll of the column while a tag is stale.  Strip 0 and the columns no strip
+// publishes read zero messages.  Strips are claimed strip-major from the
+// ticket counter ctl[0], so every upper neighbour is already running.  Inputs
+// of ADJ_PF steps ahead are kept in flight; nodes outside the grid hold zero
 // inputs and produce zero outputs.
-__global__ __launch_bounds__(64) void k_adj_sweep(const AdjShot* __restrict__ shots, int nstr,
+__global__ __launch_bounds__(64) void k_adj_sweep(const AdjShot* __restrict__ shots, int nshots, int nstr,
                                                   const float* __restrict__ lam_in,
                                                   const float4* __restrict__ coef, float* __restrict__ lam_out,
-                                                  float* __restrict__ gc, float* xmsg) {
-  const AdjShot sh = shots[blockIdx.x];
+                                                  float* __restrict__ gc, int* __restrict__ ctl,
+                                                  unsigned long long* __restrict__ xw, int ldx, unsigned tag) {
+  typedef __attribute__((address_space(1))) unsigned long long gword;
   const int lane = threadIdx.x;
-  const int NTP = sh.NTP, nxx = sh.sd.nxx;
-  for (int r = 0; r < nstr; ++r) {
+  for (;;) {
+    int tk = 0;
+    if (lane == 0) tk = atomicAdd(ctl, 1);
+    tk = __builtin_amdgcn_readfirstlane(tk);
+    if (tk >= nshots * nstr) break;
+    const int r = tk / nshots, ks = tk - r * nshots;
+    const AdjShot sh = shots[ks];
+    const int NTP = sh.NTP, nxx = sh.sd.nxx;
     const size_t base = sh.koff + (size_t)r * NTP * 64 + lane;
-    float* xin = xmsg + sh.xoff + (size_t)r * sh.xrs;
-    float* xout = xmsg + sh.xoff + (size_t)(r + 1) * sh.xrs;
-    float ql[ADJ_PF], qv[ADJ_PF], qe[ADJ_PF];
+    gword* xme = (gword*)(xw + ((size_t)ks * nstr + r) * ldx);
+    const gword* xup = (const gword*)(xw + ((size_t)ks * nstr + max(r - 1, 0)) * ldx);
+    const bool up = r > 0;
+    const bool pub = lane == 63 && r + 1 < nstr;
+    float ql[ADJ_PF];
     float4 qc[ADJ_PF];
 #pragma unroll
     for (int u = 0; u < ADJ_PF; ++u) {
       ql[u] = lam_in[base + (size_t)u * 64];
       qc[u] = coef[base + (size_t)u * 64];
-      qv[u] = xin[u + 1];
-      qe[u] = xin[NTP + 1 + u];
     }
+    // step t's words: mv of column t (word 2t) and mev of column t-1 (word
+    // 2t-1), at clamped addresses (the step ignores the columns out of range)
+    auto wv = [&](int t) { return xup + 2 * min(t, nxx - 1); };
+    auto we = [&](int t) { return xup + max(2 * min(t, nxx) - 1, 0); };
+    // unconditional (strip 0 reads its own words and ignores them)
+    auto xload = [&](int tb, unsigned long long& v, unsigned long long& e) {
+      const int t = tb + (lane & (ADJ_PF - 1));
+      v = __hip_atomic_load(wv(t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      e = __hip_atomic_load(we(t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
     float mv = 0.0f, me = 0.0f, mev = 0.0f, mevp = 0.0f;
-    for (int t0 = 0; t0 < NTP; t0 += ADJ_PF) {
+    bool gave_up = false;
+    auto block = [&](int t0, unsigned long long bv, unsigned long long be) {
 #pragma unroll
       for (int u = 0; u < ADJ_PF; ++u) {
         const int t = t0 + u;
-        const float lam = ql[u], xv = qv[u], xe = qe[u];
-        const float4 cf = qc[u];
-        const int tn = min(t + ADJ_PF, NTP - 1);
-        ql[u] = lam_in[base + (size_t)tn * 64];
-        qc[u] = coef[base + (size_t)tn * 64];
-        qv[u] = xin[tn + 1];
-        qe[u] = xin[NTP + 1 + tn];
+        float xv = 0.0f, xe = 0.0f;
+        if (up) {
+          const bool nv = t < nxx, ne = t >= 1 && t <= nxx;
+          unsigned vw = __builtin_amdgcn_readlane((unsigned)bv, u);
+          unsigned vt = __builtin_amdgcn_readlane((unsigned)(bv >> 32), u);
+          unsigned ew = __builtin_amdgcn_readlane((unsigned)be, u);
+          unsigned et = __builtin_amdgcn_readlane((unsigned)(be >> 32), u);
+          for (int it = 0; !gave_up && ((nv && vt != tag) || (ne && et != tag)); ++it) {
+            if (it > WALK_SPIN) {
+              if (lane == 0) atomicOr(ctl + 1, 1);
+              gave_up = true;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            const unsigned long long v = __hip_atomic_load(wv(t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long e = __hip_atomic_load(we(t), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            vw = __builtin_amdgcn_readfirstlane((unsigned)v);
+            vt = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+            ew = __builtin_amdgcn_readfirstlane((unsigned)e);
+            et = __builtin_amdgcn_readfirstlane((unsigned)(e >> 32));
+          }
+          xv = nv ? __uint_as_float(vw) : 0.0f;
+          xe = ne ? __uint_as_float(ew) : 0.0f;
+        }
         // messages into (ip, jp): from (ip-1, jp) [lane k-1, step t-1],
         // (ip, jp-1) [own lane, step t-1], (ip-1, jp-1) [lane k-1, step t-2]
         const float in_v = dpp_shr1(xv, mv);
         const float in_ev = dpp_shr1(xe, mevp);
-        const float mu = ((lam + in_v) + me) + in_ev;
+        const float mu = ((ql[u] + in_v) + me) + in_ev;
         mevp = mev;
+        const float4 cf = qc[u];
         const bool upd = cf.w >= 0.0f;
         mv = upd ? mu * cf.x : 0.0f;
         me = upd ? mu * cf.y : 0.0f;
@@ -463,14 +505,28 @@ __global__ __launch_bounds__(64) void k_adj_sweep(const AdjShot* __restrict__ sh
         lam_out[base + (size_t)t * 64] = upd ? 0.0f : mu;
         gc[base + (size_t)t * 64] = upd ? mu * cf.w : 0.0f;
         const int jp = t - 63;
-        if (lane == 63 && jp >= 0 && jp < nxx) {
-          xout[jp + 1] = mv;
-          xout[NTP + 1 + jp + 1] = mev;
+        if (pub && jp >= 0 && jp < nxx) {
+          __hip_atomic_store(xme + 2 * jp, walk_word(__float_as_uint(mv), tag), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(xme + 2 * jp + 1, walk_word(__float_as_uint(mev), tag), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
         }
+        // reload the slot after its last use (it keeps its register)
+        const int tn = min(t + ADJ_PF, NTP - 1);
+        ql[u] = lam_in[base + (size_t)tn * 64];
+        qc[u] = coef[base + (size_t)tn * 64];
       }
+    };
+    unsigned long long av, ae, bv, be;
+    xload(0, av, ae);
+    __builtin_amdgcn_s_waitcnt(0);  // once per strip (see k_fwd64_sweep)
+    for (int t0 = 0; t0 < NTP; t0 += 2 * ADJ_PF) {
+      xload(t0 + ADJ_PF, bv, be);
+      block(t0, av, ae);
+      if (t0 + ADJ_PF >= NTP) break;  // NTP is a multiple of ADJ_PF only
+      xload(t0 + 2 * ADJ_PF, av, ae);
+      block(t0 + ADJ_PF, bv, be);
     }
-    // the next strip's lane 0 reads what lane 63 stored
-    __threadfence_block();
   }
 }
 
@@ -608,8 +664,12 @@ void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int max_entrie
 }
 
 void launch_adj_sweep(const AdjShot* shots, int nshots, int nstr, const float* lam_in, const float4* coef,
-                      float* lam_out, float* gc, float* xmsg, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_sweep, dim3(nshots), dim3(64), 0, st, shots, nstr, lam_in, coef, lam_out, gc, xmsg);
+                      float* lam_out, float* gc, int* walk_ctl, unsigned long long* walk_x, int ldx, unsigned tag,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_walk_begin, dim3(1), dim3(64), 0, st, walk_ctl);
+  const int grid = std::min(nshots * nstr, 4096);
+  hipLaunchKernelGGL(k_adj_sweep, dim3(grid), dim3(64), 0, st, shots, nshots, nstr, lam_in, coef, lam_out, gc,
+                     walk_ctl, walk_x, ldx, tag);
 }
 
 void launch_adj_fold(int sweep, const AdjShot* shots, int nshots, int max_nodes, int nzz, const float* gc,

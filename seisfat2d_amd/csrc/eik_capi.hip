@@ -95,7 +95,7 @@ struct AdjState {
   std::vector<AdjShot> tab;
   AdjShot* d_tab = nullptr;
   int nstr = 0, max_nodes = 0, max_kent = 0;
-  size_t ksize = 0, xsize = 0;
+  size_t ksize = 0;
   double* d_T64 = nullptr;  // fp64 trajectory: T0 (initial field) .. T4, tsize each
   float* d_lamA = nullptr;
   float* d_lamB = nullptr;
@@ -103,15 +103,15 @@ struct AdjState {
   uint8_t* d_code = nullptr;
   float* d_gc = nullptr;
   float* d_gshot = nullptr;
-  float* d_xmsg = nullptr;
   size_t* d_seed_node = nullptr;
   int* d_seed_ptr = nullptr;
   int* d_seed_rec = nullptr;
   float* d_seed_w = nullptr;
   int nseed = 0;
   double* d_picks64 = nullptr;  // picks of the fp64 trajectory
-  // strip-parallel fp64 walk: ticket counter + wait-limit flag, each strip's
-  // published last row (tagged words, walk_ldx per strip), launch tag
+  // strip-parallel fp64 walk and reverse sweeps: ticket counter + wait-limit
+  // flag, each strip's published last row (tagged words, walk_ldx per strip),
+  // launch tag
   int* d_walk_ctl = nullptr;
   unsigned long long* d_walk_x = nullptr;
   int walk_ldx = 0;
@@ -238,7 +238,6 @@ void free_batch(seisfat_batch_s* b) {
   dfree(a.d_code);
   dfree(a.d_gc);
   dfree(a.d_gshot);
-  dfree(a.d_xmsg);
   dfree(a.d_seed_node);
   dfree(a.d_seed_ptr);
   dfree(a.d_seed_rec);
@@ -761,7 +760,7 @@ int adj_prepare(seisfat_batch_s* b) {
   const int nzz = h->nzz, nshots = b->nshots;
   A.nstr = (nzz + 63) / 64;
   A.tab.resize(nshots);
-  size_t koff = 0, xoff = 0;
+  size_t koff = 0;
   A.max_nodes = 0;
   A.max_kent = 0;
   for (int k = 0; k < nshots; ++k) {
@@ -770,16 +769,12 @@ int adj_prepare(seisfat_batch_s* b) {
     s.toff = b->toff[k];
     const int nxx = s.sd.nxx;
     s.NTP = (nxx + 63 + ADJ_PF - 1) / ADJ_PF * ADJ_PF;
-    s.xrs = (2 * (s.NTP + 1) + 63) / 64 * 64;
     s.koff = koff;
-    s.xoff = xoff;
     koff += (size_t)A.nstr * s.NTP * 64;
-    xoff += (size_t)(A.nstr + 1) * s.xrs;
     A.max_nodes = std::max(A.max_nodes, nzz * nxx);
     A.max_kent = std::max(A.max_kent, A.nstr * s.NTP * 64);
   }
   A.ksize = koff;
-  A.xsize = xoff;
   // pick seeds: (skewed node of the last reverse sweep, receiver, weight),
   // grouped by node with receivers in order
   struct Seed {
@@ -833,7 +828,7 @@ int adj_prepare(seisfat_batch_s* b) {
   if ((rc = dalloc(&A.d_tab, (size_t)nshots)) || (rc = dalloc(&A.d_T64, 5 * tsize)) ||
       (rc = dalloc(&A.d_lamA, A.ksize)) || (rc = dalloc(&A.d_lamB, A.ksize)) || (rc = dalloc(&A.d_coef, A.ksize)) ||
       (rc = dalloc(&A.d_code, A.ksize)) || (rc = dalloc(&A.d_gc, A.ksize)) || (rc = dalloc(&A.d_gshot, tsize)) ||
-      (rc = dalloc(&A.d_xmsg, A.xsize)) || (rc = dalloc(&A.d_seed_node, snode.size())) ||
+      (rc = dalloc(&A.d_seed_node, snode.size())) ||
       (rc = dalloc(&A.d_seed_ptr, sptr.size())) || (rc = dalloc(&A.d_seed_rec, srec.size())) ||
       (rc = dalloc(&A.d_seed_w, sw.size())) || (rc = dalloc(&A.d_w, (size_t)b->nrec)) || (rc = dalloc(&A.d_picks64, (size_t)b->nrec)) ||
       (rc = dalloc(&A.d_grad, (size_t)h->nz * h->nx)) || (rc = dalloc(&A.d_walk_ctl, 2)))
@@ -860,7 +855,6 @@ int adj_prepare(seisfat_batch_s* b) {
   HIPCHK(hipMemsetAsync(A.d_coef, 0, A.ksize * sizeof(float4), st));
   HIPCHK(hipMemsetAsync(A.d_code, 0, A.ksize, st));
   HIPCHK(hipMemsetAsync(A.d_gc, 0, A.ksize * sizeof(float), st));
-  HIPCHK(hipMemsetAsync(A.d_xmsg, 0, A.xsize * sizeof(float), st));
   // ring nodes of every trajectory grid, and T0 off the source stencil, stay 1e6
   launch_fill_d(A.d_T64, 5 * tsize, 1e6, st);
   HIPCHK(hipGetLastError());
@@ -907,7 +901,8 @@ int enqueue_vjp(seisfat_batch_s* b, const float* d_w, double* d_grad, hipStream_
   for (int s = 3; s >= 0; --s) {
     launch_adj_coef(s, A.d_tab, ns, A.max_kent, nzz, h->dx, h->dz, A.d_T64 + (s + 1) * ts, A.d_T64 + s * ts,
                     s == 3 ? nullptr : A.d_lamA, A.d_lamB, A.d_coef, A.d_code, st);
-    launch_adj_sweep(A.d_tab, ns, A.nstr, A.d_lamB, A.d_coef, A.d_lamA, A.d_gc, A.d_xmsg, st);
+    launch_adj_sweep(A.d_tab, ns, A.nstr, A.d_lamB, A.d_coef, A.d_lamA, A.d_gc, A.d_walk_ctl, A.d_walk_x,
+                     A.walk_ldx, A.walk_tag = A.walk_tag == 0xffffffffu ? 1 : A.walk_tag + 1, st);
     launch_adj_fold(s, A.d_tab, ns, A.max_nodes, nzz, A.d_gc, A.d_code, A.d_gshot, st);
   }
   launch_adj_source(A.d_tab, ns, nzz, h->dx, h->dz, A.d_lamA, A.d_gshot, st);
