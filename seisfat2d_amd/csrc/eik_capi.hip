@@ -721,7 +721,9 @@ int batch_status(seisfat_batch_s* b) {
   if (b->adj.d_walk_ctl) {
     int err = 0;
     HIPCHK(hipMemcpy(&err, b->adj.d_walk_ctl + 1, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(SEISFAT_EHIP, "fp64 trajectory: a strip's upper neighbour did not arrive within the spin limit");
+    if (err)
+      return fail(SEISFAT_EHIP, "adjoint strip walk (fp64 trajectory or reverse sweep): a strip's upper "
+                                "neighbour did not arrive within the spin limit");
   }
   return SEISFAT_OK;
 }
@@ -849,7 +851,7 @@ int adj_prepare(seisfat_batch_s* b) {
     HIPCHK(hipMemcpyAsync(A.d_seed_w, sw.data(), sw.size() * sizeof(float), hipMemcpyHostToDevice, st));
   }
   // Entries of the skewed arrays outside every grid are never written and
-  // must read as zero; message slot 0 of every shot likewise.
+  // must read as zero.
   HIPCHK(hipMemsetAsync(A.d_lamA, 0, A.ksize * sizeof(float), st));
   HIPCHK(hipMemsetAsync(A.d_lamB, 0, A.ksize * sizeof(float), st));
   HIPCHK(hipMemsetAsync(A.d_coef, 0, A.ksize * sizeof(float4), st));

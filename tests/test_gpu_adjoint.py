@@ -46,7 +46,17 @@ def gpu_obs(e, s_true, b):
 
 @pytest.mark.parametrize("sched", [SCHED_TILED, SCHED_TILED_DIAG, SCHED_PLAIN])
 def test_gradient_matches_oracle_adjoint(oracle, rng, sched):
-    s, dh, shots, rx, rz = window_case(rng)
+    check_gradient_against_oracle(oracle, rng, sched, nz=40)
+
+
+def test_gradient_matches_oracle_adjoint_strips(oracle, rng):
+    """Three 64-row strips per shot (156 padded rows): the FP64 walk and the
+    reverse sweep pass each strip's last row to the next through tagged words."""
+    check_gradient_against_oracle(oracle, rng, SCHED_TILED, nz=150)
+
+
+def check_gradient_against_oracle(oracle, rng, sched, nz):
+    s, dh, shots, rx, rz = window_case(rng, nz=nz)
     nx, nz = s.shape
     e = Eikonal(nz, nx, dh, dh)
     e.set_slowness(s)
