@@ -142,7 +142,9 @@ int seisfat_batch_destroy(seisfat_batch_t b);
  * seisfat_batch_run) followed by an fp64 solve of the same single-pass scheme
  * whose trajectory (initial field and the state after each sweep) the
  * adjoint linearises; asynchronous on `stream`.  The first call allocates the
- * adjoint state (about 20 floats per padded node per shot). */
+ * adjoint state (about 20 floats per padded node per shot).  The calls on one
+ * batch (run, run_record, vjp_device) share its device state: issue them on
+ * one stream, or order them with events; concurrent calls on one batch race. */
 int seisfat_batch_run_record(seisfat_batch_t b, void* stream);
 /* Vector-Jacobian product of the picks of the recorded run:
  *   d_grad[iz + ix*nz] = sum_r d_w[r] * d pick_r / d s[iz + ix*nz]
