@@ -332,6 +332,34 @@ def test_ani_batch_bit_exact(oracle):
     e.close()
 
 
+@pytest.mark.parametrize("nz,nx,dz,dx", [(37, 45, 5.0, 5.0), (70, 131, 10.0, 8.0), (130, 66, 7.0, 10.0)])
+def test_ani_batch_odd_shapes_bit_exact(oracle, nz, nx, dz, dx):
+    """Anisotropic batches on grids narrower than one 64-column block, one
+    strip, partial strips and blocks, unequal spacings and sources down to the
+    bottom row: most of the qP solve's staged-slowness tiles are edge tiles."""
+    s, cs, sxs = ani_setup(nz, nx, dx, 3)
+    szs = [0.0, 0.4 * (nz - 1) * dz, (nz - 1) * dz]
+    S = oracle.expand_boundary(s)
+    Cq, mn, mx = [], [], []
+    for c in cs:
+        u, a, b = oracle.compress(oracle.expand_boundary(c))
+        Cq.append(u)
+        mn.append(a)
+        mx.append(b)
+    Trefs = [oracle.solve_ani(S, dx, dz, float(x), float(z), Cq, mn, mx)[0] for x, z in zip(sxs, szs)]
+    e = Eikonal(nz, nx, dz, dx)
+    e.set_slowness(s)
+    e.set_stiffness(cs)
+    for sched in SCHEDULES:
+        b = e.batch([(float(x), float(z), 0, nx, 0, 0) for x, z in zip(sxs, szs)], type=ANI, schedule=sched)
+        b.run()
+        b.status()
+        for k, Tr in enumerate(Trefs):
+            assert np.array_equal(b.T(k), Tr), (sched, k)
+        b.close()
+    e.close()
+
+
 def test_ani_full_size_c4_shot(oracle):
     w = models.c4(nshots=2)
     e = Eikonal(w.nz, w.nx, w.dz, w.dx)
