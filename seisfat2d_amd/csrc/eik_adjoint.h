@@ -69,11 +69,11 @@ void launch_picks64(const AdjShot* shots, const int* rec_shot, const float* rx, 
                     double dx, double dz, const double* T4, double* out, hipStream_t st);
 // Local partials of sweep s from the fp64 trajectory (Tnew = T after sweep s,
 // Told = before); also moves lam_src (layout of sweep s+1) into lam_dst
-// (layout of sweep s) when lam_src is non-null.  max_entries: the largest
-// strip-skewed layout of a shot (strips x NTP x 64).
-void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int max_entries, int nzz, double dx, double dz,
-                     const double* Tnew, const double* Told, const float* lam_src, float* lam_dst, float4* coef,
-                     uint8_t* code, hipStream_t st);
+// (layout of sweep s) when lam_src is non-null.  nstr: 64-row strips per
+// shot; max_nxx: the widest shot's padded width.
+void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int nstr, int max_nxx, int nzz, double dx,
+                     double dz, const double* Tnew, const double* Told, const float* lam_src, float* lam_dst,
+                     float4* coef, uint8_t* code, hipStream_t st);
 void launch_adj_sweep(const AdjShot* shots, int nshots, int nstr, const float* lam_in, const float4* coef,
                       float* lam_out, float* gc, int* walk_ctl, unsigned long long* walk_x, int ldx, unsigned tag,
                       hipStream_t st);

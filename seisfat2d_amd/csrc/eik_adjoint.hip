@@ -372,42 +372,68 @@ __device__ __forceinline__ AdjCoef adj_local(double tv, double te, double tev, d
   return c;
 }
 
+// One workgroup per (strip rs, block of ADJ_CB reverse-local columns, shot):
+// first each wave takes whole columns of the block in natural order, lane =
+// row, so its loads of T and S are contiguous runs (a wave reading one step
+// of the strip-skewed layout reads a diagonal instead: 64 cache lines per
+// load, which bounded this kernel); the partials go to LDS by column, and
+// then the workgroup writes the skewed entries (lane k, step t) whose node
+// (64rs + k, t - k) lies in the block, each step a contiguous run of lanes.
+// Entries outside the grid are left untouched.
+constexpr int ADJ_CB = 32;
+
 template <int SA, int SB>
-__global__ __launch_bounds__(256) void k_adj_coef(const AdjShot* __restrict__ shots, int nzz, ConstsD k,
+__global__ __launch_bounds__(256) void k_adj_coef(const AdjShot* __restrict__ shots, int nzz, int nstr, ConstsD k,
                                                   const double* __restrict__ Tnew, const double* __restrict__ Told,
                                                   int pa, int pb, const float* __restrict__ lam_src,
                                                   float* __restrict__ lam_dst, float4* __restrict__ coef,
                                                   uint8_t* __restrict__ code) {
-  // one thread per entry of the strip-skewed layout (a wave writes 64
-  // consecutive entries: strip rs, step t, lanes 0..63), the node it holds
-  // read where it lies; entries outside the grid are left untouched
+  __shared__ float P[4][ADJ_CB][64];
+  __shared__ uint8_t Pc[ADJ_CB][64];
   const AdjShot sh = shots[blockIdx.y];
   const int nxx = sh.sd.nxx, NTP = sh.NTP;
-  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = (int)(o & 63);
-  const size_t row = o >> 6;
-  const int rs = (int)(row / NTP), t = (int)(row - (size_t)rs * NTP);
-  const int ip = 64 * rs + lane, jp = t - lane;
-  if (ip >= nzz || jp < 0 || jp >= nxx) return;
-  const int i = adj_ip(ip, SA, nzz), j = adj_jp(jp, SB, nxx);  // (both maps are involutions)
-  const size_t idx = i + (size_t)j * nzz;
-  const size_t ko = sh.koff + o;
-  if (lam_src) lam_dst[ko] = lam_src[sh.koff + adj_kidx(adj_ip(i, pa, nzz), adj_jp(j, pb, nxx), NTP)];
-  AdjCoef c{0.0f, 0.0f, 0.0f, -1.0f, 0};
-  if (i >= 1 && i <= nzz - 2 && j >= 1 && j <= nxx - 2) {
-    const double* __restrict__ Tn = Tnew + sh.toff;
-    const double tv = Tn[(i - SA) + (size_t)j * nzz];
-    const double te = Tn[i + (size_t)(j - SB) * nzz];
-    const double tev = Tn[(i - SA) + (size_t)(j - SB) * nzz];
-    const double told = Told[sh.toff + idx];
-    const int i1 = i - (SA + 1) / 2, j1 = j - (SB + 1) / 2;
-    const double sref = s_at(sh.sd, nzz, i1, j1);
-    const double szo = s_at(sh.sd, nzz, i1, j1 + SB);
-    const double sxo = s_at(sh.sd, nzz, i1 + SA, j1);
-    c = adj_local<SA, SB>(tv, te, tev, sref, szo, sxo, told, k);
+  const int rs = (int)(blockIdx.x % (unsigned)nstr), jp0 = (int)(blockIdx.x / (unsigned)nstr) * ADJ_CB;
+  if (jp0 >= nxx) return;  // whole workgroup: blocks past this shot's width
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ip = 64 * rs + lane;
+  const double* __restrict__ Tn = Tnew + sh.toff;
+  for (int col = w; col < ADJ_CB; col += 4) {
+    const int jp = jp0 + col;
+    AdjCoef c{0.0f, 0.0f, 0.0f, -1.0f, 0};
+    if (ip < nzz && jp < nxx) {
+      const int i = adj_ip(ip, SA, nzz), j = adj_jp(jp, SB, nxx);  // (both maps are involutions)
+      if (i >= 1 && i <= nzz - 2 && j >= 1 && j <= nxx - 2) {
+        const double tv = Tn[(i - SA) + (size_t)j * nzz];
+        const double te = Tn[i + (size_t)(j - SB) * nzz];
+        const double tev = Tn[(i - SA) + (size_t)(j - SB) * nzz];
+        const double told = Told[sh.toff + i + (size_t)j * nzz];
+        const int i1 = i - (SA + 1) / 2, j1 = j - (SB + 1) / 2;
+        const double sref = s_at(sh.sd, nzz, i1, j1);
+        const double szo = s_at(sh.sd, nzz, i1, j1 + SB);
+        const double sxo = s_at(sh.sd, nzz, i1 + SA, j1);
+        c = adj_local<SA, SB>(tv, te, tev, sref, szo, sxo, told, k);
+      }
+    }
+    P[0][col][lane] = c.dtv;
+    P[1][col][lane] = c.dte;
+    P[2][col][lane] = c.dtev;
+    P[3][col][lane] = c.ds;
+    Pc[col][lane] = (uint8_t)c.code;
   }
-  coef[ko] = make_float4(c.dtv, c.dte, c.dtev, c.ds);
-  code[ko] = (uint8_t)c.code;
+  __syncthreads();
+  // step t = jp0 + e/64 of the skewed strip, lane e%64 (= this thread's lane)
+  const size_t sbase = sh.koff + (size_t)rs * NTP * 64;
+  for (int e = threadIdx.x; e < (ADJ_CB + 63) * 64; e += 256) {
+    const int t = jp0 + (e >> 6), jp = t - lane, col = jp - jp0;
+    if (col < 0 || col >= ADJ_CB || jp >= nxx || ip >= nzz) continue;
+    const size_t ko = sbase + (size_t)t * 64 + lane;
+    if (lam_src) {
+      const int i = adj_ip(ip, SA, nzz), j = adj_jp(jp, SB, nxx);
+      lam_dst[ko] = lam_src[sh.koff + adj_kidx(adj_ip(i, pa, nzz), adj_jp(j, pb, nxx), NTP)];
+    }
+    coef[ko] = make_float4(P[0][col][lane], P[1][col][lane], P[2][col][lane], P[3][col][lane]);
+    code[ko] = Pc[col][lane];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -649,17 +675,17 @@ void launch_picks64(const AdjShot* shots, const int* rec_shot, const float* rx, 
                      dz, T4, out);
 }
 
-void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int max_entries, int nzz, double dx, double dz,
-                     const double* Tnew, const double* Told, const float* lam_src, float* lam_dst, float4* coef,
-                     uint8_t* code, hipStream_t st) {
-  const dim3 grid((max_entries + 255) / 256, nshots);
+void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int nstr, int max_nxx, int nzz, double dx,
+                     double dz, const double* Tnew, const double* Told, const float* lam_src, float* lam_dst,
+                     float4* coef, uint8_t* code, hipStream_t st) {
+  const dim3 grid(nstr * ((max_nxx + ADJ_CB - 1) / ADJ_CB), nshots);
   const int pa = adj_a(sweep + 1), pb = adj_b(sweep + 1);
   const ConstsD k = consts_d(dx, dz);
   switch (sweep) {
-    case 0: hipLaunchKernelGGL((k_adj_coef<1, 1>), grid, dim3(256), 0, st, shots, nzz, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
-    case 1: hipLaunchKernelGGL((k_adj_coef<-1, 1>), grid, dim3(256), 0, st, shots, nzz, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
-    case 2: hipLaunchKernelGGL((k_adj_coef<1, -1>), grid, dim3(256), 0, st, shots, nzz, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
-    default: hipLaunchKernelGGL((k_adj_coef<-1, -1>), grid, dim3(256), 0, st, shots, nzz, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
+    case 0: hipLaunchKernelGGL((k_adj_coef<1, 1>), grid, dim3(256), 0, st, shots, nzz, nstr, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
+    case 1: hipLaunchKernelGGL((k_adj_coef<-1, 1>), grid, dim3(256), 0, st, shots, nzz, nstr, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
+    case 2: hipLaunchKernelGGL((k_adj_coef<1, -1>), grid, dim3(256), 0, st, shots, nzz, nstr, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
+    default: hipLaunchKernelGGL((k_adj_coef<-1, -1>), grid, dim3(256), 0, st, shots, nzz, nstr, k, Tnew, Told, pa, pb, lam_src, lam_dst, coef, code); break;
   }
 }
 

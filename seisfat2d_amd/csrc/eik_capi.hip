@@ -94,7 +94,7 @@ struct AdjState {
   unsigned version = 0;
   std::vector<AdjShot> tab;
   AdjShot* d_tab = nullptr;
-  int nstr = 0, max_nodes = 0, max_kent = 0;
+  int nstr = 0, max_nodes = 0, max_nxx = 0;
   size_t ksize = 0;
   double* d_T64 = nullptr;  // fp64 trajectory: T0 (initial field) .. T4, tsize each
   float* d_lamA = nullptr;
@@ -764,7 +764,7 @@ int adj_prepare(seisfat_batch_s* b) {
   A.tab.resize(nshots);
   size_t koff = 0;
   A.max_nodes = 0;
-  A.max_kent = 0;
+  A.max_nxx = 0;
   for (int k = 0; k < nshots; ++k) {
     AdjShot& s = A.tab[k];
     s.sd = b->iso[k];
@@ -774,7 +774,7 @@ int adj_prepare(seisfat_batch_s* b) {
     s.koff = koff;
     koff += (size_t)A.nstr * s.NTP * 64;
     A.max_nodes = std::max(A.max_nodes, nzz * nxx);
-    A.max_kent = std::max(A.max_kent, A.nstr * s.NTP * 64);
+    A.max_nxx = std::max(A.max_nxx, nxx);
   }
   A.ksize = koff;
   // pick seeds: (skewed node of the last reverse sweep, receiver, weight),
@@ -835,9 +835,7 @@ int adj_prepare(seisfat_batch_s* b) {
       (rc = dalloc(&A.d_seed_w, sw.size())) || (rc = dalloc(&A.d_w, (size_t)b->nrec)) || (rc = dalloc(&A.d_picks64, (size_t)b->nrec)) ||
       (rc = dalloc(&A.d_grad, (size_t)h->nz * h->nx)) || (rc = dalloc(&A.d_walk_ctl, 2)))
     return rc;
-  int max_nxx = 0;
-  for (const ShotDev& sd : b->iso) max_nxx = std::max(max_nxx, sd.nxx);
-  A.walk_ldx = 2 * max_nxx;
+  A.walk_ldx = 2 * A.max_nxx;
   if ((rc = dalloc(&A.d_walk_x, (size_t)nshots * A.nstr * A.walk_ldx))) return rc;
   // exchange words start with tag 0, below every launch's tag (>= 1)
   HIPCHK(hipMemsetAsync(A.d_walk_x, 0, (size_t)nshots * A.nstr * A.walk_ldx * sizeof(unsigned long long), h->stream));
@@ -901,8 +899,8 @@ int enqueue_vjp(seisfat_batch_s* b, const float* d_w, double* d_grad, hipStream_
   launch_fill(A.d_gshot, b->tsize, 0.0f, st);
   const size_t ts = b->tsize;
   for (int s = 3; s >= 0; --s) {
-    launch_adj_coef(s, A.d_tab, ns, A.max_kent, nzz, h->dx, h->dz, A.d_T64 + (s + 1) * ts, A.d_T64 + s * ts,
-                    s == 3 ? nullptr : A.d_lamA, A.d_lamB, A.d_coef, A.d_code, st);
+    launch_adj_coef(s, A.d_tab, ns, A.nstr, A.max_nxx, nzz, h->dx, h->dz, A.d_T64 + (s + 1) * ts,
+                    A.d_T64 + s * ts, s == 3 ? nullptr : A.d_lamA, A.d_lamB, A.d_coef, A.d_code, st);
     launch_adj_sweep(A.d_tab, ns, A.nstr, A.d_lamB, A.d_coef, A.d_lamA, A.d_gc, A.d_walk_ctl, A.d_walk_x,
                      A.walk_ldx, A.walk_tag = A.walk_tag == 0xffffffffu ? 1 : A.walk_tag + 1, st);
     launch_adj_fold(s, A.d_tab, ns, A.max_nodes, nzz, A.d_gc, A.d_code, A.d_gshot, st);
