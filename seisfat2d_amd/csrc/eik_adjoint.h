@@ -77,7 +77,7 @@ void launch_adj_coef(int sweep, const AdjShot* shots, int nshots, int nstr, int 
 void launch_adj_sweep(const AdjShot* shots, int nshots, int nstr, const float* lam_in, const float4* coef,
                       float* lam_out, float* gc, int* walk_ctl, unsigned long long* walk_x, int ldx, unsigned tag,
                       hipStream_t st);
-void launch_adj_fold(int sweep, const AdjShot* shots, int nshots, int max_nodes, int nzz, const float* gc,
+void launch_adj_fold(int sweep, const AdjShot* shots, int nshots, int nstr, int max_nxx, int nzz, const float* gc,
                      const uint8_t* code, float* gshot, hipStream_t st);
 void launch_adj_source(const AdjShot* shots, int nshots, int nzz, double dx, double dz, const float* lam0,
                        float* gshot, hipStream_t st);

@@ -558,28 +558,64 @@ __global__ __launch_bounds__(64) void k_adj_sweep(const AdjShot* __restrict__ sh
 
 // ---------------------------------------------------------------------------
 // Slowness contributions of sweep s onto padded cell (i, j): the nodes
-// (i + di, j + dj) whose code marks offset (di, dj), in a fixed order.
+// (i + di, j + dj) whose code marks offset (di, dj), in a fixed order.  One
+// workgroup per (strip rs, block of ADJ_CB reverse-local columns, shot) owns
+// the cells whose nodes lie in the block: it first copies the block's
+// entries, with one row and one column of halo on each side, from the
+// strip-skewed layout into LDS by column (each step a contiguous run of
+// lanes), then sums each cell's contributions from LDS, a wave per column
+// with lane = row, so the cell updates are contiguous.
 template <int SA, int SB>
-__global__ __launch_bounds__(256) void k_adj_fold(const AdjShot* __restrict__ shots, int nzz,
+__global__ __launch_bounds__(256) void k_adj_fold(const AdjShot* __restrict__ shots, int nzz, int nstr,
                                                   const float* __restrict__ gc, const uint8_t* __restrict__ code,
                                                   float* __restrict__ gshot) {
+  constexpr int W = ADJ_CB + 2, H = 66;  // columns jp0-1 .. jp0+ADJ_CB, rows 64rs-1 .. 64rs+64
+  __shared__ float G[W][H];
+  __shared__ uint8_t C[W][H];
   const AdjShot sh = shots[blockIdx.y];
-  const int nxx = sh.sd.nxx;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nzz * nxx) return;
-  const int i = idx % nzz, j = idx / nzz;
-  float acc = 0.0f;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int di = c & 1, dj = c >> 1;
-    const int ni = i + di, nj = j + dj;
-    if (ni >= 1 && ni <= nzz - 2 && nj >= 1 && nj <= nxx - 2) {
-      const size_t ko = sh.koff + adj_kidx(adj_ip(ni, SA, nzz), adj_jp(nj, SB, nxx), sh.NTP);
-      const int m = code[ko];
-      if (m & (1 << c)) acc += (m & (m - 1)) ? 0.5f * gc[ko] : gc[ko];
+  const int nxx = sh.sd.nxx, NTP = sh.NTP;
+  const int rs = (int)(blockIdx.x % (unsigned)nstr), jp0 = (int)(blockIdx.x / (unsigned)nstr) * ADJ_CB;
+  if (jp0 >= nxx) return;  // whole workgroup: blocks past this shot's width
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ip = 64 * rs + lane;
+  // the strip's own rows: step t = jp0 - 1 + e/64, lane e%64
+  const size_t sbase = sh.koff + (size_t)rs * NTP * 64;
+  for (int e = threadIdx.x; e < (W + 63) * 64; e += 256) {
+    const int t = jp0 - 1 + (e >> 6), jp = t - lane;
+    if (jp < max(jp0 - 1, 0) || jp > jp0 + ADJ_CB || jp >= nxx || ip >= nzz) continue;
+    const size_t ko = sbase + (size_t)t * 64 + lane;
+    G[jp - jp0 + 1][lane + 1] = gc[ko];
+    C[jp - jp0 + 1][lane + 1] = code[ko];
+  }
+  // halo rows: the last row of strip rs-1 and the first of strip rs+1
+  if (threadIdx.x < 2 * W) {
+    const int h = threadIdx.x / W, jp = jp0 - 1 + threadIdx.x % W;
+    const int iph = h == 0 ? 64 * rs - 1 : 64 * rs + 64;
+    if (iph >= 0 && iph < nzz && jp >= 0 && jp < nxx) {
+      const size_t ko = sh.koff + adj_kidx(iph, jp, NTP);
+      G[jp - jp0 + 1][h == 0 ? 0 : H - 1] = gc[ko];
+      C[jp - jp0 + 1][h == 0 ? 0 : H - 1] = code[ko];
     }
   }
-  gshot[sh.toff + idx] += acc;
+  __syncthreads();
+  for (int col = w; col < ADJ_CB; col += 4) {
+    const int jp = jp0 + col;
+    if (ip >= nzz || jp >= nxx) continue;
+    const int i = adj_ip(ip, SA, nzz), j = adj_jp(jp, SB, nxx);
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int di = c & 1, dj = c >> 1;
+      const int ni = i + di, nj = j + dj;
+      if (ni >= 1 && ni <= nzz - 2 && nj >= 1 && nj <= nxx - 2) {
+        // node (ni, nj) in reverse-local coordinates, within the LDS window
+        const int r = adj_ip(ni, SA, nzz) - 64 * rs + 1, q = adj_jp(nj, SB, nxx) - jp0 + 1;
+        const int m = C[q][r];
+        if (m & (1 << c)) acc += (m & (m - 1)) ? 0.5f * G[q][r] : G[q][r];
+      }
+    }
+    gshot[sh.toff + i + (size_t)j * nzz] += acc;
+  }
 }
 
 // T0 = S(node) * distance on the source stencil: dT0/dS(node) = distance.
@@ -698,14 +734,14 @@ void launch_adj_sweep(const AdjShot* shots, int nshots, int nstr, const float* l
                      walk_ctl, walk_x, ldx, tag);
 }
 
-void launch_adj_fold(int sweep, const AdjShot* shots, int nshots, int max_nodes, int nzz, const float* gc,
+void launch_adj_fold(int sweep, const AdjShot* shots, int nshots, int nstr, int max_nxx, int nzz, const float* gc,
                      const uint8_t* code, float* gshot, hipStream_t st) {
-  const dim3 grid((max_nodes + 255) / 256, nshots);
+  const dim3 grid(nstr * ((max_nxx + ADJ_CB - 1) / ADJ_CB), nshots);
   switch (sweep) {
-    case 0: hipLaunchKernelGGL((k_adj_fold<1, 1>), grid, dim3(256), 0, st, shots, nzz, gc, code, gshot); break;
-    case 1: hipLaunchKernelGGL((k_adj_fold<-1, 1>), grid, dim3(256), 0, st, shots, nzz, gc, code, gshot); break;
-    case 2: hipLaunchKernelGGL((k_adj_fold<1, -1>), grid, dim3(256), 0, st, shots, nzz, gc, code, gshot); break;
-    default: hipLaunchKernelGGL((k_adj_fold<-1, -1>), grid, dim3(256), 0, st, shots, nzz, gc, code, gshot); break;
+    case 0: hipLaunchKernelGGL((k_adj_fold<1, 1>), grid, dim3(256), 0, st, shots, nzz, nstr, gc, code, gshot); break;
+    case 1: hipLaunchKernelGGL((k_adj_fold<-1, 1>), grid, dim3(256), 0, st, shots, nzz, nstr, gc, code, gshot); break;
+    case 2: hipLaunchKernelGGL((k_adj_fold<1, -1>), grid, dim3(256), 0, st, shots, nzz, nstr, gc, code, gshot); break;
+    default: hipLaunchKernelGGL((k_adj_fold<-1, -1>), grid, dim3(256), 0, st, shots, nzz, nstr, gc, code, gshot); break;
   }
 }
 

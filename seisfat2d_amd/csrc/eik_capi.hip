@@ -94,7 +94,7 @@ struct AdjState {
   unsigned version = 0;
   std::vector<AdjShot> tab;
   AdjShot* d_tab = nullptr;
-  int nstr = 0, max_nodes = 0, max_nxx = 0;
+  int nstr = 0, max_nxx = 0;
   size_t ksize = 0;
   double* d_T64 = nullptr;  // fp64 trajectory: T0 (initial field) .. T4, tsize each
   float* d_lamA = nullptr;
@@ -763,7 +763,6 @@ int adj_prepare(seisfat_batch_s* b) {
   A.nstr = (nzz + 63) / 64;
   A.tab.resize(nshots);
   size_t koff = 0;
-  A.max_nodes = 0;
   A.max_nxx = 0;
   for (int k = 0; k < nshots; ++k) {
     AdjShot& s = A.tab[k];
@@ -773,7 +772,6 @@ int adj_prepare(seisfat_batch_s* b) {
     s.NTP = (nxx + 63 + ADJ_PF - 1) / ADJ_PF * ADJ_PF;
     s.koff = koff;
     koff += (size_t)A.nstr * s.NTP * 64;
-    A.max_nodes = std::max(A.max_nodes, nzz * nxx);
     A.max_nxx = std::max(A.max_nxx, nxx);
   }
   A.ksize = koff;
@@ -903,7 +901,7 @@ int enqueue_vjp(seisfat_batch_s* b, const float* d_w, double* d_grad, hipStream_
                     A.d_T64 + s * ts, s == 3 ? nullptr : A.d_lamA, A.d_lamB, A.d_coef, A.d_code, st);
     launch_adj_sweep(A.d_tab, ns, A.nstr, A.d_lamB, A.d_coef, A.d_lamA, A.d_gc, A.d_walk_ctl, A.d_walk_x,
                      A.walk_ldx, A.walk_tag = A.walk_tag == 0xffffffffu ? 1 : A.walk_tag + 1, st);
-    launch_adj_fold(s, A.d_tab, ns, A.max_nodes, nzz, A.d_gc, A.d_code, A.d_gshot, st);
+    launch_adj_fold(s, A.d_tab, ns, A.nstr, A.max_nxx, nzz, A.d_gc, A.d_code, A.d_gshot, st);
   }
   launch_adj_source(A.d_tab, ns, nzz, h->dx, h->dz, A.d_lamA, A.d_gshot, st);
   launch_adj_reduce(A.d_tab, ns, h->nz, h->nx, nzz, A.d_gshot, d_grad, st);
