@@ -53,6 +53,12 @@ __device__ __forceinline__ double dpp_shr1_d(double old, double src) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// wave_shl:1: lane k receives lane k+1's `src`; lane 63 keeps `old`.
+__device__ __forceinline__ float dpp_shl1f(float old, float src) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), 0x130, 0xf, 0xf,
+                                                    false));
+}
+
 // The local solver (src/modeling/modeling.cu:308-357) in fp64, operation for
 // operation as the fp64 oracle evaluates it.
 __device__ __forceinline__ double local_d(double tv, double te, double tev, double sref, double szo, double sxo,
@@ -151,38 +157,50 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
     const bool row_int = fi >= 1 && fi <= nzz - 2;
     const int fic = min(max(fi, 1), nzz - 2);
     const int i = SA > 0 ? fic : nzz - 1 - fic;
-    const int i1 = i - (SA + 1) / 2;
     double qto[ADJ_PF];
-    float qsr[ADJ_PF], qsz[ADJ_PF], qsx[ADJ_PF];
-    // inputs of step t at clamped (always valid) coordinates; only interior
-    // nodes use them
-    auto load = [&](int t, double& to, float& sr, float& sz, float& sx) {
+    float qsz[ADJ_PF];
+    // Inputs of step t: the old value at clamped (always valid) coordinates
+    // and the z-other cell (i1, j1 + SB), one load per step.  The reference
+    // cell (i1, j1) is the previous step's z-other cell and the x-other cell
+    // (i1 + SA, j1) is lane k+1's z-other cell of the same step (one DPP
+    // move; lane 63's, in the next strip, arrives with the word blocks), so
+    // the z-other cell's row and column are clamped to the cell range only,
+    // keeping those identities for every interior node.
+    const int fis = min(max(fi, 1), nzz - 1);
+    const int is1 = (SA > 0 ? fis : nzz - 1 - fis) - (SA + 1) / 2;
+    auto load = [&](int t, double& to, float& sz) {
       const int fjc = min(max(t - lane, 1), nxx - 2);
       const int j = SB > 0 ? fjc : nxx - 1 - fjc;
-      const int j1 = j - (SB + 1) / 2;
       to = To[i + (size_t)j * nzz];
-      sr = s_at(sh.sd, nzz, i1, j1);
-      sz = s_at(sh.sd, nzz, i1, j1 + SB);
-      sx = s_at(sh.sd, nzz, i1 + SA, j1);
+      const int fjs = min(max(t - lane, 0), nxx - 2);
+      sz = s_at(sh.sd, nzz, is1, (SB > 0 ? fjs : nxx - 1 - fjs) - (SB + 1) / 2 + SB);
     };
     // the upper strip's words of steps tb..tb+ADJ_PF-1, one block load: lane
     // u (mod ADJ_PF) holds step tb+u's (column clamp(tb+u, 1, nxx-2)).
     // Unconditional (strip 0 reads its own words and ignores them), so no
     // branch forces a wait for it.
-    auto xload = [&](int tb, unsigned long long& h, unsigned long long& l) {
-      const int fjx = min(max(tb + (lane & (ADJ_PF - 1)), 1), nxx - 2);
+    // With each block, lane 63's x-other cells: the z-other cells of a
+    // virtual lane 64 (row 64r+64, column t-64).
+    const int f64 = min(64 * r + 64, nzz - 1);
+    const int i64 = (SA > 0 ? f64 : nzz - 1 - f64) - (SA + 1) / 2;
+    auto xload = [&](int tb, unsigned long long& h, unsigned long long& l, float& x63) {
+      const int tt = tb + (lane & (ADJ_PF - 1));
+      const int fjx = min(max(tt, 1), nxx - 2);
       h = __hip_atomic_load(xup + 2 * fjx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       l = __hip_atomic_load(xup + 2 * fjx + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int fjs = min(max(tt - 64, 0), nxx - 2);
+      x63 = s_at(sh.sd, nzz, i64, (SB > 0 ? fjs : nxx - 1 - fjs) - (SB + 1) / 2 + SB);
     };
 #pragma unroll
-    for (int u = 0; u < ADJ_PF; ++u) load(u, qto[u], qsr[u], qsz[u], qsx[u]);
+    for (int u = 0; u < ADJ_PF; ++u) load(u, qto[u], qsz[u]);
     double prev = 1e6, prev2 = 1e6, gv_prev = 1e6;
+    float sref = 0.0f;  // the previous step's z-other cell (unused at step 0)
     bool gave_up = false;
     // ADJ_PF steps from t0 with the upper strip's words in (xh, xl).  Each
     // step reloads its queue slot after its last use of the slot (and the
     // caller each word block after its block), so a slot keeps its register
     // across the loop and no copy waits for a load in flight.
-    auto block = [&](int t0, unsigned long long xh, unsigned long long xl) {
+    auto block = [&](int t0, unsigned long long xh, unsigned long long xl, float x63) {
 #pragma unroll
       for (int u = 0; u < ADJ_PF; ++u) {
         const int t = t0 + u;
@@ -218,8 +236,12 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
         const double tev = dpp_shr1_d(ge, prev2);
         const int fj = t - lane;
         double out = 1e6;
+        const float szo = qsz[u];
+        const float sxo = dpp_shl1f(__uint_as_float(__builtin_amdgcn_readlane(__float_as_uint(x63), u)), szo);
+        const float sr = sref;
+        sref = szo;
         if (row_int && fj >= 1 && fj <= nxx - 2) {
-          out = fmin(qto[u], local_d(tv, prev, tev, qsr[u], qsz[u], qsx[u], K));
+          out = fmin(qto[u], local_d(tv, prev, tev, sr, szo, sxo, K));
           const int j = SB > 0 ? fj : nxx - 1 - fj;
           Tn[i + (size_t)j * nzz] = out;
         }
@@ -231,13 +253,14 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
         }
         prev2 = prev;
         prev = out;
-        load(min(t + ADJ_PF, NT - 1), qto[u], qsr[u], qsz[u], qsx[u]);
+        load(min(t + ADJ_PF, NT - 1), qto[u], qsz[u]);
       }
     };
     // word blocks ping-pong between (ah, al) and (bh, bl), each loaded one
     // block before its use
     unsigned long long ah, al, bh, bl;
-    xload(0, ah, al);
+    float as, bs;
+    xload(0, ah, al, as);
     // Drain once per strip: inside the loop every word block and queue slot
     // was loaded a full block (over 63 memory operations) before its use,
     // which needs no wait; the compiler's count merges the loop-entry path,
@@ -245,10 +268,10 @@ __global__ __launch_bounds__(64) void k_fwd64_sweep(const AdjShot* __restrict__ 
     // every block.
     __builtin_amdgcn_s_waitcnt(0);
     for (int t0 = 0; t0 < NT; t0 += 2 * ADJ_PF) {
-      xload(t0 + ADJ_PF, bh, bl);
-      block(t0, ah, al);
-      xload(min(t0 + 2 * ADJ_PF, NT - ADJ_PF), ah, al);
-      block(t0 + ADJ_PF, bh, bl);
+      xload(t0 + ADJ_PF, bh, bl, bs);
+      block(t0, ah, al, as);
+      xload(min(t0 + 2 * ADJ_PF, NT - ADJ_PF), ah, al, as);
+      block(t0 + ADJ_PF, bh, bl, bs);
     }
   }
 }
