@@ -24,7 +24,7 @@
 
 namespace seisfat {
 
-constexpr int ADJ_PF = 16;  // prefetch distance of the reverse sweep (steps)
+constexpr int ADJ_PF = 8;  // prefetch distance and word-block length of the strip walkers (steps)
 
 // A shot as the adjoint kernels see it.
 struct AdjShot {
