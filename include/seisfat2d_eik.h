@@ -95,7 +95,7 @@ const float* seisfat_eik_device_T(seisfat_eik_t h);
  * ((nz+6)*(nx+6) floats): the D2H copy of Modeling::compute_seismogram
  * (src/modeling/modeling.cu:165) that leaves the public Modeling::T valid. */
 int seisfat_eik_copy_T(seisfat_eik_t h, float* T_padded);
-/* 1 when the batched isotropic sweeps of this geometry divide with the
+/* 1 when the batched tiled sweeps of this geometry divide with the
  * two-instruction known-divisor sequence (its four divisors passed the
  * exhaustive check at create), 0 when they use the five-instruction one; the
  * traveltimes are bit-identical either way.  Diagnostic, no reference

@@ -1128,8 +1128,8 @@ static void pair_dispatch(bool eqh, bool fd, bool skew, const ShotDev* shots_a, 
                           const int4* tiles, const int* qoff, int nzz, const EikConsts& k, const SkewView& sv_a,
                           const SkewView& sv_b, const PersistCtl& ca, const PersistCtl& cb, hipStream_t st) {
 #define SEIS_PAIR(E, F, S) pair_launch<SB, E, F, FIRST_A, S>(shots_a, shots_b, tiles, qoff, nzz, k, sv_a, sv_b, ca, cb, st)
+  // the known-divisor division where it is exact
   if (skew) {
-    // skewed slowness: the known-divisor division where it is exact
     if (eqh) {
       if (fd) SEIS_PAIR(true, true, true);
       else SEIS_PAIR(true, false, true);
@@ -1138,8 +1138,13 @@ static void pair_dispatch(bool eqh, bool fd, bool skew, const ShotDev* shots_a, 
       else SEIS_PAIR(false, false, true);
     }
   } else {
-    if (eqh) SEIS_PAIR(true, false, false);
-    else SEIS_PAIR(false, false, false);
+    if (eqh) {
+      if (fd) SEIS_PAIR(true, true, false);
+      else SEIS_PAIR(true, false, false);
+    } else {
+      if (fd) SEIS_PAIR(false, true, false);
+      else SEIS_PAIR(false, false, false);
+    }
   }
 #undef SEIS_PAIR
 }

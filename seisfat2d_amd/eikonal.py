@@ -75,7 +75,7 @@ class Eikonal:
 
     @property
     def fast_division(self):
-        """True when the batched isotropic sweeps use the two-instruction
+        """True when the batched tiled sweeps use the two-instruction
         known-divisor division (results are bit-identical either way)."""
         rc = lib().seisfat_eik_fast_division(self._h)
         if rc < 0:
