@@ -15,11 +15,20 @@
 //   expand/reduce boundary ...... hand-checked 4x5 fixture, round trip
 //   init ......................... closed form S*dist on the 3x3 stencil
 //   fsm_pass (raster) ............ literal diagonal schedule (fsm_pass_diag)
-//                                  bit-exact; plane-wave exactness; t=r/v for
-//                                  constant velocity; head-wave formula for a
-//                                  two-layer model; FP32 vs FP64; monotone
-//                                  under extra passes; x-mirror symmetry
-//   compress/decode .............. closed-form fixture, exact min/max
+//                                  bit-exact; plane-wave exactness (branch A);
+//                                  exact one-edge plane waves that only
+//                                  branch B / branch C can return, with
+//                                  dx != dz; 1-D edge running sums (cell pair
+//                                  and clamps, all four sweeps); t=r/v for
+//                                  constant velocity, also far-field on grids
+//                                  with dz:dx far from 1 (branch A's guard);
+//                                  head-wave formula for a two-layer model;
+//                                  FP32 vs FP64; monotone under extra passes;
+//                                  x-mirror symmetry
+//   compress/decode .............. closed-form fixture, exact min/max,
+//                                  constant component
+//   (tests/test_oracle_mutants.py plants 27 plausible mistakes in copies of
+//   this file and checks that each one fails at least one pin)
 //   quasi_slowness ............... isotropic stiffness -> S unchanged; VTI
 //                                  qP phase velocity (Thomsen exact formula)
 //   picks (Catmull-Rom) .......... on-node receivers exact; quadratic fields
@@ -253,8 +262,10 @@ void compress(const float* in, uint16_t* out, size_t n, float* maxv, float* minv
     mn = std::min(in[k], mn);
     mx = std::max(in[k], mx);
   }
+  // a constant input would make the reference's expression 0/0 (a NaN cast
+  // to uint16_t); code 1 decodes to mn exactly
   for (size_t k = 0; k < n; ++k)
-    out[k] = static_cast<uint16_t>(1.0f + (COMPRESS - 1) * (in[k] - mn) / (mx - mn));
+    out[k] = (mx == mn) ? uint16_t(1) : static_cast<uint16_t>(1.0f + (COMPRESS - 1) * (in[k] - mn) / (mx - mn));
   *maxv = mx;
   *minv = mn;
 }
@@ -442,6 +453,35 @@ NodeDeriv<R> node_deriv(const R* T, const R* S, int i, int j, int nzz, int nxx, 
   d.scell = (int)cref;
   d.ds = dd_s;
   return d;
+}
+
+// Which candidate decided each node update of one single-pass solve (init +
+// four raster sweeps): counts[kind] over NodeDeriv's kinds (0 old value kept,
+// 1 1-D along z, 2 1-D along x, 3 branch A, 4 branch B, 5 branch C, 6 the
+// 1e6 constant).  Lets tests show that an input exercises branches B and C.
+template <class R>
+void sweep_counted(R* T, const R* S, int nzz, int nxx, int sweep, const Consts<R>& c, long long counts[7]) {
+  int a, b;
+  sweep_signs(sweep, &a, &b);
+  for (int jj = 1; jj < nxx - 1; ++jj) {
+    const int j = (b > 0) ? jj : nxx - 1 - jj;
+    for (int ii = 1; ii < nzz - 1; ++ii) {
+      const int i = (a > 0) ? ii : nzz - 1 - ii;
+      const size_t k = i + (size_t)j * nzz;
+      const NodeDeriv<R> d = node_deriv(T, S, i, j, nzz, nxx, a, b, c, T[k]);
+      ++counts[d.kind];
+      T[k] = rmin(T[k], local_candidate(T, S, i, j, nzz, nxx, a, b, c));
+    }
+  }
+}
+
+template <class R>
+void branch_counts(const R* S, int nzz, int nxx, R dx, R dz, R sx, R sz, long long counts[7]) {
+  const Consts<R> c = make_consts(dx, dz);
+  std::vector<R> T((size_t)nzz * nxx);
+  init_time(T.data(), S, nzz, nxx, dx, dz, sx, sz, nullptr, nullptr);
+  for (int k = 0; k < 7; ++k) counts[k] = 0;
+  for (int s = 0; s < 4; ++s) sweep_counted(T.data(), S, nzz, nxx, s, c, counts);
 }
 
 template <class R>
@@ -765,6 +805,17 @@ double orc_batch_misfit_f64(const double* s, int nz, int nx, double dx, double d
                             const double* rx, const double* rz, const double* obs, double* grad, double* picks,
                             int nthreads) {
   return batch_misfit(s, nz, nx, dx, dz, nshots, sx, sz, x0, w, rec_off, nrec, rx, rz, obs, grad, picks, nthreads);
+}
+
+void orc_branch_counts_f32(const float* S, int nzz, int nxx, float dx, float dz, float sx, float sz,
+                           long long* counts) {
+  branch_counts(S, nzz, nxx, dx, dz, sx, sz, counts);
+}
+// One raster sweep on T in place, adding each node's deciding candidate to
+// counts[7] (kinds as in branch_counts).
+void orc_sweep_counted_f32(float* T, const float* S, int nzz, int nxx, float dx, float dz, int sweep,
+                           long long* counts) {
+  sweep_counted(T, S, nzz, nxx, sweep, make_consts(dx, dz), counts);
 }
 
 int orc_ray_paths(const float* T, int nz, int nx, float dx, float dz, float sx, float sz, const float* rx,

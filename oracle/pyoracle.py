@@ -31,7 +31,9 @@ ip = C.POINTER(C.c_int)
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # SEISFAT_ORACLE_LIB points the checker at another build of oracle.cpp
+        # (the mutation tests load deliberately broken builds this way)
+        path = os.environ.get("SEISFAT_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
         L = C.CDLL(path)
@@ -55,6 +57,8 @@ def lib():
             "orc_picks_f64": (None, [f64p, i, d, d, f64p, f64p, i, f64p]),
             "orc_adjoint_f32": (d, [f32p, i, i, f, f, f, f, f32p, f32p, i, f32p, f64p, f32p]),
             "orc_adjoint_f64": (d, [f64p, i, i, d, d, d, d, f64p, f64p, i, f64p, f64p, f64p]),
+            "orc_branch_counts_f32": (None, [f32p, i, i, f, f, f, f, i64p]),
+            "orc_sweep_counted_f32": (None, [f32p, f32p, i, i, f, f, i, i64p]),
             "orc_ray_paths": (i, [f32p, i, i, f, f, f, f, f32p, f32p, i, i, i32p, i32p, f32p, i]),
             "orc_batch_misfit_f32": (d, [f32p, i, i, f, f, i, f32p, f32p, i32p, i32p, i32p, i32p, f32p, f32p,
                                          f32p, C.c_void_p, C.c_void_p, i]),
@@ -120,6 +124,27 @@ def fsm(T, S, dx, dz, mode=0, max_passes=1, schedule=0):
 def sweep(T, S, dx, dz, s):
     nxx, nzz = S.shape
     lib().orc_sweep_f32(T, S, nzz, nxx, dx, dz, s)
+
+
+KINDS = ("old", "1d_z", "1d_x", "A", "B", "C", "const")
+
+
+def branch_counts(S, dx, dz, sx, sz):
+    """Which candidate decided each node update of one single-pass solve:
+    a dict kind -> count (kinds as KINDS)."""
+    S = np.ascontiguousarray(S, np.float32)
+    nxx, nzz = S.shape
+    out = np.zeros(7, np.int64)
+    lib().orc_branch_counts_f32(S, nzz, nxx, dx, dz, sx, sz, out)
+    return dict(zip(KINDS, out.tolist()))
+
+
+def sweep_counted(T, S, dx, dz, s):
+    """One raster sweep on T in place; returns the deciding-candidate counts."""
+    nxx, nzz = S.shape
+    out = np.zeros(7, np.int64)
+    lib().orc_sweep_counted_f32(T, S, nzz, nxx, dx, dz, s, out)
+    return dict(zip(KINDS, out.tolist()))
 
 
 def solve_iso(S, dx, dz, sx, sz, mode=0, max_passes=1):

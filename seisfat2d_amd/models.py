@@ -172,6 +172,28 @@ def c5(nshots=128, seed=5):
                     extra={"s_true": (1.0 / v_true).astype(np.float32), "shot_spacing_m": spacing})
 
 
+def rough(nz, nx, dz, dx, seed=0, vlo=1500.0, vhi=6000.0, sigma=0.0):
+    """Rough, high-contrast model for parity tests: white-noise velocity
+    U(vlo, vhi) per node (optionally Gaussian-smoothed with `sigma` cells and
+    rescaled to the same range), source at 37 % of the width and 20 % of the
+    depth.  With dz:dx far from 1 the local solver's one-edge branches
+    (B when dz > dx, C when dx > dz) decide many nodes."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(0.0, 1.0, (nx, nz))
+    if sigma:
+        u = gaussian_filter(u, sigma, mode="nearest")
+        u = (u - u.min()) / max(u.max() - u.min(), 1e-30)
+    v = vlo + (vhi - vlo) * u
+    sx = float(np.float32(0.37 * (nx - 1) * dx))
+    sz = float(np.float32(0.2 * (nz - 1) * dz))
+    rx = rng.uniform(0, (nx - 1) * dx, 23).astype(np.float32)
+    rz = rng.uniform(0, (nz - 1) * dz, 23).astype(np.float32)
+    w = Workload(f"rough{nz}x{nx}", nz, nx, dz, dx, (1.0 / v).astype(np.float32),
+                 [(sx, sz, 0, nx, 0, len(rx))], rx, rz)
+    w.sx, w.sz = sx, sz
+    return w
+
+
 def cell_updates(nz, widths):
     """Reference-schedule work of one pass: 4 sweeps x interior padded nodes."""
     widths = np.asarray(widths, np.int64)

@@ -83,6 +83,101 @@ def test_shapes_sources_and_spacings(oracle, rng, nz, nx, dz, dx, sxf, szf):
         assert np.array_equal(pk, pref), (sched, nz, nx)
 
 
+FDIV_FALLBACK_H = float.fromhex("0x1.94f5a6p+2")  # dz whose two-instruction division is inexact
+
+
+@pytest.mark.parametrize("nz,nx,dz,dx,seed,branch", [
+    (64, 96, 3.0, 10.0, 11, "C"),
+    (96, 64, 10.0, 3.0, 12, "B"),
+    (123, 66, 7.0, 10.0, 13, None),
+    (140, 230, FDIV_FALLBACK_H, 2.0, 14, "B"),
+    (70, 150, 2.5, 9.0, 15, "C"),
+])
+@pytest.mark.parametrize("pairs", ["1", "0"])
+def test_rough_unequal_spacing_all_schedules_bit_exact(oracle, monkeypatch, nz, nx, dz, dx, seed, branch, pairs):
+    """Rough, high-contrast models (white-noise velocity 1500-6000 m/s, no
+    smoothing) on grids with dz:dx far from 1, where the local solver's
+    one-edge branches B and C (src/modeling/modeling.cu:344-351) and branch
+    A's causality guard decide many nodes (counted by the oracle): every
+    schedule — per-diagonal launches, tile diagonals, the persistent tiles as
+    fused sweep pairs or one launch per sweep — and a batch of windows,
+    bit-exact against the oracle."""
+    monkeypatch.setenv("SEISFAT_SWEEP_PAIRS", pairs)
+    m = models.rough(nz, nx, dz, dx, seed=seed)
+    S = oracle.expand_boundary(m.s)
+    counts = oracle.branch_counts(S, dx, dz, m.sx, m.sz)
+    print(f"{nz}x{nx} dz={dz} dx={dx}: deciding candidates {counts}")
+    if branch:
+        assert counts[branch] >= 20, counts
+    Tref = oracle.solve_iso(S, dx, dz, m.sx, m.sz)
+    pref = oracle.picks(Tref, dx, dz, m.rx, m.rz)
+    e = Eikonal(nz, nx, dz, dx)
+    e.set_slowness(m.s)
+    for sched in SCHEDULES:
+        T, pk = e.solve(m.sx, m.sz, schedule=sched, rx=m.rx, rz=m.rz)
+        assert np.array_equal(T, Tref), sched
+        assert np.array_equal(pk, pref), sched
+    rng = np.random.default_rng(seed)
+    shots = []
+    for k in range(9):
+        wd = int(rng.integers(1, nx + 1)) if k else nx
+        x0 = int(rng.integers(0, nx - wd + 1))
+        shots.append((float(np.float32((x0 + rng.uniform(0.0, wd - 1)) * dx)),
+                      float(np.float32(rng.uniform(0.0, (nz - 1) * dz))), x0, wd, 0, 0))
+    b = e.batch(shots, schedule=SCHED_TILED)
+    assert b.num_sweep_launches == (2 if pairs == "1" else 4)
+    Trefs = oracle.solve_windows(m.s, dx, dz, [x[0] for x in shots], [x[1] for x in shots],
+                                 [x[2] for x in shots], [x[3] for x in shots], nthreads=16)
+    for run in range(2):
+        b.run()
+        b.status()
+        for k, Tr in enumerate(Trefs):
+            assert np.array_equal(b.T(k), Tr), (run, k)
+    b.close()
+    e.close()
+
+
+def test_rough_anisotropic_unequal_spacing_bit_exact(oracle):
+    """qP solves on a rough TTI model with dz:dx = 1:3.3 (the qP slowness
+    pass feeds one-edge branches too), every schedule, bit-exact."""
+    from seisfat2d_amd.stiffness import compute_stiffness
+    nz, nx, dz, dx = 70, 110, 3.0, 10.0
+    rng = np.random.default_rng(21)
+    vp = rng.uniform(1500.0, 6000.0, (nx, nz))
+    cs = compute_stiffness(vp, vp / np.sqrt(3), 1000.0, rng.uniform(0.0, 0.3, (nx, nz)),
+                           rng.uniform(-0.1, 0.2, (nx, nz)), rng.uniform(-30.0, 30.0, (nx, nz)))
+    s = np.sqrt(1000.0 / cs[3]).astype(np.float32)
+    cs = [c.astype(np.float32) for c in cs]
+    S = oracle.expand_boundary(s)
+    Cq, mn, mx = [], [], []
+    for c in cs:
+        u, a, b_ = oracle.compress(oracle.expand_boundary(c))
+        Cq.append(u)
+        mn.append(a)
+        mx.append(b_)
+    src = [(0.37 * (nx - 1) * dx, 0.2 * (nz - 1) * dz), (0.9 * (nx - 1) * dx, 0.0)]
+    src = [(float(np.float32(x)), float(np.float32(z))) for x, z in src]
+    Trefs, Sq = [], []
+    for x, z in src:
+        T, Sw = oracle.solve_ani(S, dx, dz, x, z, Cq, mn, mx)
+        Trefs.append(T)
+        Sq.append(Sw)
+    c = oracle.branch_counts(Sq[0], dx, dz, src[0][0], src[0][1])
+    print("qP solve deciding candidates", c)
+    assert c["C"] >= 20, c
+    e = Eikonal(nz, nx, dz, dx)
+    e.set_slowness(s)
+    e.set_stiffness(cs)
+    for sched in SCHEDULES:
+        b = e.batch([(x, z, 0, nx, 0, 0) for x, z in src], type=ANI, schedule=sched)
+        b.run()
+        b.status()
+        for k, Tr in enumerate(Trefs):
+            assert np.array_equal(b.T(k), Tr), (sched, k)
+        b.close()
+    e.close()
+
+
 def test_c2_full_size_bit_exact(oracle):
     w = models.c2()
     Tref, out = solve_both(oracle, w.s, w.dx, w.dz, 10000.0, 0.0)
@@ -246,10 +341,9 @@ def test_batch_per_sweep_launches_bit_exact(oracle, rng, monkeypatch):
     e.close()
 
 
-# dz = dx = 0x1.94f5a6p+2: the two-instruction division by dz rounds one
-# dividend per binade away from IEEE division, so this geometry must run the
-# five-instruction kernels
-FDIV_FALLBACK_H = float.fromhex("0x1.94f5a6p+2")
+# FDIV_FALLBACK_H (dz = dx = 0x1.94f5a6p+2): the two-instruction division by
+# dz rounds one dividend per binade away from IEEE division, so this geometry
+# must run the five-instruction kernels
 
 
 @pytest.mark.parametrize("dz,dx,fast", [
@@ -357,6 +451,26 @@ def test_ani_batch_odd_shapes_bit_exact(oracle, nz, nx, dz, dx):
         for k, Tr in enumerate(Trefs):
             assert np.array_equal(b.T(k), Tr), (sched, k)
         b.close()
+    e.close()
+
+
+def test_ani_vti_constant_components_bit_exact(oracle):
+    """A VTI model (no tilt): C15 and C35 are zero everywhere, so their
+    quantisation range is empty; every code is 1 and decodes to 0, and the
+    qP solve matches the oracle bit for bit."""
+    nz, nx, dh = 90, 140, 5.0
+    s, cs, sxs = ani_setup(nz, nx, dh, 2)
+    cs[2] = np.zeros_like(cs[2])
+    cs[4] = np.zeros_like(cs[4])
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    e.set_stiffness(cs)
+    b = e.batch([(float(x), 0.0, 0, nx, 0, 0) for x in sxs], type=ANI, schedule=SCHED_TILED)
+    b.run()
+    b.status()
+    for k, x in enumerate(sxs):
+        assert np.array_equal(b.T(k), oracle_ani(oracle, s, cs, dh, float(x), 0.0)), k
+    b.close()
     e.close()
 
 
