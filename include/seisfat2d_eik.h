@@ -71,6 +71,26 @@ int seisfat_eik_destroy(seisfat_eik_t h);
 int seisfat_eik_set_slowness(seisfat_eik_t h, const float* s);
 int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* stream);
 
+/* Padded slowness, (nz+6)*(nx+6) values uploaded verbatim from host memory:
+ * Modeling::copy_slowness_to_device (src/modeling/modeling.cu:281-284), which
+ * copies the caller's whole padded S.  A shot on the whole model (every
+ * seisfat_eik_solve, and batch shots with x0 = 0, width = nx) reads this
+ * padding as given, as the reference's kernels read d_S; so a tomography loop
+ * that edits only the interior of S (src/inversion/tomography_iso.cpp:77-92)
+ * keeps the starting model's border, as in the reference.  Window shots read
+ * the border rows as uploaded and replicate their own edge columns.  Every
+ * value, padding included, must be positive and finite with slowness*spacing
+ * and slowness/spacing in [1e-12, 1e12].  The adjoint entry points
+ * differentiate with respect to the unpadded model and refuse
+ * (SEISFAT_ESTATE) a padding that is not the replication of the interior. */
+int seisfat_eik_set_slowness_padded(seisfat_eik_t h, const float* S_padded);
+
+/* Device pointer of the padded slowness, (nz+6)*(nx+6) floats z-fastest: the
+ * reference's public Modeling::d_S (src/modeling/modeling.cuh:171).  Read
+ * only: change it through seisfat_eik_set_slowness*.  NULL before a slowness
+ * is set.  Valid until the next slowness upload or destroy. */
+const float* seisfat_eik_device_S(seisfat_eik_t h);
+
 /* Anisotropic stiffness C11, C13, C15, C33, C35, C55 (each nz*nx unpadded, in
  * the reference's units).  Replaces Eikonal_ANI::set_stiffness_element
  * (src/modeling/eikonal_ani.cu:3-16): pad, then uint16 compression with exact
@@ -82,7 +102,10 @@ int seisfat_eik_set_stiffness(seisfat_eik_t h, const float* const c[6]);
  * followed by Modeling::compute_seismogram (src/modeling/modeling.cu:161-194).
  * T_padded (nullable, host) receives the padded (nz+6)*(nx+6) traveltimes;
  * picks (nullable when nrec = 0, host) receive the Catmull-Rom receiver
- * times.  Synchronous. */
+ * times.  Synchronous on the handle's own stream.  The first call (and any
+ * call that changes type, schedule or receiver count) builds a one-shot
+ * batch; later calls reuse it, uploading only the source position and the
+ * receivers (no device allocation, fill or graph capture per shot). */
 int seisfat_eik_solve(seisfat_eik_t h, int type, int schedule, float sx, float sz,
                       float* T_padded, const float* rx, const float* rz, int nrec,
                       float* picks);
@@ -127,8 +150,11 @@ int seisfat_batch_num_sweep_launches(seisfat_batch_t b);
 /* Device time in ms of the sweep phase of the last completed run (from HIP
  * events recorded on the run's stream around the sweep launches). */
 int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms);
-/* Wait for the batch and report a failure the device flagged during its runs
- * (a persistent sweep whose tile dependencies did not arrive in time). */
+/* Wait for the batch's last enqueued work (run, recorded run or
+ * vector-Jacobian product; work of other streams is not waited for) and
+ * report a failure the device flagged during it (a persistent sweep whose
+ * tile dependencies did not arrive in time).  seisfat_batch_picks and
+ * seisfat_batch_copy_T wait the same way. */
 int seisfat_batch_status(seisfat_batch_t b);
 int seisfat_batch_destroy(seisfat_batch_t b);
 

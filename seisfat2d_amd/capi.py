@@ -24,7 +24,8 @@ EXPORTS = [
     "seisfat_batch_device_picks", "seisfat_batch_device_T", "seisfat_batch_copy_T",
     "seisfat_batch_num_launches", "seisfat_batch_num_sweep_launches", "seisfat_batch_sweep_ms",
     "seisfat_batch_status", "seisfat_batch_destroy", "seisfat_batch_run_record", "seisfat_batch_vjp_device",
-    "seisfat_batch_misfit_gradient", "seisfat_eik_fast_division",
+    "seisfat_batch_misfit_gradient", "seisfat_eik_fast_division", "seisfat_eik_set_slowness_padded",
+    "seisfat_eik_device_S",
 ]
 
 
@@ -56,6 +57,9 @@ def lib():
         L.seisfat_eik_destroy.argtypes = [vp]
         L.seisfat_eik_set_slowness.argtypes = [vp, f32p]
         L.seisfat_eik_set_slowness_device.argtypes = [vp, vp, vp]
+        L.seisfat_eik_set_slowness_padded.argtypes = [vp, f32p]
+        L.seisfat_eik_device_S.restype = vp
+        L.seisfat_eik_device_S.argtypes = [vp]
         L.seisfat_eik_set_stiffness.argtypes = [vp, C.POINTER(vp)]
         L.seisfat_eik_solve.argtypes = [vp, i, i, f, f, vp, vp, vp, i, vp]
         L.seisfat_eik_device_T.restype = vp
@@ -79,7 +83,7 @@ def lib():
         L.seisfat_batch_vjp_device.argtypes = [vp, vp, vp, vp]
         L.seisfat_batch_misfit_gradient.argtypes = [vp, vp, vp, vp, vp]
         ptr_ret = {"seisfat_last_error", "seisfat_eik_device_T", "seisfat_batch_device_picks",
-                   "seisfat_batch_device_T"}
+                   "seisfat_batch_device_T", "seisfat_eik_device_S"}
         for name in EXPORTS:
             if name not in ptr_ret:
                 getattr(L, name).restype = C.c_int

@@ -3,9 +3,9 @@
 // Drop-in for the device side of SeisFAT2D's Modeling classes
 // (src/modeling/modeling.cuh:17-79): the reference keeps d_S/d_T per Modeling
 // object and solves one shot per time_propagation(); here a handle owns the
-// row-padded model (and compressed stiffness), and a batch owns the per-shot
-// traveltime grids plus, for the tiled schedule, a HIP graph of the whole
-// solve.
+// padded model (the reference's d_S, plus its skewed copies and compressed
+// stiffness), and a batch owns the per-shot traveltime grids plus, for the
+// tiled schedule, HIP graphs of the whole solve.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +20,7 @@
 
 #include "../../include/seisfat2d_eik.h"
 #include "eik_adjoint.h"
+#include "eik_handle.h"
 #include "eik_kernels.h"
 
 using namespace seisfat;
@@ -62,109 +63,9 @@ void dfree(T*& p) {
 
 }  // namespace
 
-struct seisfat_eik_s {
-  int nz = 0, nx = 0, nzz = 0, nxx = 0;
-  float dz = 0.f, dx = 0.f;
-  EikConsts kc{};
-  bool eqh = false;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  float* d_szp = nullptr;  // row-padded slowness, nzz x nx
-  bool has_s = false;
-  unsigned version = 0;  // incremented by every accepted slowness upload
-  uint16_t* d_c[6] = {};
-  float cmin[6] = {}, cmax[6] = {};
-  float* d_anitab = nullptr;  // 7 planes of nzz*nxx: the qP slowness's shot-independent factors
-  bool has_c = false;
-  int* d_flag = nullptr;
-  // skewed slowness copies, one per sweep orientation (tiled schedule)
-  float* d_sk[4] = {};
-  int KP = 0, NS = 0;
-  SkewView skew[4] = {};
-  // 1: workgroups go round robin over 8 XCCs (fused sweep pairs usable), 0:
-  // not, -1: not checked yet
-  int xcc_ok = -1;
-  seisfat_batch_s* single = nullptr;  // batch of the last seisfat_eik_solve
-};
+int seisfat::set_error(int code, const char* msg) { return fail(code, msg); }
 
-// Adjoint state of a batch, allocated on its first recorded run.
-struct AdjState {
-  bool ready = false;
-  bool recorded = false;  // a recorded run exists for model `version`
-  unsigned version = 0;
-  std::vector<AdjShot> tab;
-  AdjShot* d_tab = nullptr;
-  int nstr = 0, max_nxx = 0;
-  size_t ksize = 0;
-  double* d_T64 = nullptr;  // fp64 trajectory: T0 (initial field) .. T4, tsize each
-  float* d_lamA = nullptr;
-  float* d_lamB = nullptr;
-  float4* d_coef = nullptr;
-  uint8_t* d_code = nullptr;
-  float* d_gc = nullptr;
-  float* d_gshot = nullptr;
-  size_t* d_seed_node = nullptr;
-  int* d_seed_ptr = nullptr;
-  int* d_seed_rec = nullptr;
-  float* d_seed_w = nullptr;
-  int nseed = 0;
-  double* d_picks64 = nullptr;  // picks of the fp64 trajectory
-  // strip-parallel fp64 walk and reverse sweeps: ticket counter + wait-limit
-  // flag, each strip's published last row (tagged words, walk_ldx per strip),
-  // launch tag
-  int* d_walk_ctl = nullptr;
-  unsigned long long* d_walk_x = nullptr;
-  int walk_ldx = 0;
-  unsigned walk_tag = 0;
-  float* d_w = nullptr;      // cotangents of the host-side misfit call
-  double* d_grad = nullptr;  // gradient of the host-side misfit call
-};
 
-struct seisfat_batch_s {
-  seisfat_eik_s* h = nullptr;
-  int type = SEISFAT_ISO, schedule = SEISFAT_SCHED_TILED;
-  int nshots = 0, nrec = 0;
-  std::vector<seisfat_shot> shots;
-  std::vector<ShotDev> iso, qp;  // host copies of the shot tables
-  std::vector<size_t> toff;      // offset of each shot's T grid (floats)
-  size_t tsize = 0;
-  ShotDev* d_iso = nullptr;
-  ShotDev* d_qp = nullptr;
-  float* d_T = nullptr;
-  float* d_swork = nullptr;
-  float* d_rx = nullptr;
-  float* d_rz = nullptr;
-  int* d_recshot = nullptr;
-  float* d_picks = nullptr;
-  int4* d_tiles = nullptr;
-  int ntiles = 0, nstrips = 0;
-  std::vector<int> diag_off;
-  // receivers relative to their shot's grid origin, and their shots
-  std::vector<float> rxr, rzr;
-  std::vector<int> recshot;
-  AdjState adj;
-  // persistent schedule state
-  unsigned long long* d_xch = nullptr;  // exchange words of every shot
-  size_t xch_words = 0;
-  int* d_ctl = nullptr;  // [0..NUM_COUNTERS) ticket counters, [CTL_EPOCH] epoch, [CTL_ERR] error flag
-  // fused sweep pairs (isotropic solve of the tiled schedule): per-XCD queues
-  bool pairs = false;
-  std::vector<ShotDev> iso_b, qp_b;  // iso / qp with the pairs' second-sweep exchange buffers
-  ShotDev* d_iso_b = nullptr;
-  ShotDev* d_qp_b = nullptr;
-  int4* d_ptiles = nullptr;
-  int* d_qoff = nullptr;
-  std::vector<int4> pair_tiles;  // host copies of the queues' contents
-  std::vector<int> pair_qoff;
-  PersistRun pr{};
-  int launches = 0, sweep_launches = 0;
-  hipEvent_t ev_sweep[2] = {nullptr, nullptr};
-  // The tiled solve is recorded as three graphs: initialisation, sweeps (for
-  // anisotropic batches: both solves and the qP slowness between them), and
-  // picks; the timing events are recorded on the stream between them.
-  hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
-  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
-};
 
 namespace {
 
@@ -214,6 +115,7 @@ void free_batch(seisfat_batch_s* b) {
     if (g) (void)hipGraphDestroy(g);
   for (auto& ev : b->ev_sweep)
     if (ev) (void)hipEventDestroy(ev);
+  if (b->ev_done) (void)hipEventDestroy(b->ev_done);
   dfree(b->d_iso);
   dfree(b->d_qp);
   dfree(b->d_T);
@@ -455,21 +357,16 @@ int check_xcc(seisfat_eik_s* h) {
   return SEISFAT_OK;
 }
 
-int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seisfat_shot* shots,
-                 int nrec_total, const float* rx, const float* rz, bool use_graph, seisfat_batch_s** out) {
-  *out = nullptr;
-  if (type != SEISFAT_ISO && type != SEISFAT_ANI) return fail(SEISFAT_EINVAL, "type must be ISO (0) or ANI (1)");
-  if (schedule != SEISFAT_SCHED_PLAIN && schedule != SEISFAT_SCHED_TILED && schedule != SEISFAT_SCHED_TILED_DIAG)
-    return fail(SEISFAT_EINVAL, "schedule must be PLAIN (0), TILED (1) or TILED_DIAG (2)");
-  if (!h->has_s) return fail(SEISFAT_ESTATE, "slowness not set (seisfat_eik_set_slowness)");
-  if (type == SEISFAT_ANI && !h->has_c) return fail(SEISFAT_ESTATE, "stiffness not set (seisfat_eik_set_stiffness)");
-  if (nshots <= 0 || !shots) return fail(SEISFAT_EINVAL, "nshots must be positive");
-  if (type == SEISFAT_ANI && (size_t)h->nzz * h->nxx >= ((size_t)1 << 31))
-    return fail(SEISFAT_EINVAL, "anisotropic batches take models of fewer than 2^31 padded nodes");
+// Checks a batch's shots and receivers against the model; returns, per
+// receiver, its shot and its position relative to the shot's grid origin.
+int validate_shots(const seisfat_eik_s* h, int type, int nshots, const seisfat_shot* shots, int nrec_total,
+                   const float* rx, const float* rz, std::vector<int>& recshot, std::vector<float>& rxr,
+                   std::vector<float>& rzr) {
   if (nrec_total < 0 || (nrec_total > 0 && (!rx || !rz))) return fail(SEISFAT_EINVAL, "bad receiver arrays");
   const float dx = h->dx, dz = h->dz;
-  std::vector<int> recshot(nrec_total, -1);
-  std::vector<float> rxr(nrec_total), rzr(nrec_total);
+  recshot.assign(nrec_total, -1);
+  rxr.assign(nrec_total, 0.0f);
+  rzr.assign(nrec_total, 0.0f);
   for (int k = 0; k < nshots; ++k) {
     const seisfat_shot& s = shots[k];
     const std::string tag = "shot " + std::to_string(k) + ": ";
@@ -495,6 +392,24 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   for (int r = 0; r < nrec_total; ++r)
     if (recshot[r] < 0) return fail(SEISFAT_EINVAL, "receiver " + std::to_string(r) + " belongs to no shot");
 
+  return SEISFAT_OK;
+}
+
+int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seisfat_shot* shots,
+                 int nrec_total, const float* rx, const float* rz, bool use_graph, seisfat_batch_s** out) {
+  *out = nullptr;
+  if (type != SEISFAT_ISO && type != SEISFAT_ANI) return fail(SEISFAT_EINVAL, "type must be ISO (0) or ANI (1)");
+  if (schedule != SEISFAT_SCHED_PLAIN && schedule != SEISFAT_SCHED_TILED && schedule != SEISFAT_SCHED_TILED_DIAG)
+    return fail(SEISFAT_EINVAL, "schedule must be PLAIN (0), TILED (1) or TILED_DIAG (2)");
+  if (!h->has_s) return fail(SEISFAT_ESTATE, "slowness not set (seisfat_eik_set_slowness)");
+  if (type == SEISFAT_ANI && !h->has_c) return fail(SEISFAT_ESTATE, "stiffness not set (seisfat_eik_set_stiffness)");
+  if (nshots <= 0 || !shots) return fail(SEISFAT_EINVAL, "nshots must be positive");
+  if (type == SEISFAT_ANI && (size_t)h->nzz * h->nxx >= ((size_t)1 << 31))
+    return fail(SEISFAT_EINVAL, "anisotropic batches take models of fewer than 2^31 padded nodes");
+  const float dx = h->dx;
+  std::vector<int> recshot;
+  std::vector<float> rxr, rzr;
+  if (int rc = validate_shots(h, type, nshots, shots, nrec_total, rx, rz, recshot, rxr, rzr)) return rc;
   seisfat_batch_s* b = new (std::nothrow) seisfat_batch_s();
   if (!b) return fail(SEISFAT_ENOMEM, "host allocation");
   b->h = h;
@@ -515,9 +430,17 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
     b->toff[k] = off;
     ShotDev& sd = b->iso[k];
     sd.nxx = s.width + 2 * NB;
-    sd.sbase = h->d_szp + (size_t)s.x0 * nzz;
-    sd.sofs = NB;
-    sd.smax = s.width - 1;
+    if (s.x0 == 0 && s.width == h->nx) {
+      // the whole model: its padding as uploaded (the reference's d_S)
+      sd.sbase = h->d_spad;
+      sd.sofs = 0;
+      sd.smax = h->nxx - 1;
+    } else {
+      // a window: its own replicated x border around model columns [x0, x0+width)
+      sd.sbase = h->d_spad + (size_t)(s.x0 + NB) * nzz;
+      sd.sofs = NB;
+      sd.smax = s.width - 1;
+    }
     sd.sx = s.sx - (float)s.x0 * dx;
     sd.sz = s.sz;
     sd.x0 = s.x0;
@@ -547,6 +470,10 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
       free_batch(b);
       return fail(SEISFAT_EHIP, "hipEventCreate");
     }
+  if (hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming) != hipSuccess) {
+    free_batch(b);
+    return fail(SEISFAT_EHIP, "hipEventCreate");
+  }
   TRY(dalloc(&b->d_T, b->tsize));
   for (int k = 0; k < nshots; ++k) b->iso[k].T = b->d_T + b->toff[k];
   if (type == SEISFAT_ANI) {
@@ -709,22 +636,20 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
 #undef HIPCHK_B
 }
 
-// Synchronise with the batch's work and turn a device-side failure flag into
-// an error.
+// Wait for the batch's last enqueued work (only that: other streams' work is
+// not waited for) and turn a device-side failure flag into an error.
 int batch_status(seisfat_batch_s* b) {
-  HIPCHK(hipDeviceSynchronize());
-  if (b->d_ctl) {
-    int err = 0;
-    HIPCHK(hipMemcpy(&err, b->d_ctl + CTL_ERR, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(SEISFAT_EHIP, "persistent sweep: a tile dependency did not arrive within the spin limit");
-  }
-  if (b->adj.d_walk_ctl) {
-    int err = 0;
-    HIPCHK(hipMemcpy(&err, b->adj.d_walk_ctl + 1, sizeof(int), hipMemcpyDeviceToHost));
-    if (err)
-      return fail(SEISFAT_EHIP, "adjoint strip walk (fp64 trajectory or reverse sweep): a strip's upper "
-                                "neighbour did not arrive within the spin limit");
-  }
+  HIPCHK(hipEventSynchronize(b->ev_done));
+  hipStream_t st = b->h->stream;
+  int err[2] = {0, 0};
+  if (b->d_ctl) HIPCHK(hipMemcpyAsync(&err[0], b->d_ctl + CTL_ERR, sizeof(int), hipMemcpyDeviceToHost, st));
+  if (b->adj.d_walk_ctl)
+    HIPCHK(hipMemcpyAsync(&err[1], b->adj.d_walk_ctl + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (err[0]) return fail(SEISFAT_EHIP, "persistent sweep: a tile dependency did not arrive within the spin limit");
+  if (err[1])
+    return fail(SEISFAT_EHIP, "adjoint strip walk (fp64 trajectory or reverse sweep): a strip's upper "
+                              "neighbour did not arrive within the spin limit");
   return SEISFAT_OK;
 }
 
@@ -736,12 +661,15 @@ int run_batch(seisfat_batch_s* b, hipStream_t st) {
     HIPCHK(hipGraphLaunch(b->gexec[1], st));
     HIPCHK(hipEventRecord(b->ev_sweep[1], st));
     HIPCHK(hipGraphLaunch(b->gexec[2], st));
-    return SEISFAT_OK;
+  } else {
+    if (int rc = enqueue_solve(b, st, [&](int i) -> int {
+          HIPCHK(hipEventRecord(b->ev_sweep[i], st));
+          return SEISFAT_OK;
+        }))
+      return rc;
   }
-  return enqueue_solve(b, st, [&](int i) -> int {
-    HIPCHK(hipEventRecord(b->ev_sweep[i], st));
-    return SEISFAT_OK;
-  });
+  HIPCHK(hipEventRecord(b->ev_done, st));
+  return SEISFAT_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -865,6 +793,10 @@ int adj_prepare(seisfat_batch_s* b) {
 // (T0 = initial field, T_{s+1} = sweep s of T_s).
 int run_record(seisfat_batch_s* b, hipStream_t st) {
   if (b->type != SEISFAT_ISO) return fail(SEISFAT_EINVAL, "the adjoint covers isotropic batches");
+  if (!b->h->pad_replicated)
+    return fail(SEISFAT_ESTATE, "the adjoint differentiates with respect to the unpadded model, whose padding "
+                                "replicates its edges; this slowness was uploaded with a different padding "
+                                "(seisfat_eik_set_slowness_padded)");
   if (int rc = adj_prepare(b)) return rc;
   AdjState& A = b->adj;
   A.recorded = false;
@@ -879,6 +811,7 @@ int run_record(seisfat_batch_s* b, hipStream_t st) {
   launch_picks64(A.d_tab, b->d_recshot, b->d_rx, b->d_rz, b->nrec, h->nzz, h->dx, h->dz, A.d_T64 + 4 * ts,
                  A.d_picks64, st);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->ev_done, st));
   A.recorded = true;
   A.version = h->version;
   return SEISFAT_OK;
@@ -906,6 +839,7 @@ int enqueue_vjp(seisfat_batch_s* b, const float* d_w, double* d_grad, hipStream_
   launch_adj_source(A.d_tab, ns, nzz, h->dx, h->dz, A.d_lamA, A.d_gshot, st);
   launch_adj_reduce(A.d_tab, ns, h->nz, h->nx, nzz, A.d_gshot, d_grad, st);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->ev_done, st));
   return SEISFAT_OK;
 }
 
@@ -941,8 +875,8 @@ int seisfat_eik_create(int nz, int nx, float dz, float dx, seisfat_eik_t* out) {
   }
   const int NRS = (h->nzz - 2 + 63) / 64;
   h->KP = 64 * NRS + 64;
-  h->NS = nx + 2 * SK_OFF;
-  int rc = dalloc(&h->d_szp, (size_t)h->nzz * nx);
+  h->NS = h->nxx + 2 * SK_OFF;
+  int rc = dalloc(&h->d_spad, (size_t)h->nzz * h->nxx);
   if (!rc) rc = dalloc(&h->d_flag, 1);
   for (int o = 0; o < 4 && !rc; ++o) {
     rc = dalloc(&h->d_sk[o], (size_t)h->NS * h->KP);
@@ -951,7 +885,7 @@ int seisfat_eik_create(int nz, int nx, float dz, float dx, seisfat_eik_t* out) {
   if (rc) {
     for (auto& p : h->d_sk) dfree(p);
     dfree(h->d_flag);
-    dfree(h->d_szp);
+    dfree(h->d_spad);
     (void)hipStreamDestroy(h->stream);
     delete h;
     return rc;
@@ -994,7 +928,7 @@ int seisfat_eik_destroy(seisfat_eik_t h) {
   if (!h) return SEISFAT_OK;
   (void)hipStreamSynchronize(h->stream);
   free_batch(h->single);
-  dfree(h->d_szp);
+  dfree(h->d_spad);
   dfree(h->d_flag);
   for (auto& p : h->d_sk) dfree(p);
   for (auto& c : h->d_c) dfree(c);
@@ -1011,8 +945,8 @@ int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* str
   ++h->version;  // recorded adjoint runs of the previous model become stale
   HIPCHK(hipMemsetAsync(h->d_flag, 0, sizeof(int), st));
   launch_check_positive(d_s, (size_t)h->nz * h->nx, std::min(h->dx, h->dz), std::max(h->dx, h->dz), h->d_flag, st);
-  launch_zpad(d_s, h->d_szp, h->nz, h->nx, st);
-  for (int o = 0; o < 4; ++o) launch_skew_s(h->d_szp, h->d_sk[o], h->nzz, h->nx, h->KP, h->NS, o, st);
+  launch_expand(d_s, h->d_spad, h->nz, h->nx, st);
+  for (int o = 0; o < 4; ++o) launch_skew_s(h->d_spad, h->d_sk[o], h->nzz, h->nxx, h->KP, h->NS, o, st);
   HIPCHK(hipGetLastError());
   int bad = 0;
   HIPCHK(hipMemcpyAsync(&bad, h->d_flag, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1023,6 +957,7 @@ int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* str
                                 "slowness/spacing in [1e-12, 1e12], everywhere");
   }
   h->has_s = true;
+  h->pad_replicated = true;
   return SEISFAT_OK;
 }
 
@@ -1052,6 +987,52 @@ int seisfat_eik_set_slowness(seisfat_eik_t h, const float* s) {
   return rc;
 }
 
+int seisfat_eik_set_slowness_padded(seisfat_eik_t h, const float* S_padded) {
+  if (int rc = check_handle(h)) return rc;
+  if (!S_padded) return fail(SEISFAT_EINVAL, "null slowness pointer");
+  const int nz = h->nz, nx = h->nx, nzz = h->nzz, nxx = h->nxx;
+  const size_t n = (size_t)nzz * nxx;
+  const float hmin = std::min(h->dx, h->dz), hmax = std::max(h->dx, h->dz);
+  // the padding is read by the solves (the reference's kernels read d_S
+  // whole), so every node must be in the operating range
+  for (size_t k = 0; k < n; ++k)
+    if (!slowness_ok(S_padded[k], hmin, hmax)) {
+      h->has_s = false;
+      return fail(SEISFAT_EINVAL, "slowness must be positive and finite, with slowness*spacing and "
+                                  "slowness/spacing in [1e-12, 1e12] (padded index " + std::to_string(k) + ")");
+    }
+  bool repl = true;
+  for (int j = 0; j < nxx && repl; ++j) {
+    const int jm = std::min(std::max(j, NB), nx + NB - 1);
+    for (int i = 0; i < nzz; ++i) {
+      const int im = std::min(std::max(i, NB), nz + NB - 1);
+      if (S_padded[i + (size_t)j * nzz] != S_padded[im + (size_t)jm * nzz]) {
+        repl = false;
+        break;
+      }
+    }
+  }
+  // runs of this handle's batches still in flight on any stream read the model
+  HIPCHK(hipDeviceSynchronize());
+  ++h->version;  // recorded adjoint runs of the previous model become stale
+  hipStream_t st = h->stream;
+  HIPCHK(hipMemcpyAsync(h->d_spad, S_padded, n * sizeof(float), hipMemcpyHostToDevice, st));
+  for (int o = 0; o < 4; ++o) launch_skew_s(h->d_spad, h->d_sk[o], nzz, nxx, h->KP, h->NS, o, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  h->has_s = true;
+  h->pad_replicated = repl;
+  return SEISFAT_OK;
+}
+
+const float* seisfat_eik_device_S(seisfat_eik_t h) {
+  if (!h || !h->has_s) {
+    g_err = h ? "slowness not set" : "null handle";
+    return nullptr;
+  }
+  return h->d_spad;
+}
+
 int seisfat_eik_set_stiffness(seisfat_eik_t h, const float* const c[6]) {
   if (int rc = check_handle(h)) return rc;
   if (!c) return fail(SEISFAT_EINVAL, "null stiffness array");
@@ -1077,8 +1058,11 @@ int seisfat_eik_set_stiffness(seisfat_eik_t h, const float* const c[6]) {
       mn = std::min(pad[k], mn);
       mx = std::max(pad[k], mx);
     }
+    // A constant component (every VTI model's C15 and C35) would make the
+    // reference's expression 0/0; code 1 decodes to mn + 0 * (mx - mn) = mn,
+    // the value, exactly.
     for (size_t k = 0; k < n; ++k)
-      q[k] = static_cast<uint16_t>(1.0f + (65535 - 1) * (pad[k] - mn) / (mx - mn));
+      q[k] = (mx == mn) ? uint16_t(1) : static_cast<uint16_t>(1.0f + (65535 - 1) * (pad[k] - mn) / (mx - mn));
     h->cmin[e] = mn;
     h->cmax[e] = mx;
     if (!h->d_c[e])
@@ -1096,17 +1080,47 @@ int seisfat_eik_solve(seisfat_eik_t h, int type, int schedule, float sx, float s
                       const float* rx, const float* rz, int nrec, float* picks) {
   if (int rc = check_handle(h)) return rc;
   if (nrec < 0 || (nrec > 0 && !picks)) return fail(SEISFAT_EINVAL, "picks buffer required when nrec > 0");
-  free_batch(h->single);
-  h->single = nullptr;
-  seisfat_shot s{sx, sz, 0, h->nx, 0, nrec};
-  seisfat_batch_s* b = nullptr;
-  int rc = create_batch(h, type, schedule, 1, &s, nrec, rx, rz, /*use_graph=*/false, &b);
-  if (rc) return rc;
-  h->single = b;
+  const seisfat_shot s{sx, sz, 0, h->nx, 0, nrec};
+  seisfat_batch_s* b = h->single;
+  int rc;
+  if (b && b->type == type && b->schedule == schedule && b->nrec == nrec && !b->adj.ready) {
+    // Same shape as the previous call: only the source and the receivers
+    // change.  The kernels (and the graphs that replay them) read both from
+    // device memory, so updating the shot tables and receiver arrays is all
+    // a new shot needs: no allocation, fill or capture per shot.
+    std::vector<int> recshot;
+    std::vector<float> rxr, rzr;
+    if ((rc = validate_shots(h, type, 1, &s, nrec, rx, rz, recshot, rxr, rzr))) return rc;
+    const float sxr = sx;  // the whole model: grid origin = model origin
+    for (auto* tab : {&b->iso, &b->iso_b, &b->qp, &b->qp_b})
+      for (ShotDev& sd : *tab) {
+        sd.sx = sxr;
+        sd.sz = sz;
+      }
+    b->shots[0] = s;
+    b->rxr = rxr;
+    b->rzr = rzr;
+    hipStream_t st = h->stream;
+    auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+      if (dst && bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+      return SEISFAT_OK;
+    };
+    if ((rc = up(b->d_iso, b->iso.data(), sizeof(ShotDev))) || (rc = up(b->d_iso_b, b->iso_b.data(), sizeof(ShotDev))) ||
+        (rc = up(b->d_qp, b->qp.data(), sizeof(ShotDev))) || (rc = up(b->d_qp_b, b->qp_b.data(), sizeof(ShotDev))) ||
+        (rc = up(b->d_rx, b->rxr.data(), nrec * sizeof(float))) || (rc = up(b->d_rz, b->rzr.data(), nrec * sizeof(float))))
+      return rc;
+  } else {
+    free_batch(h->single);
+    h->single = nullptr;
+    if ((rc = create_batch(h, type, schedule, 1, &s, nrec, rx, rz, /*use_graph=*/true, &b))) return rc;
+    h->single = b;
+  }
   if ((rc = run_batch(b, h->stream))) return rc;
   if ((rc = batch_status(b))) return rc;
-  if (T_padded) HIPCHK(hipMemcpy(T_padded, b->d_T, b->tsize * sizeof(float), hipMemcpyDeviceToHost));
-  if (nrec > 0) HIPCHK(hipMemcpy(picks, b->d_picks, nrec * sizeof(float), hipMemcpyDeviceToHost));
+  hipStream_t st = h->stream;
+  if (T_padded) HIPCHK(hipMemcpyAsync(T_padded, b->d_T, b->tsize * sizeof(float), hipMemcpyDeviceToHost, st));
+  if (nrec > 0) HIPCHK(hipMemcpyAsync(picks, b->d_picks, nrec * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return SEISFAT_OK;
 }
 
@@ -1147,7 +1161,10 @@ int seisfat_batch_picks(seisfat_batch_t b, float* picks) {
   if (!b) return fail(SEISFAT_EINVAL, "null batch");
   if (b->nrec > 0 && !picks) return fail(SEISFAT_EINVAL, "null picks buffer");
   if (int rc = batch_status(b)) return rc;
-  if (b->nrec > 0) HIPCHK(hipMemcpy(picks, b->d_picks, b->nrec * sizeof(float), hipMemcpyDeviceToHost));
+  if (b->nrec > 0) {
+    HIPCHK(hipMemcpyAsync(picks, b->d_picks, b->nrec * sizeof(float), hipMemcpyDeviceToHost, b->h->stream));
+    HIPCHK(hipStreamSynchronize(b->h->stream));
+  }
   return SEISFAT_OK;
 }
 
@@ -1167,7 +1184,8 @@ int seisfat_batch_copy_T(seisfat_batch_t b, int shot, float* T_padded) {
   if (!b || shot < 0 || shot >= b->nshots || !T_padded) return fail(SEISFAT_EINVAL, "bad batch, shot or buffer");
   if (int rc = batch_status(b)) return rc;
   const size_t n = (size_t)b->h->nzz * b->iso[shot].nxx;
-  HIPCHK(hipMemcpy(T_padded, b->d_T + b->toff[shot], n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(T_padded, b->d_T + b->toff[shot], n * sizeof(float), hipMemcpyDeviceToHost, b->h->stream));
+  HIPCHK(hipStreamSynchronize(b->h->stream));
   return SEISFAT_OK;
 }
 

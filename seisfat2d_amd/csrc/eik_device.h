@@ -110,9 +110,10 @@ __device__ __forceinline__ float dpp_shr1(float old, float src) {
 // A shot as the kernels see it.  T is the shot's padded traveltime grid
 // (nzz x nxx, column stride nzz).  Slowness of padded node/cell (i, j) is
 //   sbase[i + clamp(j - sofs, 0, smax) * nzz]
-// which covers both a window of the row-padded model (sbase = model column
-// x0, sofs = 3, smax = width-1: the window's own replicated x border) and a
-// fully padded per-shot grid (sofs = 0, smax = nxx-1).
+// which covers a window of the padded model (sbase = padded model column
+// x0 + 3, sofs = 3, smax = width-1: the window's own replicated x border), a
+// shot on the whole model (sbase = the padded model, sofs = 0, smax = nxx-1:
+// the model's padding as uploaded) and a per-shot padded grid (the same).
 struct ShotDev {
   float* T;
   const float* sbase;
@@ -130,9 +131,9 @@ struct ShotDev {
   int nb;
 };
 
-// Skewed copy of the row-padded model slowness for one sweep orientation
+// Skewed copy of the padded model slowness for one sweep orientation
 // (SA, SB).  In sweep-local cell coordinates (kap = cell row along the sweep's
-// z direction, model column m), cell (kap, m) is stored at
+// z direction, padded model column m), cell (kap, m) is stored at
 //   base[(sig + SK_OFF) * KP + kap],  sig = m + (kap & 63)       for SB > 0
 //                                     sig = m - (kap & 63) + 63  for SB < 0
 // so the 64 cells a tile's lanes need at one step share one sig and sit in

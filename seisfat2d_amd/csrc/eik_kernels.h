@@ -27,7 +27,7 @@ void launch_recip(const float* b, float* r, int n, hipStream_t st);
 // bad[d] |= 1 unless fdiv(a, zh[d], zl[d]) == a / b[d] for every dividend of
 // magnitude in [2^-100, 2^100)
 void launch_fastdiv_check(const float* b, const float* zh, const float* zl, int n, int* bad, hipStream_t st);
-void launch_zpad(const float* s, float* szp, int nz, int nx, hipStream_t st);
+void launch_expand(const float* s, float* spad, int nz, int nx, hipStream_t st);
 void launch_check_positive(const float* s, size_t n, float hmin, float hmax, int* bad, hipStream_t st);
 void launch_init(const ShotDev* shots, int nshots, int nzz, float dx, float dz, hipStream_t st);
 // Returns the number of kernel launches issued.
@@ -63,9 +63,11 @@ int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, i
                           const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused,
                           const PersistRun& pr, int solve, hipStream_t st);
 void launch_begin_run(const PersistRun& pr, hipStream_t st);
-void launch_skew_s(const float* szp, float* sk, int nzz, int nx, int KP, int NS, int sweep, hipStream_t st);
+void launch_skew_s(const float* spad, float* sk, int nzz, int nxx, int KP, int NS, int sweep, hipStream_t st);
+// rec_shot null: all receivers belong to shot 0; err (nullable) gets bit 1
+// for a receiver outside its shot's grid (picked as NaN)
 void launch_picks(const ShotDev* shots, const int* rec_shot, const float* rx, const float* rz, float* out,
-                  int nrec, int nzz, float dx, float dz, hipStream_t st);
+                  int nrec, int nzz, float dx, float dz, hipStream_t st, int* err = nullptr);
 void launch_ani_table(const ShotDev& model, int nzz, int nxx, const StiffDev& stf, const AniTab& tb,
                       hipStream_t st);
 void launch_quasi_slowness(const ShotDev* iso, ShotDev* qp, int nshots, int nzz, int nxx, float dx, float dz,

@@ -9,7 +9,7 @@
 //   k_picks           compute_seismogram's Catmull-Rom picks
 //                     (src/modeling/modeling.cu:161-194, src/admin/admin.cpp:121-134)
 //   k_quasi_slowness  get_quasi_slowness   (src/modeling/eikonal_ani.cu:47-111)
-//   k_zpad            the row part of expand_boundary (src/modeling/modeling.cu:204-232)
+//   k_expand          expand_boundary      (src/modeling/modeling.cu:204-232)
 #include "eik_kernels.h"
 
 #include <type_traits>
@@ -23,18 +23,20 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
-// Row-pad an unpadded nz x nx model into nzz x nx (rows 0..2 copy model row 0,
-// the last three copy row nz-1).  Columns are padded virtually by ShotDev.
-__global__ void k_zpad(const float* __restrict__ s, float* __restrict__ szp, int nz, int nx) {
+// expand_boundary of an unpadded nz x nx model into the padded nzz x nxx grid:
+// every padded node copies the nearest model node (edge rows and columns
+// replicated, src/modeling/modeling.cu:204-232).
+__global__ void k_expand(const float* __restrict__ s, float* __restrict__ spad, int nz, int nx) {
   const int nzz = nz + 2 * NB;
-  const size_t n = (size_t)nzz * nx;
+  const size_t n = (size_t)nzz * (nx + 2 * NB);
   size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; idx < n; idx += stride) {
     const int i = (int)(idx % nzz);
     const int j = (int)(idx / nzz);
     const int im = min(max(i - NB, 0), nz - 1);
-    szp[idx] = s[im + (size_t)j * nz];
+    const int jm = min(max(j - NB, 0), nx - 1);
+    spad[idx] = s[im + (size_t)jm * nz];
   }
 }
 
@@ -208,7 +210,7 @@ template <int SB>
 __device__ __forceinline__ int skew_addr(const ShotDev& sd, int nxx, int NC, int KP, int kap, int lam) {
   lam = min(max(lam, 0), NC);
   const int j = (SB > 0) ? lam : nxx - 2 - lam;
-  const int m = sd.x0 + min(max(j - NB, 0), sd.w - 1);
+  const int m = sd.x0 + sd.sofs + min(max(j - sd.sofs, 0), sd.smax);  // padded model column
   const int sig = (SB > 0) ? m + (kap & 63) : m - (kap & 63) + 63;
   return (sig + SK_OFF) * KP + kap;
 }
@@ -262,7 +264,7 @@ struct SkewStream {
   __device__ __forceinline__ SkewStream(const ShotDev& sd_, const SkewView& sv_, int kap_, int lane_, int c0_,
                                         int nxx_, int NC_)
       : sd(sd_), sv(sv_), kap(kap_), lane(lane_), c0(c0_), nxx(nxx_), NC(NC_) {
-    const int sig0 = (SB > 0) ? sd.x0 + c0 - 2 : sd.x0 + sd.w + 63 - c0;
+    const int sig0 = (SB > 0) ? sd.x0 + c0 + 1 : sd.x0 + sd.w + 66 - c0;
     colbase0 = (sig0 + SK_OFF) * sv.KP;
     rs = __builtin_amdgcn_make_buffer_rsrc((void*)sv.base, 0, 0x7fffffff, 0x00020000);
   }
@@ -793,12 +795,16 @@ __device__ __forceinline__ float cubic1d(const float P[4], float d) {
                           d * (3.0f * (P[1] - P[2]) + P[3] - P[0])));
 }
 
+// rec_shot null: every receiver belongs to shot 0.  A receiver whose 4x4
+// stencil leaves its shot's grid (possible only for receiver arrays the host
+// has not checked) gets NaN and sets *err (when err is not null) to 2.
 __global__ void k_picks(const ShotDev* __restrict__ shots, const int* __restrict__ rec_shot,
                         const float* __restrict__ rx, const float* __restrict__ rz,
-                        float* __restrict__ out, int nrec, int nzz, float dx, float dz) {
+                        float* __restrict__ out, int nrec, int nzz, float dx, float dz, int* __restrict__ err) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrec) return;
-  const gfloat* __restrict__ T = gptr(shots[rec_shot[r]].T);
+  const ShotDev& sd = shots[rec_shot ? rec_shot[r] : 0];
+  const gfloat* __restrict__ T = gptr(sd.T);
   const float x = rx[r], z = rz[r];
   const float x0 = floorf((x + 0.5f * dx) / dx) * dx;
   const float z0 = floorf((z + 0.5f * dz) / dz) * dz;
@@ -806,8 +812,16 @@ __global__ void k_picks(const ShotDev* __restrict__ shots, const int* __restrict
   const float z1 = floorf((z + 0.5f * dz) / dz) * dz + dz;
   const float xd = (x - x0) / (x1 - x0);
   const float zd = (z - z0) / (z1 - z0);
-  const int i = (int)((z + 0.5f * dz) / dz) + NB;
-  const int j = (int)((x + 0.5f * dx) / dx) + NB;
+  // position tests before the int conversions (a NaN or huge float has no int)
+  const float fi = (z + 0.5f * dz) / dz, fj = (x + 0.5f * dx) / dx;
+  if (!(fi >= 0.0f && fi < (float)(nzz - 2 * NB - 1) + 1.0f && fj >= 0.0f &&
+        fj < (float)(sd.nxx - 2 * NB - 1) + 1.0f)) {
+    out[r] = __builtin_nanf("");
+    if (err) atomicOr(err, 2);
+    return;
+  }
+  const int i = (int)fi + NB;
+  const int j = (int)fj + NB;
   float p[4];
   for (int pIdx = 0; pIdx < 4; ++pIdx) {
     float P[4];
@@ -950,10 +964,10 @@ void launch_fill(float* p, size_t n, float v, hipStream_t st) {
   hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, st, p, n, v);
 }
 
-void launch_zpad(const float* s, float* szp, int nz, int nx, hipStream_t st) {
-  const size_t n = (size_t)(nz + 2 * NB) * nx;
+void launch_expand(const float* s, float* spad, int nz, int nx, hipStream_t st) {
+  const size_t n = (size_t)(nz + 2 * NB) * (nx + 2 * NB);
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_zpad, dim3(blocks), dim3(256), 0, st, s, szp, nz, nx);
+  hipLaunchKernelGGL(k_expand, dim3(blocks), dim3(256), 0, st, s, spad, nz, nx);
 }
 
 void launch_check_positive(const float* s, size_t n, float hmin, float hmax, int* bad, hipStream_t st) {
@@ -1184,8 +1198,9 @@ void launch_begin_run(const PersistRun& pr, hipStream_t st) {
   hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(64), 0, st, pr.counters, NUM_COUNTERS, pr.epoch);
 }
 
-// Skewed slowness copy for sweep orientation (SA, SB): one thread per entry.
-__global__ void k_skew_s(const float* __restrict__ szp, float* __restrict__ sk, int nzz, int nx, int KP,
+// Skewed slowness copy for sweep orientation (SA, SB) of the padded model
+// (nzz x nxx): one thread per entry.
+__global__ void k_skew_s(const float* __restrict__ spad, float* __restrict__ sk, int nzz, int nxx, int KP,
                          int NS, int SA, int SB) {
   const size_t n = (size_t)NS * KP;
   size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -1195,24 +1210,25 @@ __global__ void k_skew_s(const float* __restrict__ szp, float* __restrict__ sk, 
     const int sig = (int)(idx / KP) - SK_OFF;
     const int m = (SB > 0) ? sig - (kap & 63) : sig + (kap & 63) - 63;
     const int i = min(max((SA > 0) ? kap : nzz - 2 - kap, 0), nzz - 1);
-    sk[idx] = szp[i + (size_t)min(max(m, 0), nx - 1) * nzz];
+    sk[idx] = spad[i + (size_t)min(max(m, 0), nxx - 1) * nzz];
   }
 }
 
-void launch_skew_s(const float* szp, float* sk, int nzz, int nx, int KP, int NS, int sweep, hipStream_t st) {
+void launch_skew_s(const float* spad, float* sk, int nzz, int nxx, int KP, int NS, int sweep, hipStream_t st) {
   const int SA = (sweep == 0 || sweep == 2) ? 1 : -1;
   const int SB = (sweep == 0 || sweep == 1) ? 1 : -1;
   const size_t n = (size_t)NS * KP;
   const int blocks = (int)std::min<size_t>((n + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_skew_s, dim3(blocks), dim3(256), 0, st, szp, sk, nzz, nx, KP, NS, SA, SB);
+  hipLaunchKernelGGL(k_skew_s, dim3(blocks), dim3(256), 0, st, spad, sk, nzz, nxx, KP, NS, SA, SB);
 }
 
 void launch_picks(const ShotDev* shots, const int* rec_shot, const float* rx, const float* rz, float* out,
-                  int nrec, int nzz, float dx, float dz, hipStream_t st) {
+                  int nrec, int nzz, float dx, float dz, hipStream_t st, int* err) {
   if (nrec <= 0) return;
   hipLaunchKernelGGL(k_picks, dim3((nrec + 255) / 256), dim3(256), 0, st, shots, rec_shot, rx, rz, out, nrec,
-                     nzz, dx, dz);
+                     nzz, dx, dz, err);
 }
+
 
 void launch_ani_table(const ShotDev& model, int nzz, int nxx, const StiffDev& stf, const AniTab& tb,
                       hipStream_t st) {

@@ -48,6 +48,19 @@ class Eikonal:
         else:
             check(lib().seisfat_eik_set_slowness(self._h, self._model(s)))
 
+    def set_slowness_padded(self, S):
+        """S: padded (nx+6, nz+6) slowness uploaded verbatim, border included
+        (Modeling::copy_slowness_to_device); whole-model solves read its
+        border as given."""
+        S = np.ascontiguousarray(S, np.float32)
+        if S.shape != (self.nxx, self.nzz):
+            raise ValueError(f"padded model must have shape (nx+6, nz+6) = {(self.nxx, self.nzz)}, got {S.shape}")
+        check(lib().seisfat_eik_set_slowness_padded(self._h, S))
+
+    def device_S_ptr(self):
+        """Device address of the padded slowness (the reference's d_S)."""
+        return lib().seisfat_eik_device_S(self._h)
+
     def set_stiffness(self, cs):
         """cs: six (nx, nz) arrays C11, C13, C15, C33, C35, C55."""
         if len(cs) != 6:
@@ -116,6 +129,7 @@ class Batch:
         check(lib().seisfat_batch_create(eik._h, type, schedule, n, arr, self.nrec,
                                          capi._ptr(self.rx), capi._ptr(self.rz), C.byref(b)))
         self._b = b
+        self.records = 0  # recorded runs so far; the adjoint linearises the latest one
 
     def run(self, stream=None):
         check(lib().seisfat_batch_run(self._b, None if stream is None else C.c_void_p(stream)))
@@ -162,6 +176,7 @@ class Batch:
     def run_record(self, stream=None):
         """Forward solve that keeps what the adjoint needs (same picks as run)."""
         check(lib().seisfat_batch_run_record(self._b, None if stream is None else C.c_void_p(stream)))
+        self.records += 1
 
     def vjp_device(self, w_ptr, grad_ptr, stream=None):
         """grad (nz*nx float64, device) = sum_r w[r] dpick_r/ds for the recorded run."""
@@ -178,6 +193,7 @@ class Batch:
         pk = np.empty(self.nrec, np.float32)
         check(lib().seisfat_batch_misfit_gradient(self._b, capi._ptr(obs), C.byref(J), capi._ptr(g),
                                                   capi._ptr(pk)))
+        self.records += 1  # it records a run of its own
         return J.value, g, pk
 
     def close(self):

@@ -52,7 +52,7 @@ void Modeling::set_parameters() {
   if (h_) seisfat_eik_destroy(h_);
   h_ = nullptr;
   check_status(seisfat_eik_create(nz, nx, dz, dx, &h_), "seisfat_eik_create");
-  check_status(seisfat_eik_set_slowness(h_, s.data()), "seisfat_eik_set_slowness");
+  copy_slowness_to_device();
   set_conditions();
 }
 
@@ -95,9 +95,8 @@ void Modeling::export_seismogram() {
 }
 
 void Modeling::copy_slowness_to_device() {
-  std::vector<float> s((size_t)nPoints);
-  reduce_boundary(S, s.data());
-  check_status(seisfat_eik_set_slowness(h_, s.data()), "seisfat_eik_set_slowness");
+  check_status(seisfat_eik_set_slowness_padded(h_, S), "seisfat_eik_set_slowness_padded");
+  d_S = seisfat_eik_device_S(h_);
 }
 
 void Modeling::expand_boundary(const float* input, float* output) const {

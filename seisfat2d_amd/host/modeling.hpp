@@ -37,6 +37,9 @@ class Modeling {
   float* T = nullptr;
   // Padded traveltimes of the last solve on the device (read by table builders).
   const float* d_T = nullptr;
+  // Padded slowness on the device (src/modeling/modeling.cuh:171), as last
+  // pushed by set_parameters() / copy_slowness_to_device(); read only.
+  const float* d_S = nullptr;
   // Receiver picks of the last solve (geometry->nrec values).
   float* seismogram = nullptr;
 
@@ -71,7 +74,8 @@ class Modeling {
   // <modeling_output_folder><modeling_type>_nStations<nrec>_shot_<srcId+1>.bin
   // (src/modeling/modeling.cu:196-202).
   void export_seismogram();
-  // Pushes the interior of S to the device (src/modeling/modeling.cu:281-284).
+  // Pushes the whole padded S, border included, to the device
+  // (src/modeling/modeling.cu:281-284).
   void copy_slowness_to_device();
 
   void expand_boundary(const float* input, float* output) const;

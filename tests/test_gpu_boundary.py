@@ -1,0 +1,161 @@
+"""The drop-in boundary on the GPU: the padded slowness upload of
+Modeling::copy_slowness_to_device (src/modeling/modeling.cu:281-284) with the
+tomography update of Tomography_ISO::model_update (src/inversion/
+tomography_iso.cpp:77-92), the public d_S, and the reused one-shot batch of
+the per-shot solve.  Everything bit-exact against the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy.ndimage import gaussian_filter
+
+from seisfat2d_amd import ANI, SCHED_PLAIN, SCHED_TILED, SCHED_TILED_DIAG, Eikonal, capi, models
+
+pytestmark = pytest.mark.gpu
+NB = 3
+SCHEDULES = (SCHED_TILED, SCHED_TILED_DIAG, SCHED_PLAIN)
+
+
+def smooth(rng, nx, nz, amp, sigma=4.0):
+    return amp * gaussian_filter(rng.standard_normal((nx, nz)), sigma)
+
+
+def model_update(S, dS):
+    """Tomography_ISO::model_update: S[(i+nb) + (j+nb)*nzz] += dS[i + j*nz]
+    over the interior only; the border keeps its earlier values."""
+    S = S.copy()
+    S[NB:-NB, NB:-NB] = (S[NB:-NB, NB:-NB] + dS).astype(np.float32)
+    return S
+
+
+def window_view(S, x0, w):
+    """What a window shot of a model uploaded padded reads: the model's rows
+    (its padding rows included) over columns [x0, x0+w), the window's own edge
+    columns replicated."""
+    cols = x0 + NB + np.clip(np.arange(w + 2 * NB) - NB, 0, w - 1)
+    return np.ascontiguousarray(S[cols])
+
+
+def test_padded_upload_keeps_the_stale_border_of_a_tomography_update(oracle):
+    rng = np.random.default_rng(31)
+    nz, nx, dz, dx = 83, 140, 10.0, 8.0
+    s0 = (1.0 / (1800.0 + 2.0 * np.arange(nz)[None, :] * dz + smooth(rng, nx, nz, 300.0))).astype(np.float32)
+    S = oracle.expand_boundary(s0)                       # set_parameters: expand_boundary + upload
+    e = Eikonal(nz, nx, dz, dx)
+    e.set_slowness_padded(S)
+    rx = np.linspace(0.0, (nx - 1) * dx, 37).astype(np.float32)
+    rz = np.zeros_like(rx)
+    for it in range(2):
+        dS = (smooth(rng, nx, nz, 1.0) * 2e-5).astype(np.float32)
+        S = model_update(S, dS)                          # interior only
+        e.set_slowness_padded(S)                         # copy_slowness_to_device
+        dev = np.empty(S.size, np.float32)
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        assert hip.hipMemcpy(dev.ctypes.data_as(C.c_void_p), C.c_void_p(e.device_S_ptr()), S.nbytes, 2) == 0
+        assert np.array_equal(dev.reshape(S.shape), S)   # d_S is the padded model as uploaded
+        for sx, sz in ((0.0, 0.0), (0.41 * (nx - 1) * dx, 0.3 * (nz - 1) * dz), ((nx - 1) * dx, (nz - 1) * dz)):
+            Tref = oracle.solve_iso(S, dx, dz, sx, sz)
+            pref = oracle.picks(Tref, dx, dz, rx, rz)
+            for sched in SCHEDULES:
+                T, pk = e.solve(sx, sz, schedule=sched, rx=rx, rz=rz)
+                assert np.array_equal(T, Tref), (it, sx, sched)
+                assert np.array_equal(pk, pref), (it, sx, sched)
+        # the stale border matters: re-padding the updated interior changes
+        # the traveltimes near the model's edges
+        Trepad = oracle.solve_iso(oracle.expand_boundary(oracle.reduce_boundary(S)), dx, dz, 0.0, 0.0)
+        assert not np.array_equal(Trepad, oracle.solve_iso(S, dx, dz, 0.0, 0.0))
+    # batch shots: the whole-model shot reads the border as uploaded, a
+    # window replicates its own edge columns under the model's padding rows
+    shots = [(300.0, 0.0, 0, nx, 0, 0), (500.0, 40.0, 30, 70, 0, 0), (20.0, 0.0, 0, 50, 0, 0)]
+    for sched in (SCHED_TILED, SCHED_PLAIN):
+        b = e.batch(shots, schedule=sched)
+        b.run()
+        b.status()
+        for k, (sx, sz, x0, w, _, _) in enumerate(shots):
+            Sv = S if w == nx else window_view(S, x0, w)
+            assert np.array_equal(b.T(k), oracle.solve_iso(Sv, dx, dz, sx - x0 * dx, sz)), (sched, k)
+        # the adjoint differentiates with respect to the unpadded model
+        with pytest.raises(capi.SeisfatError) as ex:
+            b.run_record()
+        assert ex.value.code == capi.ESTATE
+        b.close()
+    # an unpadded upload (or a padded one whose border is the replication)
+    # makes the adjoint available again
+    e.set_slowness_padded(oracle.expand_boundary(oracle.reduce_boundary(S)))
+    b = e.batch([(300.0, 0.0, 0, nx, 0, len(rx))], rx, rz)
+    b.run_record()
+    b.status()
+    b.close()
+    e.close()
+
+
+def test_padded_upload_anisotropic(oracle):
+    """Eikonal_ANI reads d_S whole (src/modeling/eikonal_ani.cu:33-45): the qP
+    solve of a model with a stale border matches the oracle on that padded S."""
+    from seisfat2d_amd.stiffness import compute_stiffness
+    rng = np.random.default_rng(32)
+    nz, nx, dh = 70, 96, 5.0
+    vp = 2000.0 + 3.0 * np.arange(nz)[None, :] * dh + smooth(rng, nx, nz, 200.0)
+    cs = compute_stiffness(vp, vp / np.sqrt(3), 1000.0, 0.15, 0.05, 20.0)
+    s = np.sqrt(1000.0 / cs[3]).astype(np.float32)
+    cs = [c.astype(np.float32) for c in cs]
+    S = model_update(oracle.expand_boundary(s), (smooth(rng, nx, nz, 1.0) * 1e-5).astype(np.float32))
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness_padded(S)
+    e.set_stiffness(cs)
+    Cq, mn, mx = [], [], []
+    for c in cs:
+        u, a, b_ = oracle.compress(oracle.expand_boundary(c))
+        Cq.append(u)
+        mn.append(a)
+        mx.append(b_)
+    for sx in (0.0, 0.55 * (nx - 1) * dh):
+        Tref = oracle.solve_ani(S, dh, dh, sx, 0.0, Cq, mn, mx)[0]
+        for sched in SCHEDULES:
+            T, _ = e.solve(sx, 0.0, type=ANI, schedule=sched)
+            assert np.array_equal(T, Tref), (sx, sched)
+    e.close()
+
+
+def test_padded_upload_rejects_bad_border():
+    nz, nx = 20, 30
+    e = Eikonal(nz, nx, 10.0, 10.0)
+    S = np.full((nx + 6, nz + 6), 5e-4, np.float32)
+    S[1, 2] = 0.0                                        # in the border
+    with pytest.raises(capi.SeisfatError) as ex:
+        e.set_slowness_padded(S)
+    assert ex.value.code == capi.EINVAL
+    assert not e.device_S_ptr()
+    with pytest.raises(ValueError):
+        e.set_slowness_padded(np.full((nx, nz), 5e-4, np.float32))
+    e.close()
+
+
+def test_per_shot_solve_reuses_its_batch(oracle):
+    """seisfat_eik_solve keeps one one-shot batch while type, schedule and
+    receiver count hold; every call only moves the source and receivers.  A
+    sequence of shots with changing receivers, then a changed receiver count
+    and schedule, all bit-exact."""
+    rng = np.random.default_rng(33)
+    w = models.c1()
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    S = oracle.expand_boundary(w.s)
+    for sched in (SCHED_TILED, SCHED_PLAIN):
+        for k in range(4):
+            n = 50 if k < 3 else 7
+            sx = float(rng.uniform(0, (w.nx - 1) * w.dx))
+            sz = float(rng.uniform(0, (w.nz - 1) * w.dz)) if k % 2 else 0.0
+            rx = rng.uniform(0, (w.nx - 1) * w.dx, n).astype(np.float32)
+            rz = rng.uniform(0, (w.nz - 1) * w.dz, n).astype(np.float32)
+            T, pk = e.solve(sx, sz, schedule=sched, rx=rx, rz=rz)
+            Tref = oracle.solve_iso(S, w.dx, w.dz, sx, sz)
+            assert np.array_equal(T, Tref), (sched, k)
+            assert np.array_equal(pk, oracle.picks(Tref, w.dx, w.dz, rx, rz)), (sched, k)
+        # a rejected source leaves the kept batch usable
+        with pytest.raises(capi.SeisfatError):
+            e.solve(-5.0, 0.0, schedule=sched, rx=rx, rz=rz)
+        T, _ = e.solve(100.0, 10.0, schedule=sched, rx=rx, rz=rz)
+        assert np.array_equal(T, oracle.solve_iso(S, w.dx, w.dz, 100.0, 10.0))
+    e.close()
