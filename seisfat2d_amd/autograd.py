@@ -42,12 +42,20 @@ class EikonalPicks(torch.autograd.Function):
             view = torch.as_tensor(_DeviceArray(batch.device_picks_ptr(), batch.nrec), device=slowness.device)
             picks = view.clone()
         ctx.batch = batch
+        ctx.record = batch.records
         ctx.dtype = slowness.dtype
         return picks
 
     @staticmethod
     def backward(ctx, grad_picks: torch.Tensor):
         b = ctx.batch
+        if b.records != ctx.record:
+            # the batch holds one recorded trajectory: the one of its latest
+            # forward, which is not this graph's
+            raise RuntimeError("eikonal_picks backward: the batch ran another forward (record "
+                               f"{b.records}) after this graph's (record {ctx.record}); its adjoint state "
+                               "no longer describes this graph's slowness. Use one batch per live graph, or "
+                               "call backward before the next forward.")
         w = grad_picks.detach().to(torch.float32).contiguous()
         g = torch.empty((b.eik.nx, b.eik.nz), dtype=torch.float64, device=w.device)
         b.vjp_device(w.data_ptr() if w.numel() else 0, g.data_ptr(), torch.cuda.current_stream(w.device).cuda_stream)

@@ -52,8 +52,7 @@ def _run(cmd):
 
 
 def build_eik(verbose=False):
-    srcs = [os.path.join(CSRC, f) for f in ("eik_kernels.hip", "eik_capi.hip", "eik_adjoint.hip")]
-    srcs = [s for s in srcs if os.path.exists(s)]
+    srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
     tmp = EIK_LIB + ".tmp"
     out = _run([_hipcc(), *HIP_FLAGS, "-shared", "-I", INCLUDE, "-o", tmp, *srcs])
     os.replace(tmp, EIK_LIB)
@@ -79,14 +78,9 @@ def build_host(verbose=False):
     return HOST_LIB
 
 
-def build_oracle():
-    _run(["make", "-B", "-C", os.path.join(ROOT, "oracle")])
-
-
 def build_all(verbose=False):
     build_eik(verbose)
     build_host(verbose)
-    build_oracle()
 
 
 if __name__ == "__main__":

@@ -209,6 +209,33 @@ def test_autograd_function(rng):
     e.close()
 
 
+def test_autograd_backward_refuses_a_stale_record(rng):
+    """One batch holds one recorded trajectory.  After a second forward on the
+    same batch (a line search, a second loss), the first graph's backward
+    would linearise the second slowness: it raises instead; the second
+    graph's backward is exact."""
+    import torch
+    from seisfat2d_amd.autograd import eikonal_picks
+    s, dh, shots, rx, rz = window_case(rng, nz=40, nx=120)
+    nx, nz = s.shape
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    b = e.batch(shots, rx, rz)
+    s1 = torch.from_numpy(s).cuda().requires_grad_(True)
+    s2 = torch.from_numpy((s * np.float32(1.02)).astype(np.float32)).cuda().requires_grad_(True)
+    p1 = eikonal_picks(s1, b)
+    p2 = eikonal_picks(s2, b)
+    with pytest.raises(RuntimeError, match="another forward"):
+        p1.sum().backward()
+    p2.sum().backward()
+    g = torch.empty((nx, nz), dtype=torch.float64, device="cuda")
+    ones = torch.ones(len(rx), dtype=torch.float32, device="cuda")
+    b.vjp_device(ones.data_ptr(), g.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    assert torch.equal(s2.grad, g.float())
+    b.close()
+    e.close()
+
+
 def test_ray_gradient_correlation(oracle, rng):
     """The reference's tomography gradient G^T r (ray-traced, src/inversion/
     inversion.cpp:95-177) and the adjoint gradient of the same misfit agree in
