@@ -4,6 +4,8 @@ Everything is built in-tree so the shared objects travel with the source tree:
   seisfat2d_amd/libseisfat2d_eik.so    HIP kernels + C ABI (include/seisfat2d_eik.h)
   seisfat2d_amd/libseisfat2d_host.so   C++ Modeling / Eikonal_ISO / Eikonal_ANI classes
   seisfat2d_amd/seisfat2d_modeling     the per-shot modeling executable
+  seisfat2d_amd/seisfat2d_tables       traveltime tables for migration
+  seisfat2d_amd/seisfat2d_inversion    first-arrival tomography
 """
 from __future__ import annotations
 
@@ -20,6 +22,8 @@ INCLUDE = os.path.join(ROOT, "include")
 EIK_LIB = os.path.join(PKG, "libseisfat2d_eik.so")
 HOST_LIB = os.path.join(PKG, "libseisfat2d_host.so")
 MODELING_EXE = os.path.join(PKG, "seisfat2d_modeling")
+TABLES_EXE = os.path.join(PKG, "seisfat2d_tables")
+INVERSION_EXE = os.path.join(PKG, "seisfat2d_inversion")
 
 # Device arithmetic contract: no FMA contraction, correctly rounded fp32
 # division and square root (bit parity with the oracle); no packed fp32 math.
@@ -62,19 +66,21 @@ def build_eik(verbose=False):
 
 
 def build_host(verbose=False):
-    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp") and f != "modeling_main.cpp")
+    srcs = sorted(os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".cpp") and not f.endswith("_main.cpp"))
     if not srcs:
         return None
     tmp = HOST_LIB + ".tmp"
-    _run(["g++", *HOST_FLAGS, "-shared", "-I", INCLUDE, "-o", tmp, *srcs,
+    _run(["g++", *HOST_FLAGS, "-fopenmp", "-shared", "-I", INCLUDE, "-o", tmp, *srcs,
           "-L", PKG, "-lseisfat2d_eik", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, HOST_LIB)
-    main = os.path.join(HOST, "modeling_main.cpp")
-    if os.path.exists(main):
-        tmp = MODELING_EXE + ".tmp"
-        _run(["g++", *HOST_FLAGS, "-I", INCLUDE, "-o", tmp, main, "-L", PKG,
-              "-lseisfat2d_host", "-lseisfat2d_eik", "-Wl,-rpath,$ORIGIN"])
-        os.replace(tmp, MODELING_EXE)
+    # one executable per driver: <name>_main.cpp -> seisfat2d_<name>
+    for f in sorted(os.listdir(HOST)):
+        if f.endswith("_main.cpp"):
+            exe = os.path.join(PKG, "seisfat2d_" + f[:-len("_main.cpp")])
+            tmp = exe + ".tmp"
+            _run(["g++", *HOST_FLAGS, "-fopenmp", "-I", INCLUDE, "-o", tmp, os.path.join(HOST, f), "-L", PKG,
+                  "-lseisfat2d_host", "-lseisfat2d_eik", "-Wl,-rpath,$ORIGIN"])
+            os.replace(tmp, exe)
     return HOST_LIB
 
 

@@ -98,6 +98,28 @@ class Eikonal:
     def batch(self, shots, rx=None, rz=None, type=ISO, schedule=SCHED_TILED):
         return Batch(self, shots, rx, rz, type, schedule)
 
+    def traveltime_tables(self, x, z, type=ISO, schedule=SCHED_TILED, chunk=64):
+        """Padded traveltime grids (n, nx+6, nz+6) of whole-model solves from
+        each position (x[k], z[k]): the tables of Migration::set_src_travel_times
+        and set_rec_travel_times (src/migration/migration.cu:116-171), solved
+        `chunk` positions per batch."""
+        x = np.asarray(x, np.float64).ravel()
+        z = np.asarray(z, np.float64).ravel()
+        if x.shape != z.shape:
+            raise ValueError("x and z must have the same length")
+        out = np.empty((len(x), self.nxx, self.nzz), np.float32)
+        for k0 in range(0, len(x), chunk):
+            shots = [(float(x[k]), float(z[k]), 0, self.nx, 0, 0) for k in range(k0, min(k0 + chunk, len(x)))]
+            b = Batch(self, shots, None, None, type, schedule)
+            try:
+                b.run()
+                b.status()
+                for q in range(len(shots)):
+                    out[k0 + q] = b.T(q)
+            finally:
+                b.close()
+        return out
+
     def close(self):
         if getattr(self, "_h", None):
             lib().seisfat_eik_destroy(self._h)

@@ -136,3 +136,47 @@ def test_driver_ani_outputs_match_oracle(tmp_path, oracle):
             got = np.fromfile(os.path.join(tmp, "out", f"eikonal_ani_nStations{len(rx)}_shot_{k + 1}.bin"),
                               np.float32)
             assert np.array_equal(got, want), (mode, k)
+
+
+def _tables_case(tmp, ani):
+    p, s = write_case(tmp, ani=ani)
+    os.makedirs(os.path.join(tmp, "tables"), exist_ok=True)
+    with open(p, "a") as f:
+        f.write(f"\nmig_tables_folder = {tmp}/tables/\nanisotropy = {'true' if ani else 'false'}\n")
+    return p, s
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ani", [False, True])
+def test_tables_driver_matches_oracle(tmp_path, oracle, ani):
+    """seisfat2d_tables writes the migration tables the reference writes
+    (src/migration/migration.cu:116-171): one padded traveltime grid per
+    source (eikonal_src_<k>.bin) and per receiver (eikonal_rec_<k>.bin), k from
+    1, each equal to the oracle's single-pass solve from that position; the
+    positions run in batches (chunk 8 here: several batches and a partial one)."""
+    tmp = str(tmp_path)
+    p, s = _tables_case(tmp, ani)
+    r = subprocess.run([build.TABLES_EXE, p, "--chunk", "8"], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    nz, nx, dh = 41, 61, 10.0
+    S = oracle.expand_boundary(s)
+    if ani:
+        Cq, mn, mx = [], [], []
+        for name in ("C11", "C13", "C15", "C33", "C35", "C55"):
+            c = np.fromfile(os.path.join(tmp, f"model_{name}.bin"), np.float32).reshape(nx, nz)
+            u, a, b = oracle.compress(oracle.expand_boundary(c))
+            Cq.append(u)
+            mn.append(a)
+            mx.append(b)
+        solve = lambda x, z: oracle.solve_ani(S, dh, dh, x, z, Cq, mn, mx)[0]
+    else:
+        solve = lambda x, z: oracle.solve_iso(S, dh, dh, x, z)
+    srcs = [(100.0 + 150.0 * k, 0.0) for k in range(3)]
+    recs = [(float(x), 0.0) for x in np.arange(0, nx, 3) * dh]
+    for prefix, pos in (("eikonal_src_", srcs), ("eikonal_rec_", recs)):
+        for k, (x, z) in enumerate(pos):
+            got = np.fromfile(os.path.join(tmp, "tables", f"{prefix}{k + 1}.bin"), np.float32)
+            assert got.size == (nz + 6) * (nx + 6)
+            assert np.array_equal(got.reshape(nx + 6, nz + 6), solve(x, z)), (prefix, k)
+    assert not os.path.exists(os.path.join(tmp, "tables", f"eikonal_rec_{len(recs) + 1}.bin"))
