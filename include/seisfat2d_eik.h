@@ -81,8 +81,11 @@ int seisfat_eik_set_slowness_device(seisfat_eik_t h, const float* d_s, void* str
  * the border rows as uploaded and replicate their own edge columns.  Every
  * value, padding included, must be positive and finite with slowness*spacing
  * and slowness/spacing in [1e-12, 1e12].  The adjoint entry points
- * differentiate with respect to the unpadded model and refuse
- * (SEISFAT_ESTATE) a padding that is not the replication of the interior. */
+ * differentiate with respect to the unpadded model: when the border is the
+ * replication of the interior (always after seisfat_eik_set_slowness) the
+ * border's derivative folds onto the edge cells it copies; otherwise the
+ * border is a constant and only interior cells (and a window's own
+ * replicated columns) carry derivative. */
 int seisfat_eik_set_slowness_padded(seisfat_eik_t h, const float* S_padded);
 
 /* Device pointer of the padded slowness, (nz+6)*(nx+6) floats z-fastest: the
@@ -179,6 +182,12 @@ int seisfat_batch_run_record(seisfat_batch_t b, void* stream);
  * asynchronous on `stream`.  SEISFAT_ESTATE if there is no recorded run or the
  * slowness changed since it. */
 int seisfat_batch_vjp_device(seisfat_batch_t b, const float* d_w, double* d_grad, void* stream);
+/* seisfat_batch_vjp_device with host buffers (w: nrec_total floats, grad:
+ * nz*nx doubles), synchronous: the products of the reference's tomography
+ * with its ray matrix, G^T w (src/inversion/inversion.cpp:277-278), for the
+ * exact Jacobian of the recorded solve; w = 1 gives the column sums of G, the
+ * ray coverage (src/inversion/tomography_iso.cpp:34-38). */
+int seisfat_batch_vjp(seisfat_batch_t b, const float* w, double* grad);
 /* Misfit J = 1/2 sum_r (pick_r - obs_r)^2 and its gradient dJ/ds (nz*nx
  * doubles) at the handle's current slowness: run_record, residuals and the
  * vector-Jacobian product, host buffers, synchronous.  J and the residuals use

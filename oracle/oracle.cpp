@@ -690,6 +690,91 @@ int ray_paths(const float* T, int nz, int nx, float dx, float dz, float sx, floa
   return n;
 }
 
+// ---------------------------------------------------------------------------
+// Tomography operators.
+//
+// Inversion::set_regularization_matrix (src/inversion/inversion.cpp:209-262):
+// order 0 is the identity; order k >= 1 is the k-th forward difference over
+// the flattened (z-fastest) model vector, n - k rows, row r holding df[q] at
+// column r + q, where df = [-1, 1] convolved with [-1, 1] k - 1 times (built
+// by the reference's df1/df2 recurrence).  y = D x.
+void tikhonov_coefficients(int order, std::vector<float>& df) {
+  df.assign(order + 1, 0.0f);
+  if (order <= 0) {
+    df[0] = 1.0f;
+    return;
+  }
+  std::vector<float> d1(order + 2, 0.0f), d2(order + 2, 0.0f);
+  df[0] = -1.0f;
+  df[1] = 1.0f;
+  for (int it = 1; it < order; ++it)
+    for (int q = 0; q < order + 1; ++q) {
+      d2[q] = df[q];
+      d1[q + 1] = df[q];
+      df[q] = d1[q] - d2[q];
+    }
+}
+
+void tikhonov_apply(int order, const double* x, long long n, double* y) {
+  std::vector<float> df;
+  tikhonov_coefficients(order, df);
+  const long long rows = n - std::max(order, 0);
+  for (long long r = 0; r < rows; ++r) {
+    double acc = 0.0;
+    for (int q = 0; q < (int)df.size(); ++q) acc += (double)df[q] * x[r + q];
+    y[r] = acc;
+  }
+}
+
+// D^T y (n outputs from n - order inputs).
+void tikhonov_adjoint(int order, const double* y, long long n, double* x) {
+  std::vector<float> df;
+  tikhonov_coefficients(order, df);
+  const long long rows = n - std::max(order, 0);
+  for (long long c = 0; c < n; ++c) x[c] = 0.0;
+  for (long long r = 0; r < rows; ++r)
+    for (int q = 0; q < (int)df.size(); ++q) x[r + q] += (double)df[q] * y[r];
+}
+
+// Inversion::model_smoothing + smooth_matrix (src/inversion/inversion.cpp:
+// 347-447): the model is embedded in a zero border of `samples` cells, then
+// convolved with a samples x samples Gaussian (weights exp(-r^2/(2 stdv^2))
+// / (pi stdv), normalised to sum 1; a unit delta when stdv = 0) at every node
+// at least samples/2 from the embedding's edge (the others keep their
+// value), and the model region is read back.  FP32 as the reference.
+void smooth_model(const float* in, float* out, int nz, int nx, float stdv, int samples) {
+  const int anx = nx + 2 * samples, anz = nz + 2 * samples;
+  std::vector<float> aux((size_t)anx * anz, 0.0f), sm((size_t)anx * anz);
+  for (int j = 0; j < nx; ++j)
+    for (int i = 0; i < nz; ++i) aux[(i + samples) + (size_t)(j + samples) * anz] = in[i + (size_t)j * nz];
+  const int nk = samples * samples, mid = samples / 2;
+  std::vector<float> kernel(nk, 0.0f);
+  sm = aux;
+  kernel[mid + mid * samples] = 1.0f;
+  if (stdv != 0.0f) {
+    float sum = 0.0f;
+    for (int x = -mid; x <= mid; ++x)
+      for (int z = -mid; z <= mid; ++z) {
+        const int index = (z + mid) + (x + mid) * samples;
+        const float r = std::sqrt((float)(x * x + z * z));
+        // the reference's 1.0f / (M_PI * stdv) is a double expression
+        kernel[index] = (float)(1.0 / (M_PI * (double)stdv) * (double)std::exp(-((r * r) / (2.0f * stdv * stdv))));
+        sum += kernel[index];
+      }
+    for (int q = 0; q < nk; ++q) kernel[q] /= sum;
+  }
+  for (int j = mid; j < anx - mid; ++j)
+    for (int i = mid; i < anz - mid; ++i) {
+      float acc = 0.0f;
+      for (int xk = 0; xk < samples; ++xk)
+        for (int zk = 0; zk < samples; ++zk)
+          acc += aux[(i - mid + zk) + (size_t)(j - mid + xk) * anz] * kernel[zk + xk * samples];
+      sm[i + (size_t)j * anz] = acc;
+    }
+  for (int j = 0; j < nx; ++j)
+    for (int i = 0; i < nz; ++i) out[i + (size_t)j * nz] = sm[(i + samples) + (size_t)(j + samples) * anz];
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -816,6 +901,12 @@ void orc_branch_counts_f32(const float* S, int nzz, int nxx, float dx, float dz,
 void orc_sweep_counted_f32(float* T, const float* S, int nzz, int nxx, float dx, float dz, int sweep,
                            long long* counts) {
   sweep_counted(T, S, nzz, nxx, sweep, make_consts(dx, dz), counts);
+}
+
+void orc_tikhonov_apply(int order, const double* x, long long n, double* y) { tikhonov_apply(order, x, n, y); }
+void orc_tikhonov_adjoint(int order, const double* y, long long n, double* x) { tikhonov_adjoint(order, y, n, x); }
+void orc_smooth_model(const float* in, float* out, int nz, int nx, float stdv, int samples) {
+  smooth_model(in, out, nz, nx, stdv, samples);
 }
 
 int orc_ray_paths(const float* T, int nz, int nx, float dx, float dz, float sx, float sz, const float* rx,

@@ -59,6 +59,9 @@ def lib():
             "orc_adjoint_f64": (d, [f64p, i, i, d, d, d, d, f64p, f64p, i, f64p, f64p, f64p]),
             "orc_branch_counts_f32": (None, [f32p, i, i, f, f, f, f, i64p]),
             "orc_sweep_counted_f32": (None, [f32p, f32p, i, i, f, f, i, i64p]),
+            "orc_tikhonov_apply": (None, [i, f64p, C.c_longlong, f64p]),
+            "orc_tikhonov_adjoint": (None, [i, f64p, C.c_longlong, f64p]),
+            "orc_smooth_model": (None, [f32p, f32p, i, i, f, i]),
             "orc_ray_paths": (i, [f32p, i, i, f, f, f, f, f32p, f32p, i, i, i32p, i32p, f32p, i]),
             "orc_batch_misfit_f32": (d, [f32p, i, i, f, f, i, f32p, f32p, i32p, i32p, i32p, i32p, f32p, f32p,
                                          f32p, C.c_void_p, C.c_void_p, i]),
@@ -255,3 +258,28 @@ def ray_paths(T, nz, nx, dx, dz, sx, sz, rx, rz, max_steps=200000, cap=None):
     if n > cap:
         raise RuntimeError("COO capacity exceeded")
     return rec[:n], cell[:n], ln[:n]
+
+
+def tikhonov(order, x):
+    """The reference's Tikhonov operator D (inversion.cpp:209-262) on the
+    flattened model x: n - order rows."""
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    y = np.zeros(len(x) - max(order, 0))
+    lib().orc_tikhonov_apply(order, x, len(x), y)
+    return y
+
+
+def tikhonov_adjoint(order, y, n):
+    y = np.ascontiguousarray(y, np.float64).ravel()
+    x = np.zeros(n)
+    lib().orc_tikhonov_adjoint(order, y, n, x)
+    return x
+
+
+def smooth_model(m, stdv, samples):
+    """Inversion::model_smoothing (inversion.cpp:347-447) of an (nx, nz) model."""
+    m = np.ascontiguousarray(m, np.float32)
+    nx, nz = m.shape
+    out = np.empty_like(m)
+    lib().orc_smooth_model(m, out, nz, nx, stdv, samples)
+    return out

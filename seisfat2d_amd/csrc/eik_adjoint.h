@@ -81,7 +81,9 @@ void launch_adj_fold(int sweep, const AdjShot* shots, int nshots, int nstr, int 
                      const uint8_t* code, float* gshot, hipStream_t st);
 void launch_adj_source(const AdjShot* shots, int nshots, int nzz, double dx, double dz, const float* lam0,
                        float* gshot, hipStream_t st);
+// fixed_border: the model's border is not the replication of its edges (a
+// padded upload), so only interior nodes (and windows' replicated columns) fold
 void launch_adj_reduce(const AdjShot* shots, int nshots, int nz, int nx, int nzz, const float* gshot,
-                       double* grad, hipStream_t st);
+                       double* grad, bool fixed_border, hipStream_t st);
 
 }  // namespace seisfat

@@ -659,22 +659,26 @@ __global__ void k_adj_source(const AdjShot* __restrict__ shots, int nshots, int 
 
 // d/ds of the unpadded model: every padded node of every shot maps to the
 // model cell it replicates (rows clamped to the model, columns clamped to the
-// shot's window); sums in fp64, shots in order.
+// shot's window); sums in fp64, shots in order.  fixed_border: the model's
+// border was uploaded as given (not the replication of its edges), so its
+// rows, and the border columns of shots on the whole model, are constants
+// with no derivative; a window's own replicated columns still fold.
 __global__ __launch_bounds__(256) void k_adj_reduce(const AdjShot* __restrict__ shots, int nshots, int nz,
                                                     int nx, int nzz, const float* __restrict__ gshot,
-                                                    double* __restrict__ grad) {
+                                                    double* __restrict__ grad, int fixed_border) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nz * nx) return;
   const int iz = idx % nz, ix = idx / nz;
-  const int plo = iz == 0 ? 0 : iz + NB;
-  const int phi = iz == nz - 1 ? nz - 1 + 2 * NB : iz + NB;
+  const int plo = (iz == 0 && !fixed_border) ? 0 : iz + NB;
+  const int phi = (iz == nz - 1 && !fixed_border) ? nz - 1 + 2 * NB : iz + NB;
   double acc = 0.0;
   for (int s = 0; s < nshots; ++s) {
     const int x0 = shots[s].sd.x0, w = shots[s].sd.w;
     if (ix < x0 || ix >= x0 + w) continue;
     const int lq = ix - x0;
-    const int qlo = lq == 0 ? 0 : lq + NB;
-    const int qhi = lq == w - 1 ? w - 1 + 2 * NB : lq + NB;
+    const bool cfix = fixed_border && shots[s].sd.sofs == 0;  // a whole-model shot reads the border as given
+    const int qlo = (lq == 0 && !cfix) ? 0 : lq + NB;
+    const int qhi = (lq == w - 1 && !cfix) ? w - 1 + 2 * NB : lq + NB;
     const float* __restrict__ g = gshot + shots[s].toff;
     for (int q = qlo; q <= qhi; ++q)
       for (int p = plo; p <= phi; ++p) acc += (double)g[p + (size_t)q * nzz];
@@ -776,10 +780,10 @@ void launch_adj_source(const AdjShot* shots, int nshots, int nzz, double dx, dou
 }
 
 void launch_adj_reduce(const AdjShot* shots, int nshots, int nz, int nx, int nzz, const float* gshot,
-                       double* grad, hipStream_t st) {
+                       double* grad, bool fixed_border, hipStream_t st) {
   const int n = nz * nx;
   hipLaunchKernelGGL(k_adj_reduce, dim3((n + 255) / 256), dim3(256), 0, st, shots, nshots, nz, nx, nzz, gshot,
-                     grad);
+                     grad, fixed_border ? 1 : 0);
 }
 
 }  // namespace seisfat

@@ -793,10 +793,7 @@ int adj_prepare(seisfat_batch_s* b) {
 // (T0 = initial field, T_{s+1} = sweep s of T_s).
 int run_record(seisfat_batch_s* b, hipStream_t st) {
   if (b->type != SEISFAT_ISO) return fail(SEISFAT_EINVAL, "the adjoint covers isotropic batches");
-  if (!b->h->pad_replicated)
-    return fail(SEISFAT_ESTATE, "the adjoint differentiates with respect to the unpadded model, whose padding "
-                                "replicates its edges; this slowness was uploaded with a different padding "
-                                "(seisfat_eik_set_slowness_padded)");
+
   if (int rc = adj_prepare(b)) return rc;
   AdjState& A = b->adj;
   A.recorded = false;
@@ -814,6 +811,7 @@ int run_record(seisfat_batch_s* b, hipStream_t st) {
   HIPCHK(hipEventRecord(b->ev_done, st));
   A.recorded = true;
   A.version = h->version;
+  A.pad_replicated = h->pad_replicated;
   return SEISFAT_OK;
 }
 
@@ -837,7 +835,7 @@ int enqueue_vjp(seisfat_batch_s* b, const float* d_w, double* d_grad, hipStream_
     launch_adj_fold(s, A.d_tab, ns, A.nstr, A.max_nxx, nzz, A.d_gc, A.d_code, A.d_gshot, st);
   }
   launch_adj_source(A.d_tab, ns, nzz, h->dx, h->dz, A.d_lamA, A.d_gshot, st);
-  launch_adj_reduce(A.d_tab, ns, h->nz, h->nx, nzz, A.d_gshot, d_grad, st);
+  launch_adj_reduce(A.d_tab, ns, h->nz, h->nx, nzz, A.d_gshot, d_grad, !A.pad_replicated, st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(b->ev_done, st));
   return SEISFAT_OK;
@@ -1242,6 +1240,23 @@ int seisfat_batch_misfit_gradient(seisfat_batch_t b, const float* obs, double* J
   if (J) *J = j;
   if (picks && b->nrec > 0) std::memcpy(picks, pk.data(), b->nrec * sizeof(float));
   return SEISFAT_OK;
+}
+
+int seisfat_batch_vjp(seisfat_batch_t b, const float* w, double* grad) {
+  if (!b) return fail(SEISFAT_EINVAL, "null batch");
+  if (!grad) return fail(SEISFAT_EINVAL, "null gradient buffer");
+  if (b->nrec > 0 && !w) return fail(SEISFAT_EINVAL, "null cotangents");
+  seisfat_eik_s* h = b->h;
+  AdjState& A = b->adj;
+  if (!A.recorded) return fail(SEISFAT_ESTATE, "no recorded run (seisfat_batch_run_record)");
+  hipStream_t st = h->stream;
+  // the recorded run may have gone to another stream: wait for it first
+  HIPCHK(hipStreamWaitEvent(st, b->ev_done, 0));
+  if (b->nrec > 0) HIPCHK(hipMemcpyAsync(A.d_w, w, b->nrec * sizeof(float), hipMemcpyHostToDevice, st));
+  if (int rc = enqueue_vjp(b, A.d_w, A.d_grad, st)) return rc;
+  HIPCHK(hipMemcpyAsync(grad, A.d_grad, (size_t)h->nz * h->nx * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return batch_status(b);
 }
 
 int seisfat_batch_destroy(seisfat_batch_t b) {

@@ -56,6 +56,7 @@ struct AdjState {
   bool ready = false;
   bool recorded = false;  // a recorded run exists for model `version`
   unsigned version = 0;
+  bool pad_replicated = true;  // the recorded model's border replicated its edges
   std::vector<AdjShot> tab;
   AdjShot* d_tab = nullptr;
   int nstr = 0, max_nxx = 0;

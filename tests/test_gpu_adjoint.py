@@ -216,7 +216,7 @@ def test_autograd_backward_refuses_a_stale_record(rng):
     graph's backward is exact."""
     import torch
     from seisfat2d_amd.autograd import eikonal_picks
-    s, dh, shots, rx, rz = window_case(rng, nz=40, nx=120)
+    s, dh, shots, rx, rz = window_case(rng, nz=40)
     nx, nz = s.shape
     e = Eikonal(nz, nx, dh, dh)
     e.set_slowness(s)

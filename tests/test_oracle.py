@@ -440,3 +440,51 @@ def test_batch_misfit_window_fold_matches_central_differences(oracle, rng):
     _, gp, _ = oracle.adjoint(oracle.expand_boundary(s32), dh, dh, 200.0, 0.0, rx[:n0].astype(np.float32),
                               rz[:n0].astype(np.float32), obs[:n0].astype(np.float32))
     assert np.array_equal(g1, oracle.reduce_boundary_adjoint(gp))
+
+
+# ---------------------------------------------- tomography operators (f1)
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_tikhonov_operator_on_polynomials(oracle, order):
+    # The reference's regularisation matrix (inversion.cpp:209-262) is the
+    # order-k forward difference of the flattened model: it annihilates
+    # polynomials of degree < k and maps n^k to the constant k!.
+    from math import factorial
+    n = 40
+    t = np.arange(n, dtype=np.float64)
+    for deg in range(order):
+        y = oracle.tikhonov(order, 3.0 - 0.5 * t ** deg)
+        assert y.shape == (n - order,)
+        assert np.max(np.abs(y)) < 1e-9
+    assert np.allclose(oracle.tikhonov(order, t ** order), factorial(order), rtol=0, atol=1e-6)
+    assert np.array_equal(oracle.tikhonov(0, t), t)               # order 0: identity
+    # the adjoint is the transpose: <D x, y> = <x, D^T y>
+    rng = np.random.default_rng(order)
+    x, y = rng.standard_normal(n), rng.standard_normal(n - order)
+    assert np.isclose(oracle.tikhonov(order, x) @ y, x @ oracle.tikhonov_adjoint(order, y, n), rtol=1e-12)
+    assert np.array_equal(oracle.tikhonov(1, np.eye(3)[1]), [1.0, -1.0])  # df = [-1, 1]
+
+
+def test_model_smoothing_closed_forms(oracle, rng):
+    # Gaussian smoothing of the model update (inversion.cpp:347-447): the
+    # kernel sums to one and is symmetric, so a linear field is unchanged
+    # wherever the kernel stays inside the model; the model is embedded in
+    # zeros, so near an edge a constant c becomes c times the kernel mass
+    # inside the model; stdv 0 is the identity.
+    nz, nx, stdv, ns = 30, 40, 2.0, 7
+    mid = ns // 2
+    x = np.arange(nx)[:, None].astype(np.float32)
+    z = np.arange(nz)[None, :].astype(np.float32)
+    lin = (1.0 + 0.01 * x - 0.02 * z).astype(np.float32) * np.ones((nx, nz), np.float32)
+    out = oracle.smooth_model(lin, stdv, ns)
+    inner = (slice(mid, nx - mid), slice(mid, nz - mid))
+    assert np.max(np.abs(out[inner] - lin[inner])) < 2e-6
+    c = np.full((nx, nz), 3.0, np.float32)
+    out = oracle.smooth_model(c, stdv, ns)
+    k1 = np.exp(-np.arange(-mid, mid + 1) ** 2 / (2 * stdv ** 2))     # separable Gaussian weights
+    k1 = k1 / k1.sum()
+    for j, i in ((0, 0), (1, 5), (nx - 1, nz - 2), (20, 0)):
+        wx = sum(k1[q + mid] for q in range(-mid, mid + 1) if 0 <= j + q < nx)
+        wz = sum(k1[q + mid] for q in range(-mid, mid + 1) if 0 <= i + q < nz)
+        assert abs(out[j, i] - 3.0 * wx * wz) < 1e-5, (j, i)
+    m = rng.standard_normal((nx, nz)).astype(np.float32)
+    assert np.array_equal(oracle.smooth_model(m, 0.0, ns), m)
