@@ -9,12 +9,15 @@ is one solve of all 256 shots, including initialisation and receiver picks,
 with the model resident in HBM.
 
 Reported beside it: the plain per-shot GPU solve (one launch per
-anti-diagonal, the reference's schedule) on a sample of the same windows, the
-same windows solved one shot at a time with the tiled schedule, the HBM
-roofline fraction of the sweep kernel, the CPU oracle timed on a sample of
-the same windows, and two secondary workloads: the batched anisotropic (TTI
-qP) solve of 128 shots on 801 x 3001, and the adjoint-state misfit gradient
-of 128 shots x 401 receivers on 301 x 1201.
+anti-diagonal, the reference's schedule) on a sample of the same windows
+(rate and spread), the same windows solved one shot at a time with the tiled
+schedule, the drop-in per-shot driver loop (seisfat_eik_solve on the whole
+model), the HBM roofline fraction of the sweep kernel, the CPU oracle timed on
+samples of the same windows (one thread, and shot-parallel on the host cores),
+traveltime errors of every config against the FP32 and FP64 oracle and t = r/v,
+and two secondary workloads with their own rooflines: the batched anisotropic
+(TTI qP) solve of 128 shots on 801 x 3001, and the adjoint-state misfit
+gradient of 128 shots x 401 receivers on 301 x 1201.
 
 Prints one JSON line.  `--gpus` above 1 is refused: this build runs on one GPU.
 """
@@ -53,10 +56,45 @@ def measured_traffic():
     return profile_figure("traffic_bytes_per_launch")
 
 
+ANI_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ani_c4_128shots_traffic.txt")
+ADJ_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adjoint_c5_128shots_traffic.txt")
+
+
+def profile_figures(path):
+    """key: number pairs of a committed counter summary ({} if absent)."""
+    out = {}
+    try:
+        for line in open(path):
+            if ":" in line:
+                k, v = line.split(":", 1)
+                try:
+                    out[k.strip()] = float(v)
+                except ValueError:
+                    pass
+    except OSError:
+        pass
+    return out
+
+
+def _cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 0, model
+
+
 def cpu_baseline(w, batch, budget_s):
-    """Oracle (single-threaded C++ restatement) on the first windows of the
-    workload until `budget_s` seconds of CPU work; also checks those windows
-    against the GPU traveltimes bit for bit."""
+    """The oracle (the C++ restatement of the reference's single-pass solve)
+    on windows of the workload: (a) one thread, on the first windows until
+    `budget_s` seconds; (b) shot-parallel over the host cores this process may
+    use (at most 16, the box's share), on the next windows for about the same
+    time.  Every window solved is also compared with the GPU traveltimes bit
+    for bit."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
@@ -71,10 +109,33 @@ def cpu_baseline(w, batch, budget_s):
             mismatches += 1
         cu += models.cell_updates(w.nz, [wd])
         done += 1
-    return {"value": cu / t_cpu, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} of the {len(w.shots)} windows, one pass each, oracle/oracle.cpp "
-                      f"(g++ -O3, FP32, single thread), {t_cpu:.1f} s",
-            "shots_per_s": done / t_cpu, "gpu_windows_bit_exact": f"{done - mismatches}/{done}"}
+    single = cu / t_cpu
+    nproc, model = _cpu_info()
+    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc))
+    # about budget_s of wall time: the single-thread rate x threads predicts it
+    k0, k1, cu_mt = done, done, 0
+    target = single * threads * budget_s
+    while k1 < len(w.shots) and cu_mt < target:
+        cu_mt += models.cell_updates(w.nz, [w.shots[k1][3]])
+        k1 += 1
+    if k1 == k0:  # every window went to the single-thread leg: time them again
+        k0, k1, cu_mt = 0, done, cu
+    sel = w.shots[k0:k1]
+    t0 = time.perf_counter()
+    Ts = pyoracle.solve_windows(w.s, w.dx, w.dz, [x[0] for x in sel], [x[1] for x in sel], [x[2] for x in sel],
+                                [x[3] for x in sel], nthreads=threads)
+    t_mt = time.perf_counter() - t0
+    for k, T in zip(range(k0, k1), Ts):
+        if not np.array_equal(T, batch.T(k)):
+            mismatches += 1
+    checked = done + (k1 - k0 if k0 == done else 0)
+    return {"value": cu_mt / t_mt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"windows {k0}-{k1 - 1} of the {len(w.shots)}, one pass each, oracle/oracle.cpp (g++ -O3, "
+                      f"FP32), shot-parallel on {threads} threads, {t_mt:.1f} s; single-thread leg on windows "
+                      f"0-{done - 1}, {t_cpu:.1f} s",
+            "shots_per_s": (k1 - k0) / t_mt, "nproc": nproc, "cpu_model": model,
+            "single_core": {"value": single, "unit": "cell-updates/s", "cores": 1, "shots_per_s": done / t_cpu},
+            "gpu_windows_bit_exact": f"{checked - mismatches}/{checked}"}
 
 
 def bench_ani(steps):
@@ -88,6 +149,7 @@ def bench_ani(steps):
     e.set_stiffness(w.stiffness)
     b = e.batch(w.shots, w.rx, w.rz, type=ANI, schedule=SCHED_TILED)
     b.run()
+    b.status()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -95,10 +157,32 @@ def bench_ani(steps):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     b.status()
-    cu = 2 * models.cell_updates(w.nz, [s[3] for s in w.shots])
+    # device time of the solve phase (both solves and the qP slowness pass,
+    # HIP events around them inside the run), mean of 3 runs
+    ph = []
+    for _ in range(3):
+        b.run()
+        ph.append(b.sweep_ms())
+    b.status()
+    phase_s = sum(ph) / len(ph) / 1e3
+    ncu = models.cell_updates(w.nz, [s[3] for s in w.shots])
+    cu = 2 * ncu
+    # algorithmic bytes: 12 B per cell-update of both solves, and the qP pass's
+    # 24 B per padded node and shot (T 4-neighbour read once, 6 x u16 stiffness
+    # (shared, counted per shot), S read and written)
+    nodes = (w.nz + 6) * (w.nx + 6) * len(w.shots)
+    alg = BYTES_PER_CELL_UPDATE * cu + 24 * nodes
+    achieved = alg / phase_s / 1e9
+    tr = profile_figures(ANI_TRAFFIC_FILE)
     out = {"workload": f"batched TTI qP FSM, {len(w.shots)} full-model shots on {w.nz}x{w.nx} dh=5 m "
                        "(eps, delta, tilt smooth seeded fields), two single-pass solves + qP slowness per shot",
-           "ms_per_step": dt * 1e3, "shots_per_s": len(w.shots) / dt, "cell_updates_per_s": cu / dt}
+           "ms_per_step": dt * 1e3, "shots_per_s": len(w.shots) / dt, "cell_updates_per_s": cu / dt,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": tr.get("traffic_bytes_per_step"),
+                        "traffic_source": os.path.relpath(ANI_TRAFFIC_FILE, ROOT),
+                        "scope": "solve phase (4 k_sweep_pair launches + k_ani_table + k_quasi_slowness)",
+                        "algorithmic_bytes_per_step": alg, "phase_ms": phase_s * 1e3}}
     b.close()
     e.close()
     return out
@@ -132,12 +216,134 @@ def bench_adjoint(steps):
             t_rec += t1 - t0
             t_vjp += t2 - t1
     t_rec, t_vjp = t_rec / steps, t_vjp / steps
+    b.status()
+    # algorithmic bytes: the forward solve's 12 B per cell-update and the
+    # adjoint's 16 B (T, S and lambda read, lambda written)
+    cu = models.cell_updates(w.nz, [s[3] for s in w.shots])
+    alg = (BYTES_PER_CELL_UPDATE + 16) * cu
+    achieved = alg / (t_rec + t_vjp) / 1e9
+    tr = profile_figures(ADJ_TRAFFIC_FILE)
     out = {"workload": f"misfit gradient, {len(w.shots)} shots x 401 receivers on {w.nz}x{w.nx} dh=10 m "
                        "(v(z) start model, picks of v(z)+4 anomalies)",
            "ms_record": t_rec * 1e3, "ms_vjp": t_vjp * 1e3, "ms_gradient": (t_rec + t_vjp) * 1e3,
-           "shots_per_s": len(w.shots) / (t_rec + t_vjp), "grad_norm": float(g.norm())}
+           "shots_per_s": len(w.shots) / (t_rec + t_vjp), "grad_norm": float(g.norm()),
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": tr.get("traffic_bytes_per_gradient"),
+                        "traffic_source": os.path.relpath(ADJ_TRAFFIC_FILE, ROOT),
+                        "scope": "whole gradient (recorded forward + vector-Jacobian product), wall clock",
+                        "algorithmic_bytes_per_gradient": alg}}
     b.close()
     e.close()
+    return out
+
+
+def calibration_copy():
+    """Counter calibration for profiler runs: a plain device copy of exactly
+    256 MiB (reads 256 MiB, writes 256 MiB)."""
+    import torch
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"profile: calibration copy of {src.numel() * 4} bytes done")
+
+
+def drop_in_per_shot(nshots=10):
+    """The reference's per-shot driver loop through the drop-in C ABI
+    (seisfat_eik_solve, whose one-shot batch is built once and reused): shots
+    on the whole 1001 x 4001 model, surface receivers every 20 m picked on the
+    device; once without and once with the D2H copy of the padded traveltimes
+    that Modeling::compute_seismogram makes (src/modeling/modeling.cu:165)."""
+    import numpy as np
+    from seisfat2d_amd import SCHED_TILED, Eikonal, models
+    w = models.c2()
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    xs = np.linspace(100.0, (w.nx - 1) * w.dx - 100.0, nshots)
+    e.solve(float(xs[0]), 0.0, schedule=SCHED_TILED, rx=w.rx, rz=w.rz, return_T=False)  # builds the batch
+    out = {}
+    for with_T in (False, True):
+        t0 = time.perf_counter()
+        for x in xs:
+            e.solve(float(x), 0.0, schedule=SCHED_TILED, rx=w.rx, rz=w.rz, return_T=with_T)
+        dt = time.perf_counter() - t0
+        out["shots_per_s_with_T_copy" if with_T else "shots_per_s"] = nshots / dt
+    e.close()
+    out["workload"] = (f"{nshots} shots on the whole {w.nz}x{w.nx} model, one seisfat_eik_solve per shot "
+                       f"(tiled schedule), {len(w.rx)} receivers")
+    return out
+
+
+def traveltime_errors():
+    """Max relative traveltime error over interior nodes (T_ref > 0) of the GPU
+    solve against the FP32 oracle (the reference's CPU path restated; the
+    parity gate, 1e-5) and the FP64 oracle, per config, and of C1's
+    constant-velocity variant against t = r/v."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as o
+    from seisfat2d_amd import ANI, Eikonal, models
+
+    def rel(a, ref):
+        a = a[3:-3, 3:-3].astype(np.float64)
+        ref = ref[3:-3, 3:-3].astype(np.float64)
+        m = ref > 0
+        return float(np.max(np.abs(a[m] - ref[m]) / ref[m]))
+
+    out = {}
+    for name, w, (sx, sz) in (("C1", models.c1(), (5000.0, 0.0)), ("C2", models.c2(), (10000.0, 0.0)),
+                              ("C5", models.c5(), (models.c5().shots[37][0], 0.0))):
+        e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+        e.set_slowness(w.s)
+        T, _ = e.solve(sx, sz)
+        e.close()
+        S32 = o.expand_boundary(w.s)
+        r = {"vs_fp32_oracle": rel(T, o.solve_iso(S32, w.dx, w.dz, sx, sz)),
+             "vs_fp64_oracle": rel(T, o.solve_iso(o.expand_boundary(w.s.astype(np.float64)), w.dx, w.dz, sx, sz))}
+        if name == "C1":
+            wc = models.c1(constant=True)
+            e = Eikonal(wc.nz, wc.nx, wc.dz, wc.dx)
+            e.set_slowness(wc.s)
+            Tc, _ = e.solve(sx, sz)
+            e.close()
+            x = (np.arange(wc.nx + 6) - 3) * wc.dx
+            z = (np.arange(wc.nz + 6) - 3) * wc.dz
+            r["constant_velocity_vs_r_over_v"] = rel(Tc, np.hypot(x[:, None] - sx, z[None, :] - sz) / 2000.0)
+        out[name] = r
+    # C3: two windows of the batch workload against FP32 and FP64
+    w = models.c3()
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    worst32 = worst64 = 0.0
+    for k in (0, 200):
+        sx, sz, x0, wd, _, _ = w.shots[k]
+        b = e.batch([(sx, sz, x0, wd, 0, 0)])
+        b.run()
+        T = b.T(0)
+        b.close()
+        crop = np.ascontiguousarray(w.s[x0:x0 + wd])
+        worst32 = max(worst32, rel(T, o.solve_iso(o.expand_boundary(crop), w.dx, w.dz, sx - x0 * w.dx, sz)))
+        worst64 = max(worst64, rel(T, o.solve_iso(o.expand_boundary(crop.astype(np.float64)), w.dx, w.dz,
+                                                  sx - x0 * w.dx, sz)))
+    e.close()
+    out["C3"] = {"vs_fp32_oracle": worst32, "vs_fp64_oracle": worst64, "sample": "windows 0 and 200"}
+    # C4: one shot of the TTI batch against the FP32 oracle (the oracle's
+    # anisotropic path is FP32 only)
+    w = models.c4(nshots=2)
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    e.set_stiffness(w.stiffness)
+    sx = w.shots[1][0]
+    T, _ = e.solve(sx, 0.0, type=ANI)
+    e.close()
+    Cq, mn, mx = [], [], []
+    for c in w.stiffness:
+        u, a, bb = o.compress(o.expand_boundary(c))
+        Cq.append(u)
+        mn.append(a)
+        mx.append(bb)
+    out["C4"] = {"vs_fp32_oracle": rel(T, o.solve_ani(o.expand_boundary(w.s), w.dx, w.dz, sx, 0.0, Cq, mn, mx)[0]),
+                 "sample": "shot 2 of 2 evenly spaced"}
     return out
 
 
@@ -153,7 +359,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--shots", type=int, default=256)
-    ap.add_argument("--plain-shots", type=int, default=3, help="windows timed with the per-diagonal schedule")
+    ap.add_argument("--plain-shots", type=int, default=8, help="windows timed with the per-diagonal schedule")
+    ap.add_argument("--errors", type=int, default=1, help="report traveltime errors of every config against "
+                                                          "the FP32 and FP64 oracle and against t = r/v")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU oracle budget (0 = skip)")
     ap.add_argument("--extras", type=int, default=1, help="also time the anisotropic and adjoint workloads")
     ap.add_argument("--profile", action="store_true",
@@ -177,6 +385,8 @@ def main():
     if args.only:
         run = bench_ani if args.only == "ani" else bench_adjoint
         print(json.dumps(run(max(1, args.steps))))
+        if args.profile:
+            calibration_copy()
         return
     dev = torch.cuda.get_device_properties(0)
     log(f"device {dev.name} CUs {dev.multi_processor_count}")
@@ -190,12 +400,14 @@ def main():
 
     for _ in range(args.warmup):
         b.run()
+    b.status()  # raises if a persistent sweep flagged a failure
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         b.run()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    b.status()  # the timed runs completed without a device-side failure
     ms_step = dt / args.steps * 1e3
     value = cu * args.steps / dt
     shots_per_s = len(w.shots) * args.steps / dt
@@ -217,41 +429,38 @@ def main():
     log(f"sweep phase {sweep_ms:.2f} ms over {nlaunch} launches -> {achieved:.0f} GB/s algorithmic")
 
     if args.profile:
-        # counter calibration: a plain copy of exactly 256 MiB (reads 256 MiB, writes 256 MiB)
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"profile: calibration copy of {src.numel() * 4} bytes done")
+        calibration_copy()
         b.close()
         e.close()
         return
 
-    # per-shot comparisons on the first windows
+    # per-shot comparisons on the first windows: the reference's schedule
+    # (one launch per anti-diagonal) and the tiled one, one window at a time
     nps = max(1, min(args.plain_shots, len(w.shots)))
-    t_plain = 0.0
-    for k in range(nps):
-        bp = e.batch([w.shots[k][:4] + (0, 0)], schedule=SCHED_PLAIN)
-        bp.run()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        bp.run()
-        torch.cuda.synchronize()
-        t_plain += time.perf_counter() - t1
-        bp.close()
-    plain_shots_per_s = nps / t_plain
-    t_single = 0.0
-    for k in range(nps):
-        bs = e.batch([w.shots[k][:4] + (0, 0)], schedule=SCHED_TILED)
-        bs.run()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        bs.run()
-        torch.cuda.synchronize()
-        t_single += time.perf_counter() - t1
-        bs.close()
-    single_shots_per_s = nps / t_single
-    log(f"per-shot: plain {plain_shots_per_s:.1f} shots/s, tiled {single_shots_per_s:.1f} shots/s")
+
+    def per_shot(sched):
+        rates = []
+        for k in range(nps):
+            bp = e.batch([w.shots[k][:4] + (0, 0)], schedule=sched)
+            bp.run()
+            bp.status()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            bp.run()
+            torch.cuda.synchronize()
+            rates.append(1.0 / (time.perf_counter() - t1))
+            bp.status()
+            bp.close()
+        rates = np.array(rates)
+        return nps / np.sum(1.0 / rates), {"min": float(rates.min()), "median": float(np.median(rates)),
+                                           "max": float(rates.max()), "windows": nps}
+
+    plain_shots_per_s, plain_spread = per_shot(SCHED_PLAIN)
+    single_shots_per_s, single_spread = per_shot(SCHED_TILED)
+    log(f"per-shot: plain {plain_shots_per_s:.1f} shots/s ({plain_spread}), tiled {single_shots_per_s:.1f} shots/s")
+    drop_in = drop_in_per_shot()
+    log(f"drop-in per-shot driver (whole C2 model): {drop_in['shots_per_s']:.1f} shots/s, "
+        f"{drop_in['shots_per_s_with_T_copy']:.1f} with the T copy")
 
     extras = {}
     if args.extras:
@@ -264,7 +473,11 @@ def main():
     cpu = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(w, b, args.cpu_seconds)
-        log(f"cpu oracle: {cpu['value'] / 1e6:.1f} M cell-updates/s, windows bit-exact {cpu['gpu_windows_bit_exact']}")
+        log(f"cpu oracle: {cpu['value'] / 1e6:.1f} M cell-updates/s on {cpu['cores']} threads "
+            f"({cpu['single_core']['value'] / 1e6:.1f} M on one), windows bit-exact {cpu['gpu_windows_bit_exact']}")
+    errors = traveltime_errors() if args.errors else None
+    if errors:
+        log(f"errors: {errors}")
 
     out = {
         "metric": "eikonal cell-updates/s & shots/s, batched vs per-shot, at ≤1e-5 rel err; HBM %",
@@ -285,10 +498,15 @@ def main():
                    "nz": w.nz, "nx": w.nx, "shots": len(w.shots), "cell_updates_per_step": cu},
         "shots_per_s": shots_per_s,
         "per_shot_plain_shots_per_s": plain_shots_per_s,
+        "per_shot_plain_spread": plain_spread,
         "per_shot_tiled_shots_per_s": single_shots_per_s,
+        "per_shot_tiled_spread": single_spread,
         "batched_vs_plain_per_shot": shots_per_s / plain_shots_per_s,
+        "batched_vs_plain_per_shot_worst": shots_per_s / plain_spread["max"],
+        "drop_in_per_shot": drop_in,
         "rel_err_vs_oracle": 0.0 if (cpu and cpu["gpu_windows_bit_exact"].split("/")[0] ==
                                     cpu["gpu_windows_bit_exact"].split("/")[1]) else None,
+        "errors": errors,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
