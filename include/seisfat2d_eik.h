@@ -113,6 +113,24 @@ int seisfat_eik_solve(seisfat_eik_t h, int type, int schedule, float sx, float s
                       float* T_padded, const float* rx, const float* rz, int nrec,
                       float* picks);
 
+/* The per-shot solve of seisfat_eik_solve for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Isotropic, on
+ * the whole model (its padding as uploaded), device pointers, asynchronous on
+ * `stream`: d_T receives the padded (nz+6)*(nx+6) traveltimes, d_picks the
+ * nrec Catmull-Rom picks at (d_rx, d_rz), and d_err (nullable) 0, or nonzero
+ * when a sweep's tile dependency timed out (1) or a receiver lay outside the
+ * model (2, its pick NaN).  It allocates nothing: its work buffers live in a
+ * fixed 8 MiB device array per GPU, set up on the first call on that device;
+ * models whose tile queue and exchange words exceed it (about 60 M padded
+ * nodes, fewer for models narrower than a few hundred columns) are refused
+ * with SEISFAT_EINVAL.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (from two
+ * streams or threads) race.  Same traveltimes and picks as seisfat_eik_solve,
+ * bit for bit. */
+int seisfat_eik_solve_noalloc(seisfat_eik_t h, float sx, float sz, float* d_T, const float* d_rx,
+                              const float* d_rz, int nrec, float* d_picks, int* d_err, void* stream);
+
 /* Device pointer of the padded traveltimes of the last seisfat_eik_solve (the
  * reference's public Modeling::d_T, read by migration,
  * src/migration/migration.cu:130,167).  Valid until the next solve/destroy. */

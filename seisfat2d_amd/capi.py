@@ -25,7 +25,7 @@ EXPORTS = [
     "seisfat_batch_num_launches", "seisfat_batch_num_sweep_launches", "seisfat_batch_sweep_ms",
     "seisfat_batch_status", "seisfat_batch_destroy", "seisfat_batch_run_record", "seisfat_batch_vjp_device",
     "seisfat_batch_misfit_gradient", "seisfat_eik_fast_division", "seisfat_eik_set_slowness_padded",
-    "seisfat_eik_device_S", "seisfat_batch_vjp",
+    "seisfat_eik_device_S", "seisfat_eik_solve_noalloc", "seisfat_batch_vjp",
 ]
 
 
@@ -62,6 +62,7 @@ def lib():
         L.seisfat_eik_device_S.argtypes = [vp]
         L.seisfat_eik_set_stiffness.argtypes = [vp, C.POINTER(vp)]
         L.seisfat_eik_solve.argtypes = [vp, i, i, f, f, vp, vp, vp, i, vp]
+        L.seisfat_eik_solve_noalloc.argtypes = [vp, f, f, vp, vp, vp, i, vp, vp, vp]
         L.seisfat_eik_device_T.restype = vp
         L.seisfat_eik_device_T.argtypes = [vp]
         L.seisfat_eik_copy_T.argtypes = [vp, vp]
