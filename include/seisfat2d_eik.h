@@ -171,6 +171,17 @@ int seisfat_batch_num_sweep_launches(seisfat_batch_t b);
 /* Device time in ms of the sweep phase of the last completed run (from HIP
  * events recorded on the run's stream around the sweep launches). */
 int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms);
+/* Converged mode (no reference counterpart: the reference runs exactly one
+ * pass of four sweeps, src/modeling/modeling.cu:130-158): passes repeat until
+ * one leaves every traveltime of every shot unchanged, at most max_passes
+ * (passes receives the count, the unchanged pass included, as the oracle's
+ * converged mode counts them).  Each tile votes whether it lowered a
+ * traveltime (one wave vote, one global word per pass); passes run as HIP
+ * graphs of `chunk` passes (rounded up to even) whose passes after
+ * convergence exit at once, with a host check between chunks; then the
+ * picks.  Isotropic batches of the tiled schedule (one persistent launch per
+ * sweep).  Synchronous on `stream`. */
+int seisfat_batch_run_converged(seisfat_batch_t b, int max_passes, int chunk, int* passes, void* stream);
 /* Wait for the batch's last enqueued work (run, recorded run or
  * vector-Jacobian product; work of other streams is not waited for) and
  * report a failure the device flagged during it (a persistent sweep whose

@@ -25,7 +25,7 @@ EXPORTS = [
     "seisfat_batch_num_launches", "seisfat_batch_num_sweep_launches", "seisfat_batch_sweep_ms",
     "seisfat_batch_status", "seisfat_batch_destroy", "seisfat_batch_run_record", "seisfat_batch_vjp_device",
     "seisfat_batch_misfit_gradient", "seisfat_eik_fast_division", "seisfat_eik_set_slowness_padded",
-    "seisfat_eik_device_S", "seisfat_eik_solve_noalloc", "seisfat_batch_vjp",
+    "seisfat_eik_device_S", "seisfat_eik_solve_noalloc", "seisfat_batch_vjp", "seisfat_batch_run_converged",
 ]
 
 
@@ -84,6 +84,7 @@ def lib():
         L.seisfat_batch_vjp_device.argtypes = [vp, vp, vp, vp]
         L.seisfat_batch_misfit_gradient.argtypes = [vp, vp, vp, vp, vp]
         L.seisfat_batch_vjp.argtypes = [vp, vp, vp]
+        L.seisfat_batch_run_converged.argtypes = [vp, i, i, C.POINTER(i), vp]
         ptr_ret = {"seisfat_last_error", "seisfat_eik_device_T", "seisfat_batch_device_picks",
                    "seisfat_batch_device_T", "seisfat_eik_device_S"}
         for name in EXPORTS:

@@ -113,6 +113,11 @@ void free_batch(seisfat_batch_s* b) {
     if (g) (void)hipGraphExecDestroy(g);
   for (auto& g : b->graph)
     if (g) (void)hipGraphDestroy(g);
+  for (auto& g : b->conv_exec)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& g : b->conv_graph)
+    if (g) (void)hipGraphDestroy(g);
+  dfree(b->d_conv);
   for (auto& ev : b->ev_sweep)
     if (ev) (void)hipEventDestroy(ev);
   if (b->ev_done) (void)hipEventDestroy(b->ev_done);
@@ -672,6 +677,76 @@ int run_batch(seisfat_batch_s* b, hipStream_t st) {
   return SEISFAT_OK;
 }
 
+// Converged mode: passes of the four sweeps until one leaves every
+// traveltime unchanged (at most max_passes).  Three graphs: the first pass
+// (initialisation fused into its first sweep), `chunk` gated passes (each
+// skipped at once after convergence), and the picks; the host reads the skip
+// word after each chunk.
+int run_converged(seisfat_batch_s* b, int max_passes, int chunk, int* passes, hipStream_t st) {
+  seisfat_eik_s* h = b->h;
+  if (b->type != SEISFAT_ISO || b->schedule != SEISFAT_SCHED_TILED)
+    return fail(SEISFAT_EINVAL, "converged mode covers isotropic batches of the tiled schedule");
+  if (max_passes < 1 || max_passes > (1 << 20)) return fail(SEISFAT_EINVAL, "max_passes must be in [1, 2^20]");
+  if (chunk < 2 || chunk > 256) return fail(SEISFAT_EINVAL, "chunk must be in [2, 256]");
+  if (!h->has_s) return fail(SEISFAT_ESTATE, "slowness rejected or not set since the batch was created");
+  chunk += chunk & 1;  // even: the change words alternate by pass parity
+  if (!b->d_conv)
+    if (int rc = dalloc(&b->d_conv, (size_t)CONV_WORDS)) return rc;
+  if (b->conv_chunk != chunk || b->conv_max != max_passes) {
+    for (auto& g : b->conv_exec)
+      if (g) (void)hipGraphExecDestroy(g);
+    for (auto& g : b->conv_graph)
+      if (g) (void)hipGraphDestroy(g);
+    for (int q = 0; q < 3; ++q) {
+      b->conv_exec[q] = nullptr;
+      b->conv_graph[q] = nullptr;
+    }
+    hipStream_t cs = h->stream;
+    const int nzz = h->nzz;
+    for (int part = 0; part < 3; ++part) {
+      HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      if (part == 0) {
+        launch_conv_init(b->d_conv, max_passes, cs);
+        launch_begin_run(b->pr, cs);
+        launch_sweeps_persist(b->d_iso, b->d_tiles, b->ntiles, b->nstrips, nzz, h->kc, h->eqh, h->skew, true, b->pr,
+                              0, cs, b->d_conv + 2, b->d_conv);
+      } else if (part == 1) {
+        for (int q = 0; q < chunk; ++q) {  // passes 1 + q (+ a multiple of chunk)
+          launch_conv_gate(b->d_conv, cs);
+          launch_begin_run(b->pr, cs);
+          launch_sweeps_persist(b->d_iso, b->d_tiles, b->ntiles, b->nstrips, nzz, h->kc, h->eqh, h->skew, false,
+                                b->pr, 0, cs, b->d_conv + 2 + ((1 + q) & 1), b->d_conv);
+        }
+      } else {
+        launch_picks(b->d_iso, b->d_recshot, b->d_rx, b->d_rz, b->d_picks, b->nrec, nzz, h->dx, h->dz, cs);
+        // one more node, so the graph is not empty when the batch has no receivers
+        launch_fill(reinterpret_cast<float*>(b->d_conv + 6), 1, 0.0f, cs);
+      }
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(cs, &g);
+      if (e != hipSuccess) return fail(SEISFAT_EHIP, std::string("converged-mode capture: ") + hipGetErrorString(e));
+      b->conv_graph[part] = g;
+      HIPCHK(hipGraphInstantiate(&b->conv_exec[part], g, nullptr, nullptr, 0));
+    }
+    b->conv_chunk = chunk;
+    b->conv_max = max_passes;
+  }
+  HIPCHK(hipGraphLaunch(b->conv_exec[0], st));
+  int cw[2] = {0, 1};
+  for (int launched = 1; launched < max_passes; launched += chunk) {
+    HIPCHK(hipGraphLaunch(b->conv_exec[1], st));
+    HIPCHK(hipMemcpyAsync(cw, b->d_conv, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (cw[0]) break;
+  }
+  HIPCHK(hipGraphLaunch(b->conv_exec[2], st));
+  HIPCHK(hipMemcpyAsync(cw, b->d_conv, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(b->ev_done, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (passes) *passes = cw[1];
+  return batch_status(b);
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint.  Layout of the per-batch state: see eik_adjoint.h.
 void cubic_weights(double d, double w[4]) {
@@ -1153,6 +1228,11 @@ int seisfat_batch_create(seisfat_eik_t h, int type, int schedule, int nshots, co
 int seisfat_batch_run(seisfat_batch_t b, void* stream) {
   if (!b) return fail(SEISFAT_EINVAL, "null batch");
   return run_batch(b, (hipStream_t)stream);
+}
+
+int seisfat_batch_run_converged(seisfat_batch_t b, int max_passes, int chunk, int* passes, void* stream) {
+  if (!b) return fail(SEISFAT_EINVAL, "null batch");
+  return run_converged(b, max_passes, chunk, passes, (hipStream_t)stream);
 }
 
 int seisfat_batch_picks(seisfat_batch_t b, float* picks) {

@@ -132,4 +132,10 @@ struct seisfat_batch_s {
   // picks; the timing events are recorded on the stream between them.
   hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
   hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  // converged mode: control words, and graphs of the first pass, of `conv_chunk`
+  // gated passes and of the picks, captured for pass limit `conv_max`
+  int* d_conv = nullptr;
+  int conv_chunk = 0, conv_max = 0;
+  hipGraph_t conv_graph[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t conv_exec[3] = {nullptr, nullptr, nullptr};
 };

@@ -59,9 +59,17 @@ int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const in
                         const PersistRun& pr, int solve, hipStream_t st);
 // out[i] = XCC id of workgroup i (i < n)
 void launch_xcc_ids(int* out, int n, hipStream_t st);
+// changed / skip (converged mode, skewed slowness only): every tile that
+// lowers a traveltime sets *changed; the launches exit at once while *skip
+// is set.
 int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, int nstrips, int nzz,
                           const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused,
-                          const PersistRun& pr, int solve, hipStream_t st);
+                          const PersistRun& pr, int solve, hipStream_t st, int* changed = nullptr,
+                          const int* skip = nullptr);
+// Converged mode's control words (see k_conv_init / k_conv_gate).
+constexpr int CONV_WORDS = 8;
+void launch_conv_init(int* cw, int max_passes, hipStream_t st);
+void launch_conv_gate(int* cw, hipStream_t st);
 void launch_begin_run(const PersistRun& pr, hipStream_t st);
 void launch_skew_s(const float* spad, float* sk, int nzz, int nxx, int KP, int NS, int sweep, hipStream_t st);
 // rec_shot null: all receivers belong to shot 0; err (nullable) gets bit 1

@@ -220,11 +220,11 @@ __device__ __forceinline__ int skew_addr(const ShotDev& sd, int nxx, int NC, int
 // caller ahead of time (software pipelining across the solver's branches).
 // ALLCOLS: every column the tile visits is an interior node column, so only
 // the row test remains.
-template <bool EQH, bool FD, bool ALLCOLS, bool XDPP = false>
+template <bool EQH, bool FD, bool ALLCOLS, bool XDPP = false, bool CONV = false>
 __device__ __forceinline__ void tile_step(float* __restrict__ sT, int tab, int t, int lane, bool row_ok, int c0,
                                           int NC,
                                           float told, float h, float szo, float sxo, float& te, float& tev,
-                                          CellQ& sref, const EikConsts& k) {
+                                          CellQ& sref, const EikConsts& k, bool& chg) {
   const CellQ zo = cell_q<EQH, FD>(szo, k);  // this step's z-other cell, next step's reference cell
   const float tv = dpp_shr1(h, te);
   const int c = c0 + t - lane;
@@ -234,6 +234,7 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int tab, int t
   const float dxs_xo = !XDPP ? k.dx * sxo : (EQH ? dpp_shl1(sxo, zo.dzs) : k.dx * dpp_shl1(sxo, szo));
   const float m = eik_local_sel<EQH, FD>(tv, te, tev, sref, zo.dzs, dxs_xo, k, told);  // min(told, candidates)
   const float res = act ? m : told;
+  if constexpr (CONV) chg = chg | (res < told);  // converged mode: did this lane lower a traveltime
   ring_at(sT, tab) = res;
   tev = tv;
   te = res;
@@ -297,10 +298,11 @@ __device__ __forceinline__ void skew_prefetch(SkewPre& p, const SkewStream<SB, F
   p.s0 = gptr(st.sv.base)[skew_addr<SB>(st.sd, st.nxx, st.NC, st.sv.KP, st.kap, st.c0 - st.lane)];
 }
 
-template <int SB, bool EQH, bool FD, bool FAST>
+template <int SB, bool EQH, bool FD, bool FAST, bool CONV = false>
 __device__ __forceinline__ void tile_steps_skew(const SkewStream<SB, FAST>& st, SkewPre& pre, float* __restrict__ sT,
                                                 const float* __restrict__ sHX, int lane, bool row_ok, int c0,
-                                                int NC, float& te, float& tev, CellQ& sref, const EikConsts& k) {
+                                                int NC, float& te, float& tev, CellQ& sref, const EikConsts& k,
+                                                bool& chg) {
   float* zq = pre.zq;
   auto load_z = [&](int t) -> float { return st.z(t); };
   int tab = 4 * trow(0, lane);
@@ -319,7 +321,8 @@ __device__ __forceinline__ void tile_steps_skew(const SkewStream<SB, FAST>& st, 
       hx_n = reinterpret_cast<const float2*>(sHX)[t + 1];
       const float szo = zq[u];
       if constexpr (decltype(fetch)::value) zq[u] = load_z(t + PF);
-      tile_step<EQH, FD, FAST, true>(sT, tab, t, lane, row_ok, c0, NC, told, hx.x, szo, hx.y, te, tev, sref, k);
+      tile_step<EQH, FD, FAST, true, CONV>(sT, tab, t, lane, row_ok, c0, NC, told, hx.x, szo, hx.y, te, tev, sref,
+                                           k, chg);
       tab = tn;
     }
   };
@@ -387,6 +390,10 @@ struct PersistCtl {
   int nstrips;             // strips per shot (NRS)
   unsigned sweep_id;       // 0..7 within a run
   unsigned xtag;           // fused pairs: tag of the partner sweep's done words
+  // converged mode: set to 1 by every tile that lowers a traveltime, and the
+  // word that makes the launch exit at once (a pass after convergence)
+  int* changed;
+  const int* skip;
 };
 
 // One 64x64 skewed tile of a sweep.
@@ -423,8 +430,8 @@ struct PersistCtl {
 // earlier tiles.  Both sweeps of a shot run on one XCD, whose L2 is shared,
 // so completion at L2 is all such a reader needs, and no L1 invalidation
 // (which would also drop the other waves' lines) is.
-template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST, int XMODE, class AfterStage,
-          class BeforeWriteBack>
+template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool PERSIST, int XMODE, bool CONV,
+          class AfterStage, class BeforeWriteBack>
 __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __restrict__ shots, int nzz,
                                              const EikConsts& k, const SkewView& sv, const PersistCtl& ctl,
                                              unsigned tagb, float* __restrict__ sT, float* __restrict__ sS,
@@ -584,14 +591,15 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   // per-step broadcasts: lane t's upwind-row value is step t's; lane j's
   // x-other cell of lane 63 is step 63-j's
   sHX[2 * lane] = hrow;
+  bool chg = false;  // converged mode: this lane lowered a traveltime
   if (SKEW) {
     CellQ sref = cell_q<EQH, FD>(pre.s0, k);
     sHX[2 * (63 - lane) + 1] = EQH ? k.dx * pre.xr : pre.xr;
     wave_lds_sync();
     if (fast)
-      tile_steps_skew<SB, EQH, FD, true>(st_fast, pre, sT, sHX, lane, row_ok, c0, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, FD, true, CONV>(st_fast, pre, sT, sHX, lane, row_ok, c0, NC, te, tev, sref, k, chg);
     else
-      tile_steps_skew<SB, EQH, FD, false>(st_edge, pre, sT, sHX, lane, row_ok, c0, NC, te, tev, sref, k);
+      tile_steps_skew<SB, EQH, FD, false, CONV>(st_edge, pre, sT, sHX, lane, row_ok, c0, NC, te, tev, sref, k, chg);
   } else {
     // Slowness in SS_STEPS-step phases (the next phase's loads in flight
     // while this one runs); lane k's x-other cell is lane k+1's z-other cell
@@ -618,8 +626,8 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
           told_n = ring_at(sT, tn);
           hx_n = reinterpret_cast<const float2*>(sHX)[t + 1];
           z_n = sS[((t + 1) & (SS_STEPS - 1)) * 64 + lane];
-          tile_step<EQH, FD, false, true>(sT, tab, t, lane, row_ok, c0, NC, told, hx.x, szo, hx.y, te, tev, sref,
-                                          k);
+          tile_step<EQH, FD, false, true, CONV>(sT, tab, t, lane, row_ok, c0, NC, told, hx.x, szo, hx.y, te, tev,
+                                                sref, k, chg);
           tab = tn;
         }
       }
@@ -627,6 +635,10 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     }
   }
   wave_lds_sync();
+  // converged mode: one wave vote per tile, one global word per pass
+  if constexpr (CONV) {
+    if (__any(chg) && lane == 0) atomicOr(ctl.changed, 1);
+  }
 
   if (PERSIST) {
     // publish: lane 63's results are strip r+1's upwind row (columns c0+t-63),
@@ -677,7 +689,8 @@ __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ 
   __shared__ float sS[SKEW ? 1 : SS_STEPS * 64];
   __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
   const PersistCtl ctl{};
-  tile_process<SA, SB, EQH, false, SKEW, FIRST, false, 0>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT, sS, sHX,
+  tile_process<SA, SB, EQH, false, SKEW, FIRST, false, 0, false>(tiles[blockIdx.x], shots, nzz, k, sv, ctl, 0u, sT,
+                                                                 sS, sHX,
                                                        [] {}, [] {});
 }
 
@@ -685,13 +698,17 @@ __global__ __launch_bounds__(64) void k_sweep_tiles(const ShotDev* __restrict__ 
 // tile-diagonal order until the sweep's tiles run out.  A tile only waits on
 // tiles with smaller tickets, which running workgroups hold, so the sweep
 // always drains.
-template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST>
+template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool CONV = false>
 __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict__ shots,
                                                       const int4* __restrict__ tiles, int nzz, EikConsts k,
                                                       SkewView sv, PersistCtl ctl) {
   __shared__ float sT[64 * RING_LD];
   __shared__ float sS[SKEW ? 1 : SS_STEPS * 64];
   __shared__ __attribute__((aligned(8))) float sHX[HX_WORDS];
+  // converged mode: a pass after convergence does nothing
+  if constexpr (CONV) {
+    if (__hip_atomic_load(ctl.skip, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  }
   const unsigned tagb = (*ctl.epoch << 11) | (ctl.sweep_id << 8);
   auto claim_issue = [&]() -> int {
     int t = 0;
@@ -714,7 +731,7 @@ __global__ __launch_bounds__(64) void k_sweep_persist(const ShotDev* __restrict_
     // workgroup's tickets increase, so the smallest unfinished ticket is
     // always one a workgroup is running, and its predecessors are done.
     int raw = 0, next = 0;
-    tile_process<SA, SB, EQH, FD, SKEW, FIRST, true, 0>(
+    tile_process<SA, SB, EQH, FD, SKEW, FIRST, true, 0, CONV>(
         tl, shots, nzz, k, sv, ctl, tagb, sT, sS, sHX, [&] { raw = claim_issue(); },
         [&] { next = __builtin_amdgcn_readfirstlane(raw); });
     tk = next;
@@ -760,14 +777,47 @@ __global__ __launch_bounds__(64) void k_sweep_pair(const ShotDev* __restrict__ s
     }
     int raw = 0, next = 0;
     if (tl.w & 4)
-      tile_process<-1, SB, EQH, FD, SKEW, false, true, 2>(
+      tile_process<-1, SB, EQH, FD, SKEW, false, true, 2, false>(
           tl, shots_b, nzz, k, sv_b, cb, tag_b, sT, sS, sHX, [&] { raw = claim_issue(); },
           [&] { next = __builtin_amdgcn_readfirstlane(raw); });
     else
-      tile_process<1, SB, EQH, FD, SKEW, FIRST_A, true, 1>(
+      tile_process<1, SB, EQH, FD, SKEW, FIRST_A, true, 1, false>(
           tl, shots_a, nzz, k, sv_a, ctl_a, tag_a, sT, sS, sHX, [&] { raw = claim_issue(); },
           [&] { next = __builtin_amdgcn_readfirstlane(raw); });
     tk = next;
+  }
+}
+
+// Converged mode's control words: [0] skip (set once a pass left every
+// traveltime unchanged, or the pass limit is reached), [1] passes run,
+// [2], [3] the change words of even and odd passes, [4] index of the next
+// pass, [5] the pass limit.  Start of a run: pass 0 is about to run.
+__global__ void k_conv_init(int* __restrict__ cw, int max_passes) {
+  if (threadIdx.x == 0) {
+    cw[0] = 0;
+    cw[1] = 1;
+    cw[2] = 0;
+    cw[3] = 0;
+    cw[4] = 1;
+    cw[5] = max_passes;
+  }
+}
+
+// Before pass p = cw[4]: skip it (and every later one) if pass p-1 lowered
+// no traveltime or the limit is reached; otherwise count it and clear its
+// change word.
+__global__ void k_conv_gate(int* __restrict__ cw) {
+  if (threadIdx.x == 0) {
+    const int p = cw[4];
+    if (!cw[0]) {
+      if (cw[1] >= cw[5] || cw[2 + ((p - 1) & 1)] == 0) {
+        cw[0] = 1;
+      } else {
+        cw[2 + (p & 1)] = 0;
+        cw[1] += 1;
+      }
+    }
+    cw[4] = p + 1;
   }
 }
 
@@ -1023,29 +1073,39 @@ int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh
   return launches;
 }
 
-template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST>
+template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool CONV = false>
 static void persist_launch(const ShotDev* shots, const int4* tiles, int nzz, const EikConsts& k,
                            const SkewView& sv, const PersistCtl& ctl, hipStream_t st) {
   static int grid = 0;
   if (grid == 0) {
     int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persist<SA, SB, EQH, FD, SKEW, FIRST>, 64,
-                                                     0) != hipSuccess || per_cu <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persist<SA, SB, EQH, FD, SKEW, FIRST, CONV>,
+                                                     64, 0) != hipSuccess || per_cu <= 0)
       per_cu = 4;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
     grid = per_cu * cus;
   }
-  hipLaunchKernelGGL((k_sweep_persist<SA, SB, EQH, FD, SKEW, FIRST>), dim3(std::min(grid, ctl.ntiles)), dim3(64), 0,
-                     st, shots, tiles, nzz, k, sv, ctl);
+  hipLaunchKernelGGL((k_sweep_persist<SA, SB, EQH, FD, SKEW, FIRST, CONV>), dim3(std::min(grid, ctl.ntiles)),
+                     dim3(64), 0, st, shots, tiles, nzz, k, sv, ctl);
 }
 
 template <int SA, int SB, bool EQH, bool SKEW, bool FIRST>
 static void tiles_diag(const ShotDev* shots, const int4* tiles, int n, int nzz, const EikConsts& k,
                        const SkewView& sv, const PersistCtl* ctl, hipStream_t st) {
   if (ctl) {
-    // the fdiv variant exists for the skewed-slowness (isotropic batch) path
+    // the fdiv variant exists for the skewed-slowness (isotropic batch) path,
+    // and so does converged mode (a change word in the control block)
+    if constexpr (SKEW) {
+      if (ctl->changed) {
+        if (k.fastdiv)
+          persist_launch<SA, SB, EQH, true, true, FIRST, true>(shots, tiles, nzz, k, sv, *ctl, st);
+        else
+          persist_launch<SA, SB, EQH, false, true, FIRST, true>(shots, tiles, nzz, k, sv, *ctl, st);
+        return;
+      }
+    }
     if (SKEW && k.fastdiv)
       persist_launch<SA, SB, EQH, SKEW, SKEW, FIRST>(shots, tiles, nzz, k, sv, *ctl, st);
     else
@@ -1103,9 +1163,11 @@ int launch_sweeps_tiled(const ShotDev* shots, const int4* tiles, const std::vect
 
 int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, int nstrips, int nzz,
                           const EikConsts& k, bool eqh, const SkewView* skew, bool first_fused,
-                          const PersistRun& pr, int solve, hipStream_t st) {
+                          const PersistRun& pr, int solve, hipStream_t st, int* changed, const int* skip) {
   for (int sweep = 0; sweep < 4; ++sweep) {
-    PersistCtl ctl;
+    PersistCtl ctl{};
+    ctl.changed = changed;
+    ctl.skip = skip;
     ctl.counter = pr.counters + solve * 4 + sweep;
     ctl.epoch = pr.epoch;
     ctl.err = pr.err;
@@ -1168,7 +1230,7 @@ int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const in
                         const PersistRun& pr, int solve, hipStream_t st) {
   const SkewView none{nullptr, 0};
   for (int p = 0; p < 2; ++p) {
-    PersistCtl ca, cb;
+    PersistCtl ca{}, cb{};
     ca.counter = pr.counters + PAIR_COUNTERS + 16 * solve + 8 * p;
     ca.epoch = cb.epoch = pr.epoch;
     ca.err = cb.err = pr.err;
@@ -1193,6 +1255,12 @@ int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const in
 void launch_xcc_ids(int* out, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_xcc_ids, dim3(n), dim3(64), 0, st, out);
 }
+
+void launch_conv_init(int* cw, int max_passes, hipStream_t st) {
+  hipLaunchKernelGGL(k_conv_init, dim3(1), dim3(64), 0, st, cw, max_passes);
+}
+
+void launch_conv_gate(int* cw, hipStream_t st) { hipLaunchKernelGGL(k_conv_gate, dim3(1), dim3(64), 0, st, cw); }
 
 void launch_begin_run(const PersistRun& pr, hipStream_t st) {
   hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(64), 0, st, pr.counters, NUM_COUNTERS, pr.epoch);

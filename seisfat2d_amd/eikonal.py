@@ -156,6 +156,14 @@ class Batch:
     def run(self, stream=None):
         check(lib().seisfat_batch_run(self._b, None if stream is None else C.c_void_p(stream)))
 
+    def run_converged(self, max_passes=100, chunk=8, stream=None):
+        """Passes until one changes nothing (at most max_passes); returns the
+        number of passes (synchronous)."""
+        n = C.c_int()
+        check(lib().seisfat_batch_run_converged(self._b, int(max_passes), int(chunk), C.byref(n),
+                                                None if stream is None else C.c_void_p(stream)))
+        return n.value
+
     def picks(self):
         out = np.empty(self.nrec, np.float32)
         check(lib().seisfat_batch_picks(self._b, capi._ptr(out)))
