@@ -485,6 +485,39 @@ def test_ani_full_size_c4_shot(oracle):
     e.close()
 
 
+def test_ani_c4_benched_batch_all_128_shots_bit_exact(oracle):
+    """The benchmarked anisotropic batch itself: 128 full-model TTI shots on
+    801x3001, every shot's traveltimes against the oracle's
+    Eikonal_ANI::time_propagation (shots solved on host threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+    w = models.c4()
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    e.set_stiffness(w.stiffness)
+    b = e.batch(w.shots, w.rx, w.rz, type=ANI, schedule=SCHED_TILED)
+    b.run()
+    picks = b.picks()
+    b.status()
+    S = oracle.expand_boundary(w.s)
+    Cq, mn, mx = [], [], []
+    for c in w.stiffness:
+        u, a, bb = oracle.compress(oracle.expand_boundary(c))
+        Cq.append(u)
+        mn.append(a)
+        mx.append(bb)
+
+    def ref(k):
+        return oracle.solve_ani(S, w.dx, w.dz, w.shots[k][0], 0.0, Cq, mn, mx)[0]
+
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        for k, Tr in enumerate(ex.map(ref, range(len(w.shots)))):
+            assert np.array_equal(b.T(k), Tr), k
+            _, _, _, _, ro, nr = w.shots[k]
+            assert np.array_equal(picks[ro:ro + nr], oracle.picks(Tr, w.dx, w.dz, w.rx[ro:ro + nr], w.rz[ro:ro + nr])), k
+    b.close()
+    e.close()
+
+
 def test_invalid_inputs_rejected():
     e = Eikonal(20, 30, 10.0, 10.0)
     s = np.full((30, 20), 5e-4, np.float32)
