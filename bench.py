@@ -274,6 +274,43 @@ def drop_in_per_shot(nshots=10):
     return out
 
 
+def bench_tables(ntables=64):
+    """Traveltime tables for migration (Migration::set_src_travel_times,
+    src/migration/migration.cu:116-134): padded grids of whole-model solves
+    on the 1001 x 4001 model from `ntables` surface positions, as one batch;
+    tables/s with the tables left in HBM, and with each copied to host memory
+    (the reference writes every table to a file from host memory)."""
+    import numpy as np
+    import torch
+    from seisfat2d_amd import SCHED_TILED, Eikonal, models
+    w = models.c2()
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    xs = np.linspace(0.0, (w.nx - 1) * w.dx, ntables)
+    b = e.batch([(float(x), 0.0, 0, w.nx, 0, 0) for x in xs], schedule=SCHED_TILED)
+    b.run()
+    b.status()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        b.run()
+    torch.cuda.synchronize()
+    t_dev = (time.perf_counter() - t0) / reps
+    b.status()
+    host = np.empty((w.nx + 6, w.nz + 6), np.float32)
+    t0 = time.perf_counter()
+    b.run()
+    for k in range(ntables):
+        host[...] = b.T(k)
+    t_host = time.perf_counter() - t0
+    b.close()
+    e.close()
+    return {"workload": f"{ntables} traveltime tables (padded {w.nz + 6}x{w.nx + 6} grids, one per surface position) "
+                        f"on the whole {w.nz}x{w.nx} model, one batch",
+            "tables_per_s": ntables / t_dev, "tables_per_s_with_host_copy": ntables / t_host}
+
+
 def traveltime_errors():
     """Max relative traveltime error over interior nodes (T_ref > 0) of the GPU
     solve against the FP32 oracle (the reference's CPU path restated; the
@@ -459,6 +496,8 @@ def main():
     single_shots_per_s, single_spread = per_shot(SCHED_TILED)
     log(f"per-shot: plain {plain_shots_per_s:.1f} shots/s ({plain_spread}), tiled {single_shots_per_s:.1f} shots/s")
     drop_in = drop_in_per_shot()
+    tables = bench_tables()
+    log(f"tables: {tables['tables_per_s']:.0f}/s in HBM, {tables['tables_per_s_with_host_copy']:.0f}/s copied to host")
     log(f"drop-in per-shot driver (whole C2 model): {drop_in['shots_per_s']:.1f} shots/s, "
         f"{drop_in['shots_per_s_with_T_copy']:.1f} with the T copy")
 
@@ -504,6 +543,7 @@ def main():
         "batched_vs_plain_per_shot": shots_per_s / plain_shots_per_s,
         "batched_vs_plain_per_shot_worst": shots_per_s / plain_spread["max"],
         "drop_in_per_shot": drop_in,
+        "traveltime_tables": tables,
         "rel_err_vs_oracle": 0.0 if (cpu and cpu["gpu_windows_bit_exact"].split("/")[0] ==
                                     cpu["gpu_windows_bit_exact"].split("/")[1]) else None,
         "errors": errors,
