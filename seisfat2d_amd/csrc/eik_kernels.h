@@ -47,6 +47,7 @@ struct PersistRun {
   int* counters;
   unsigned* epoch;
   int* err;
+  int wpcu = 0;  // persistent workgroups per CU (0 = as many as fit)
 };
 // Both sweeps of each pair (0,1) and (2,3) in one launch each, shots
 // partitioned over the 8 XCDs: ptiles holds each XCD's tiles of both sweeps

@@ -255,7 +255,9 @@ __device__ __forceinline__ void tile_step(float* __restrict__ sT, int tab, int t
 // Per-step slowness addressing of a tile in the skewed copy.  FAST: the
 // z-other cell of step t sits at column sig0 + SB*t, a uniform base plus the
 // lane's row (raw buffer loads: per-lane byte offset + per-step uniform
-// offset); otherwise each lane computes its clamped address.
+// offset, from a buffer resource based at the tile's lowest column, so the
+// 32-bit offsets stay within 64 columns whatever the size of the copy);
+// otherwise each lane computes its clamped address.
 template <int SB, bool FAST>
 struct SkewStream {
   const ShotDev& sd;
@@ -267,11 +269,16 @@ struct SkewStream {
       : sd(sd_), sv(sv_), kap(kap_), lane(lane_), c0(c0_), nxx(nxx_), NC(NC_) {
     const int sig0 = (SB > 0) ? sd.x0 + c0 + 1 : sd.x0 + sd.w + 66 - c0;
     colbase0 = (sig0 + SK_OFF) * sv.KP;
-    rs = __builtin_amdgcn_make_buffer_rsrc((void*)sv.base, 0, 0x7fffffff, 0x00020000);
+    if (FAST) {
+      const size_t clo = (size_t)max(colbase0 - (SB > 0 ? 0 : 63 * sv.KP), 0);  // (edge tiles never load)
+      rs = __builtin_amdgcn_make_buffer_rsrc((void*)(sv.base + clo), 0, 0x7fffffff, 0x00020000);
+    }
   }
   __device__ __forceinline__ float z(int t) const {
+    // column of step t relative to the resource's base column
     if (FAST)
-      return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, kap * 4, (colbase0 + SB * t * sv.KP) * 4, 0));
+      return __uint_as_float(
+          __builtin_amdgcn_raw_buffer_load_b32(rs, kap * 4, ((SB > 0) ? t : 63 - t) * sv.KP * 4, 0));
     return gptr(sv.base)[skew_addr<SB>(sd, nxx, NC, sv.KP, kap, c0 + t - lane + 1)];
   }
   // lane 63's x-other cells (row kap63+1 = next strip, one skew period away),
@@ -394,6 +401,8 @@ struct PersistCtl {
   // word that makes the launch exit at once (a pass after convergence)
   int* changed;
   const int* skip;
+  // host side: persistent workgroups per CU (0 = as many as fit)
+  int wpcu;
 };
 
 // One 64x64 skewed tile of a sweep.
@@ -1076,17 +1085,17 @@ int launch_sweeps_plain(const ShotDev& sd, int nzz, const EikConsts& k, bool eqh
 template <int SA, int SB, bool EQH, bool FD, bool SKEW, bool FIRST, bool CONV = false>
 static void persist_launch(const ShotDev* shots, const int4* tiles, int nzz, const EikConsts& k,
                            const SkewView& sv, const PersistCtl& ctl, hipStream_t st) {
-  static int grid = 0;
-  if (grid == 0) {
-    int per_cu = 0, dev = 0, cus = 0;
+  static int per_cu = 0, cus = 0;
+  if (per_cu == 0) {
+    int dev = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_persist<SA, SB, EQH, FD, SKEW, FIRST, CONV>,
                                                      64, 0) != hipSuccess || per_cu <= 0)
       per_cu = 4;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
-    grid = per_cu * cus;
   }
+  const int grid = (ctl.wpcu > 0 ? std::min(ctl.wpcu, per_cu) : per_cu) * cus;
   hipLaunchKernelGGL((k_sweep_persist<SA, SB, EQH, FD, SKEW, FIRST, CONV>), dim3(std::min(grid, ctl.ntiles)),
                      dim3(64), 0, st, shots, tiles, nzz, k, sv, ctl);
 }
@@ -1168,6 +1177,7 @@ int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, i
     PersistCtl ctl{};
     ctl.changed = changed;
     ctl.skip = skip;
+    ctl.wpcu = pr.wpcu;
     ctl.counter = pr.counters + solve * 4 + sweep;
     ctl.epoch = pr.epoch;
     ctl.err = pr.err;
@@ -1183,9 +1193,9 @@ template <int SB, bool EQH, bool FD, bool FIRST_A, bool SKEW>
 static void pair_launch(const ShotDev* shots_a, const ShotDev* shots_b, const int4* tiles, const int* qoff,
                         int nzz, const EikConsts& k, const SkewView& sv_a, const SkewView& sv_b,
                         const PersistCtl& ca, const PersistCtl& cb, hipStream_t st) {
-  static int grid = 0;
-  if (grid == 0) {
-    int per_cu = 0, dev = 0, cus = 0;
+  static int per_cu = 0, cus = 0;
+  if (per_cu == 0) {
+    int dev = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sweep_pair<SB, EQH, FD, FIRST_A, SKEW>, 64, 0) !=
             hipSuccess ||
         per_cu <= 0)
@@ -1193,8 +1203,8 @@ static void pair_launch(const ShotDev* shots_a, const ShotDev* shots_b, const in
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
-    grid = per_cu * cus;
   }
+  const int grid = (ca.wpcu > 0 ? std::min(ca.wpcu, per_cu) : per_cu) * cus;
   hipLaunchKernelGGL((k_sweep_pair<SB, EQH, FD, FIRST_A, SKEW>), dim3(grid), dim3(64), 0, st, shots_a, shots_b, tiles,
                      qoff, nzz, k, sv_a, sv_b, ca, cb);
 }
@@ -1231,6 +1241,7 @@ int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const in
   const SkewView none{nullptr, 0};
   for (int p = 0; p < 2; ++p) {
     PersistCtl ca{}, cb{};
+    ca.wpcu = cb.wpcu = pr.wpcu;
     ca.counter = pr.counters + PAIR_COUNTERS + 16 * solve + 8 * p;
     ca.epoch = cb.epoch = pr.epoch;
     ca.err = cb.err = pr.err;

@@ -485,6 +485,39 @@ def test_ani_full_size_c4_shot(oracle):
     e.close()
 
 
+def test_window_at_the_far_end_of_a_model_with_skewed_copies_over_2_gib(oracle, monkeypatch):
+    """A 8000 x 70000 model, whose skewed slowness copies (one per sweep
+    orientation) each exceed 2^31 bytes: a narrow window at the model's right
+    edge reads them at their largest offsets (beyond a 32-bit byte offset from
+    the copy's start), bit-exact against the oracle with the fused sweep pairs
+    and with one persistent launch per sweep."""
+    nz, nx, dh = 8000, 70000, 10.0
+    nzz, nxx = nz + 6, nx + 6
+    KP = 64 * ((nzz - 2 + 63) // 64) + 64
+    NS = nxx + 2 * 160
+    assert NS * KP * 4 > 2 ** 31
+    z = np.arange(nz, dtype=np.float32) * dh
+    x = np.arange(nx, dtype=np.float32) * dh
+    s = np.empty((nx, nz), np.float32)
+    for j0 in range(0, nx, 10000):          # 1/v = 1/(1500 + 0.4 z + 300 sin(x / 2 km)), in column blocks
+        xs = x[j0:j0 + 10000, None]
+        s[j0:j0 + 10000] = (1.0 / (1500.0 + 0.4 * z[None, :] + 300.0 * np.sin(xs / 2000.0))).astype(np.float32)
+    w, x0 = 300, nx - 300
+    sx, sz = float(np.float32((x0 + 123) * dh)), 2500.0
+    Tr = oracle.solve_windows(s, dh, dh, [sx], [sz], [x0], [w], nthreads=1)[0]
+    for pairs in ("1", "0"):
+        monkeypatch.setenv("SEISFAT_SWEEP_PAIRS", pairs)
+        e = Eikonal(nz, nx, dh, dh)
+        e.set_slowness(s)
+        b = e.batch([(sx, sz, x0, w, 0, 0)], schedule=SCHED_TILED)
+        b.run()
+        b.status()
+        T = b.T(0)
+        b.close()
+        e.close()
+        assert np.array_equal(T, Tr), (pairs, np.abs(T - Tr).max())
+
+
 def test_ani_c4_benched_batch_all_128_shots_bit_exact(oracle):
     """The benchmarked anisotropic batch itself: 128 full-model TTI shots on
     801x3001, every shot's traveltimes against the oracle's
