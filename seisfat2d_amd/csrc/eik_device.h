@@ -249,9 +249,15 @@ __device__ __forceinline__ float eik_local_sel(float tv, float te, float tev, co
   const float de = te - tev;
   const float dv = tv - tev;
   // non-short-circuit conjunctions: plain mask arithmetic, no control flow
-  const bool cA = (tv <= te + r.dxs) & (te <= tv + r.dzs) & (de >= 0.0f) & (dv >= 0.0f);
-  const bool cB = (de <= r.thrB) & (de > 0.0f);
-  const bool cC = (dv <= r.thrC) & (dv > 0.0f);
+  // Branch A's four conditions as one test: for finite floats a, b,
+  // RN(a - b) >= 0 exactly when a >= b (a difference of floats rounds to
+  // zero only when it is zero), and a minimum is >= 0 exactly when every
+  // operand is.  B and C: with thr > 0 finite, 0 < d <= thr exactly when
+  // bits(d) - 1 < bits(thr) as unsigned words (positive floats order as their
+  // bit patterns; +0, -0 and negatives wrap to at least 0x7fffffff).
+  const bool cA = vmin3(te + r.dxs - tv, tv + r.dzs - te, vmin(de, dv)) >= 0.0f;
+  const bool cB = __float_as_uint(de) - 1u < __float_as_uint(r.thrB);
+  const bool cC = __float_as_uint(dv) - 1u < __float_as_uint(r.thrC);
   // branch B precedes C (the reference's else-if order)
   const float dd = cB ? de : dv;
   const float h1 = EQH ? k.dz : (cB ? k.dz : k.dx);
