@@ -171,6 +171,13 @@ int seisfat_batch_num_sweep_launches(seisfat_batch_t b);
 /* Device time in ms of the sweep phase of the last completed run (from HIP
  * events recorded on the run's stream around the sweep launches). */
 int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms);
+/* Tiles of the last completed run (or converged-mode pass) of the persistent
+ * tiled schedule: *total = the tiles of all its sweeps, *skipped = those whose
+ * every input (old traveltimes, upwind row, carry-in) was the "not reached"
+ * value 1e6, which the geometry provably keeps at 1e6, so the kernel did not
+ * run their steps (results unchanged).  Diagnostics with no reference
+ * counterpart; other schedules report SEISFAT_EINVAL. */
+int seisfat_batch_tile_counts(seisfat_batch_t b, long long* total, long long* skipped);
 /* Converged mode (no reference counterpart: the reference runs exactly one
  * pass of four sweeps, src/modeling/modeling.cu:130-158): passes repeat until
  * one leaves every traveltime of every shot unchanged, at most max_passes

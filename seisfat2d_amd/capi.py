@@ -26,6 +26,7 @@ EXPORTS = [
     "seisfat_batch_status", "seisfat_batch_destroy", "seisfat_batch_run_record", "seisfat_batch_vjp_device",
     "seisfat_batch_misfit_gradient", "seisfat_eik_fast_division", "seisfat_eik_set_slowness_padded",
     "seisfat_eik_device_S", "seisfat_eik_solve_noalloc", "seisfat_batch_vjp", "seisfat_batch_run_converged",
+    "seisfat_batch_tile_counts",
 ]
 
 
@@ -78,6 +79,7 @@ def lib():
         L.seisfat_batch_num_launches.argtypes = [vp]
         L.seisfat_batch_num_sweep_launches.argtypes = [vp]
         L.seisfat_batch_sweep_ms.argtypes = [vp, C.POINTER(f)]
+        L.seisfat_batch_tile_counts.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         L.seisfat_batch_destroy.argtypes = [vp]
         L.seisfat_batch_status.argtypes = [vp]
         L.seisfat_batch_run_record.argtypes = [vp, vp]

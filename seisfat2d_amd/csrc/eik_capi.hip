@@ -104,6 +104,8 @@ EikConsts make_consts(float dx, float dz) {
   two_word(k.hyp, k.zh_hyp, k.zl_hyp);
   two_word(k.dz, k.zh_dz, k.zl_dz);
   two_word(k.dx, k.zh_dx, k.zl_dx);
+  const float a = TINF * k.dz2i + TINF * k.dx2i;  // RN(RN(.) + RN(.)): no contraction in this build
+  k.tinf_fixed = (a / k.sum) >= TINF;
   return k;
 }
 
@@ -1281,6 +1283,19 @@ int seisfat_batch_sweep_ms(seisfat_batch_t b, float* ms) {
 int seisfat_batch_status(seisfat_batch_t b) {
   if (!b) return fail(SEISFAT_EINVAL, "null batch");
   return batch_status(b);
+}
+
+int seisfat_batch_tile_counts(seisfat_batch_t b, long long* total, long long* skipped) {
+  if (!b || !total || !skipped) return fail(SEISFAT_EINVAL, "null batch or output");
+  if (!b->d_ctl) return fail(SEISFAT_EINVAL, "tile counts exist for the persistent tiled schedule only");
+  const long long solves = b->type == SEISFAT_ANI ? 2 : 1;
+  *total = solves * (b->pairs ? 2LL * (long long)b->pair_tiles.size() : 4LL * b->ntiles);
+  int n = 0;
+  HIPCHK(hipEventSynchronize(b->ev_done));
+  HIPCHK(hipMemcpyAsync(&n, b->d_ctl + CTL_SKIP, sizeof(int), hipMemcpyDeviceToHost, b->h->stream));
+  HIPCHK(hipStreamSynchronize(b->h->stream));
+  *skipped = n;
+  return SEISFAT_OK;
 }
 
 int seisfat_batch_run_record(seisfat_batch_t b, void* stream) {

@@ -35,6 +35,12 @@ struct EikConsts {
   // host-side: fdiv reproduces the correctly rounded quotient for all four
   // divisors (checked exhaustively when the geometry is created)
   int fastdiv;
+  // a node whose old traveltime and upwind neighbours are all TINF keeps
+  // TINF: its 1-D candidates are >= TINF, branches B and C are not taken
+  // (te - tev = tv - tev = 0) and branch A's value is at least
+  // RN(RN(RN(TINF dz2i) + RN(TINF dx2i)) / sum), which this geometry keeps
+  // >= TINF (set when the geometry is made)
+  int tinf_fixed;
 };
 
 // ---------------------------------------------------------------------------

@@ -42,7 +42,9 @@ int launch_sweeps_tiled(const ShotDev* shots, const int4* tiles, const std::vect
 // run epoch and the spin-limit error flag.
 constexpr int PAIR_COUNTERS = 8;   // first fused-pair counter
 constexpr int NUM_COUNTERS = 40;   // counters zeroed at the start of a run
-constexpr int CTL_EPOCH = 40, CTL_ERR = 41, CTL_WORDS = 48;  // control block layout (ints)
+// control block layout (ints): epoch, error flag, tiles skipped by the
+// last run (all inputs TINF)
+constexpr int CTL_EPOCH = 40, CTL_ERR = 41, CTL_SKIP = 42, CTL_WORDS = 48;
 struct PersistRun {
   int* counters;
   unsigned* epoch;

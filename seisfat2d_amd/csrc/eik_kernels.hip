@@ -401,6 +401,8 @@ struct PersistCtl {
   // word that makes the launch exit at once (a pass after convergence)
   int* changed;
   const int* skip;
+  // tiles not run because all their inputs are TINF (counted when set)
+  int* nskip;
   // host side: persistent workgroups per CU (0 = as many as fit)
   int wpcu;
 };
@@ -462,6 +464,10 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   const int jb = (SB > 0) ? 1 + c0 : nxx - 2 - c0;  // grid column of local column c0
   constexpr int STAGE_CPOL = XMODE == 2 ? 16 : 0;   // cache policy of the staging loads: sc1 (agent scope)
 
+  // old traveltimes known to be all TINF: a first sweep's tile without the
+  // source stencil, or (pairs) a second-sweep tile whose first-sweep tiles
+  // beneath all skipped (done word value 1)
+  bool told_tinf = false;
   if (XMODE == 2) {
     // The partner sweep runs the rows the other way, columns the same: this
     // tile's local rows kk1 are its rows NR-1-kk1 (strips r0, at most two),
@@ -487,6 +493,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     const XDep deps[4] = {{fp, need, ctl.xtag}, {fp, false, 0u}, {fp, false, 0u}, {fp, false, 0u}};
     float dv[4];
     xwait4(deps, dv, ctl.err);
+    told_tinf = __all(!need || dv[0] == 1.0f);
   }
 
   // ---- old traveltimes, step-major --------------------------------------
@@ -516,6 +523,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
       }
       sT[trow(t, lane)] = val;
     }
+    told_tinf = !has_src;
   } else {
     // Column loads straight into LDS: instruction v fetches, for lane k, the
     // node of step (v+k)&63, column c0-k+((v+k)&63) = c0+v or c0+v-64 (two
@@ -597,11 +605,21 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
 
   if (!FIRST) __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0), lgkmcnt(0): the column loads have landed in LDS
 
-  // per-step broadcasts: lane t's upwind-row value is step t's; lane j's
-  // x-other cell of lane 63 is step 63-j's
-  sHX[2 * lane] = hrow;
+  // A tile whose inputs are all TINF (old traveltimes, upwind row, carry-in)
+  // leaves every node at TINF where the geometry keeps such a neighbourhood
+  // at TINF (k.tinf_fixed): its steps are not run, as in a first sweep the
+  // region the front has not reached.  A skipped second-sweep tile also
+  // leaves T as it is; a first sweep writes its TINF values back.
+  const bool skip = k.tinf_fixed && (FIRST || XMODE == 2) && told_tinf &&
+                    __all((hrow == TINF) & (te == TINF) & (tev == TINF));
+  if (skip && ctl.nskip && lane == 0) atomicAdd(ctl.nskip, 1);
+
   bool chg = false;  // converged mode: this lane lowered a traveltime
-  if (SKEW) {
+  if (skip) {
+  } else if (SKEW) {
+    // per-step broadcasts: lane t's upwind-row value is step t's; lane j's
+    // x-other cell of lane 63 is step 63-j's
+    sHX[2 * lane] = hrow;
     CellQ sref = cell_q<EQH, FD>(pre.s0, k);
     sHX[2 * (63 - lane) + 1] = EQH ? k.dx * pre.xr : pre.xr;
     wave_lds_sync();
@@ -614,6 +632,7 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
     // while this one runs); lane k's x-other cell is lane k+1's z-other cell
     // of the same step (one DPP move), lane 63's comes from the broadcasts.
     constexpr int NPH = 64 / SS_STEPS;
+    sHX[2 * lane] = hrow;
     CellQ sref = cell_q<EQH, FD>(cell_at<SA, SB>(sd, nzz, kgc, min(max(c0 - lane, 0), NC)), k);
     ss_store(sv_, sS, lane);
     sHX[2 * lane + 1] = EQH ? k.dx * sx63 : sx63;
@@ -664,7 +683,9 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
 
   before_wb();
   // write-back, row v of LDS = the nodes of column loads v
-  if (interior) {
+  if (skip && XMODE == 2) {
+    // T already holds these TINF values
+  } else if (interior) {
     const int irow = (SA > 0) ? 1 + kgc : nzz - 2 - kgc;
     const unsigned o0 = (unsigned)(irow + (SB > 0 ? 64 * nzz : 0));
     const unsigned o1 = (unsigned)(irow + (SB > 0 ? 0 : 64 * nzz));
@@ -685,7 +706,8 @@ __device__ __forceinline__ void tile_process(const int4 tl, const ShotDev* __res
   }
   if (XMODE == 1) {
     __builtin_amdgcn_s_waitcnt(0);  // the write-back stores have reached L2
-    if (lane == 0) xstore(sd.xd + (size_t)r * sd.nb + bb, 0.0f, tagb);
+    // value 1: every node of the tile is TINF (it was skipped)
+    if (lane == 0) xstore(sd.xd + (size_t)r * sd.nb + bb, skip ? 1.0f : 0.0f, tagb);
   }
 }
 
@@ -841,7 +863,10 @@ __global__ void k_xcc_ids(int* __restrict__ out) {
 __global__ void k_begin_run(int* __restrict__ counters, int n, unsigned* __restrict__ epoch) {
   const int i = threadIdx.x;
   if (i < n) counters[i] = 0;
-  if (i == 0) *epoch = (*epoch + 1) % 2097151u;
+  if (i == 0) {
+    *epoch = (*epoch + 1) % 2097151u;
+    counters[CTL_SKIP] = 0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1178,6 +1203,7 @@ int launch_sweeps_persist(const ShotDev* shots, const int4* tiles, int ntiles, i
     ctl.changed = changed;
     ctl.skip = skip;
     ctl.wpcu = pr.wpcu;
+    ctl.nskip = pr.counters + CTL_SKIP;
     ctl.counter = pr.counters + solve * 4 + sweep;
     ctl.epoch = pr.epoch;
     ctl.err = pr.err;
@@ -1242,6 +1268,7 @@ int launch_sweeps_pairs(const ShotDev* shots_a, const ShotDev* shots_b, const in
   for (int p = 0; p < 2; ++p) {
     PersistCtl ca{}, cb{};
     ca.wpcu = cb.wpcu = pr.wpcu;
+    ca.nskip = cb.nskip = pr.counters + CTL_SKIP;
     ca.counter = pr.counters + PAIR_COUNTERS + 16 * solve + 8 * p;
     ca.epoch = cb.epoch = pr.epoch;
     ca.err = cb.err = pr.err;

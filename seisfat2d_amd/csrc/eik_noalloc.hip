@@ -71,6 +71,7 @@ __global__ void k_single_setup(ShotDev sd, ShotDev* __restrict__ d_shot, int4* _
     unsigned* ep = reinterpret_cast<unsigned*>(ctl + CTL_EPOCH);
     *ep = (*ep + 1) % 2097151u;
     ctl[CTL_ERR] = 0;
+    ctl[CTL_SKIP] = 0;
   }
 }
 

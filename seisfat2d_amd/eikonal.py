@@ -198,6 +198,14 @@ class Batch:
         check(lib().seisfat_batch_sweep_ms(self._b, C.byref(ms)))
         return ms.value
 
+    def tile_counts(self):
+        """(total, skipped) tiles of the last completed run of the persistent
+        tiled schedule; skipped tiles had only "not reached" (1e6) inputs and
+        provably leave their nodes at 1e6, so their steps did not run."""
+        total, skipped = C.c_longlong(), C.c_longlong()
+        check(lib().seisfat_batch_tile_counts(self._b, C.byref(total), C.byref(skipped)))
+        return total.value, skipped.value
+
     def status(self):
         """Synchronise and raise if the device flagged a failure during the runs."""
         check(lib().seisfat_batch_status(self._b))

@@ -485,6 +485,56 @@ def test_ani_full_size_c4_shot(oracle):
     e.close()
 
 
+def tinf_fixed(dx, dz):
+    """Whether the geometry keeps an all-1e6 neighbourhood at 1e6: branch A's
+    value there is at least RN(RN(RN(1e6/dz^2) + RN(1e6/dx^2)) / sum)."""
+    f = np.float32
+    dz2i, dx2i = f(1) / (f(dz) * f(dz)), f(1) / (f(dx) * f(dx))
+    a = f(1e6) * dz2i + f(1e6) * dx2i
+    return bool(a / (dz2i + dx2i) >= f(1e6))
+
+
+@pytest.mark.parametrize("dh", [5.0, 7.0])
+@pytest.mark.parametrize("pairs", ["1", "0"])
+def test_unreached_tiles_skipped_only_where_the_geometry_keeps_1e6(oracle, monkeypatch, dh, pairs):
+    """Tiles whose every input is 1e6 ("not reached": in a first sweep the
+    region against the front's direction) are not run where the geometry
+    provably keeps such nodes at 1e6 (dh = 5 m), and run where it does not
+    (dh = 7 m: RN(RN(RN(1e6/dh^2) * 2) / (2/dh^2)) = 999999.94, so the
+    reference's own arithmetic lowers them); every window bit-exact against
+    the oracle either way, with the fused sweep pairs (whose second sweep also
+    skips above skipped first-sweep tiles) and with one launch per sweep."""
+    assert tinf_fixed(5.0, 5.0) and not tinf_fixed(7.0, 7.0)
+    monkeypatch.setenv("SEISFAT_SWEEP_PAIRS", pairs)
+    rng = np.random.default_rng(31)
+    nz, nx = 300, 700
+    s = smooth_slowness(rng, nz, nx, dh)
+    shots = []
+    for k in range(6):
+        w = int(rng.integers(200, nx + 1))
+        x0 = int(rng.integers(0, nx - w + 1))
+        shots.append((float(np.float32((x0 + rng.uniform(0.3, 0.7) * (w - 1)) * dh)),
+                      float(np.float32(rng.uniform(0.0, 0.5) * (nz - 1) * dh)), x0, w, 0, 0))
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    b = e.batch(shots, schedule=SCHED_TILED)
+    Trefs = oracle.solve_windows(s, dh, dh, [x[0] for x in shots], [x[1] for x in shots],
+                                 [x[2] for x in shots], [x[3] for x in shots], nthreads=16)
+    for run in range(2):
+        b.run()
+        b.status()
+        total, skipped = b.tile_counts()
+        print(f"dh={dh} pairs={pairs}: {skipped} of {total} tiles skipped")
+        if tinf_fixed(dh, dh):
+            assert skipped > 0.05 * total
+        else:
+            assert skipped == 0
+        for k, Tr in enumerate(Trefs):
+            assert np.array_equal(b.T(k), Tr), (run, k)
+    b.close()
+    e.close()
+
+
 def test_window_at_the_far_end_of_a_model_with_skewed_copies_over_2_gib(oracle, monkeypatch):
     """A 8000 x 70000 model, whose skewed slowness copies (one per sweep
     orientation) each exceed 2^31 bytes: a narrow window at the model's right
