@@ -401,6 +401,10 @@ def main():
                                                           "the FP32 and FP64 oracle and against t = r/v")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU oracle budget (0 = skip)")
     ap.add_argument("--extras", type=int, default=1, help="also time the anisotropic and adjoint workloads")
+    ap.add_argument("--tune", type=int, default=1, help="tune the batch's schedule options on this device first "
+                                                        "(seisfat2d_amd.tune, not cached)")
+    ap.add_argument("--opts", type=str, default=None,
+                    help='schedule options as JSON (e.g. \'{"wpcu": 8}\'); no tuning')
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: the batched steps only, plus a 256 MiB device copy that calibrates "
                          "per-dispatch byte counters")
@@ -409,6 +413,7 @@ def main():
     args = ap.parse_args()
     if args.profile:
         args.plain_shots, args.cpu_seconds = 0, 0.0
+        args.tune = 0  # a profile holds the timed kernels only
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(shots would shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
@@ -432,8 +437,20 @@ def main():
     cu = models.cell_updates(w.nz, widths)
     e = Eikonal(w.nz, w.nx, w.dz, w.dx)
     e.set_slowness(w.s)
-    b = e.batch(w.shots, w.rx, w.rz, schedule=SCHED_TILED)
-    log(f"batch: {len(w.shots)} shots, {b.num_launches} launches/step ({b.num_sweep_launches} sweep)")
+    tuned = None
+    if args.opts:
+        opts = json.loads(args.opts)
+    elif args.tune:
+        from seisfat2d_amd import tune
+        tuned = tune.tune(e, w.shots, w.rx, w.rz, force=True, cache=False)
+        opts = tuned["opts"]
+        log(f"tuned schedule {opts} ({len(tuned['trials'])} candidates, best sweep phase {tuned['ms']:.2f} ms)")
+    else:
+        opts = None
+    b = e.batch(w.shots, w.rx, w.rz, schedule=SCHED_TILED, opts=opts)
+    sched = b.opts
+    log(f"batch: {len(w.shots)} shots, {b.num_launches} launches/step ({b.num_sweep_launches} sweep), "
+        f"schedule {sched}")
 
     for _ in range(args.warmup):
         b.run()
@@ -445,6 +462,7 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     b.status()  # the timed runs completed without a device-side failure
+    tiles_total, tiles_skipped = b.tile_counts()
     ms_step = dt / args.steps * 1e3
     value = cu * args.steps / dt
     shots_per_s = len(w.shots) * args.steps / dt
@@ -533,8 +551,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"batched iso FSM, {len(w.shots)} shots on windows 1200-4001 wide of the "
                                f"1001x4001 dh=5 m layered model (seeded), one pass of 4 sweeps, tiled "
-                               f"schedule ({sweep_kernel} launches), one HIP graph per step",
-                   "nz": w.nz, "nx": w.nx, "shots": len(w.shots), "cell_updates_per_step": cu},
+                               f"schedule ({sweep_kernel} launches) on 64x64 tiles with schedule options "
+                               f"{json.dumps(sched, sort_keys=True)}"
+                               f"{' (tuned on this device)' if tuned else ''}, one HIP graph per step",
+                   "nz": w.nz, "nx": w.nx, "shots": len(w.shots), "cell_updates_per_step": cu,
+                   "schedule": sched},
+        "schedule_tuning": ({"trials": tuned["trials"], "best_sweep_ms": tuned["ms"]} if tuned else None),
+        "tiles_per_step": tiles_total,
+        "tiles_skipped_per_step": tiles_skipped,
         "shots_per_s": shots_per_s,
         "per_shot_plain_shots_per_s": plain_shots_per_s,
         "per_shot_plain_spread": plain_spread,
@@ -549,6 +573,9 @@ def main():
         "errors": errors,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     # the same over the tiles the kernels ran (tiles with only
+                     # not-reached inputs are not run where exact)
+                     "frac_run_tiles": achieved / HBM_PEAK_GBS * (1.0 - tiles_skipped / max(tiles_total, 1)),
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT),
                      # the binding resource: SQ_ACTIVE_INST_VALU over the sweep kernels
                      "valu_busy": profile_figure("valu_busy"),

@@ -154,6 +154,26 @@ int seisfat_eik_fast_division(seisfat_eik_t h);
 int seisfat_batch_create(seisfat_eik_t h, int type, int schedule, int nshots,
                          const seisfat_shot* shots, int nrec_total, const float* rx,
                          const float* rz, seisfat_batch_t* out);
+/* Schedule options of a batch of the persistent tiled schedule (no reference
+ * counterpart).  Every combination gives bit-identical traveltimes; they
+ * trade launch shape against load balance, and are what the package's
+ * schedule tuner (seisfat2d_amd.tune) searches per workload.
+ * seisfat_batch_create uses the defaults. */
+typedef struct seisfat_sched_opts {
+  int pairs;    /* 1 (default): both sweeps of each pair of sweeps in one launch where the device runs
+                   workgroups round robin over 8 XCDs; 0: one persistent launch per sweep */
+  int wpcu;     /* persistent workgroups per CU; 0 (default): as many as fit */
+  int skip;     /* 1 (default): tiles whose every input is the not-reached value 1e6 are not run where
+                   the geometry provably keeps them at 1e6; 0: every tile runs */
+  int priority; /* 1 (default): wave priorities from each shot's scheduling slack; 0: all equal */
+} seisfat_sched_opts;
+/* seisfat_batch_create with schedule options (null opts: the defaults). */
+int seisfat_batch_create_opts(seisfat_eik_t h, int type, int schedule, int nshots,
+                              const seisfat_shot* shots, int nrec_total, const float* rx,
+                              const float* rz, const seisfat_sched_opts* opts, seisfat_batch_t* out);
+/* The options a batch runs with (pairs reads 0 where the fused pairs are
+ * unavailable). */
+int seisfat_batch_get_opts(seisfat_batch_t b, seisfat_sched_opts* opts);
 /* Run the batch asynchronously on `stream` (graph replay for the tiled schedule). */
 int seisfat_batch_run(seisfat_batch_t b, void* stream);
 /* Wait for the batch and copy all nrec_total picks to host memory. */

@@ -26,7 +26,7 @@ EXPORTS = [
     "seisfat_batch_status", "seisfat_batch_destroy", "seisfat_batch_run_record", "seisfat_batch_vjp_device",
     "seisfat_batch_misfit_gradient", "seisfat_eik_fast_division", "seisfat_eik_set_slowness_padded",
     "seisfat_eik_device_S", "seisfat_eik_solve_noalloc", "seisfat_batch_vjp", "seisfat_batch_run_converged",
-    "seisfat_batch_tile_counts",
+    "seisfat_batch_tile_counts", "seisfat_batch_create_opts", "seisfat_batch_get_opts",
 ]
 
 
@@ -34,6 +34,25 @@ class SeisfatError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"seisfat error {code}: {msg}")
         self.code = code
+
+
+class SchedOpts(C.Structure):
+    """seisfat_sched_opts: schedule options of a tiled batch (bit-identical results)."""
+    _fields_ = [("pairs", C.c_int32), ("wpcu", C.c_int32), ("skip", C.c_int32), ("priority", C.c_int32)]
+
+    FIELDS = ("pairs", "wpcu", "skip", "priority")
+
+    @classmethod
+    def of(cls, d):
+        o = cls(1, 0, 1, 1)
+        for k, v in (d or {}).items():
+            if k not in cls.FIELDS:
+                raise ValueError(f"unknown schedule option {k!r} (known: {', '.join(cls.FIELDS)})")
+            setattr(o, k, int(v))
+        return o
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k in self.FIELDS}
 
 
 class Shot(C.Structure):
@@ -69,6 +88,9 @@ def lib():
         L.seisfat_eik_copy_T.argtypes = [vp, vp]
         L.seisfat_eik_fast_division.argtypes = [vp]
         L.seisfat_batch_create.argtypes = [vp, i, i, i, C.POINTER(Shot), i, vp, vp, C.POINTER(vp)]
+        L.seisfat_batch_create_opts.argtypes = [vp, i, i, i, C.POINTER(Shot), i, vp, vp, C.POINTER(SchedOpts),
+                                                C.POINTER(vp)]
+        L.seisfat_batch_get_opts.argtypes = [vp, C.POINTER(SchedOpts)]
         L.seisfat_batch_run.argtypes = [vp, vp]
         L.seisfat_batch_picks.argtypes = [vp, vp]
         L.seisfat_batch_device_picks.restype = vp

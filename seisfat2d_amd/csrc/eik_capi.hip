@@ -180,13 +180,13 @@ int enqueue_solve(seisfat_batch_s* b, hipStream_t st, const Mark& mark) {
       if (int rc = mark(0)) return rc;
     int n = 0;
     if (b->schedule == SEISFAT_SCHED_TILED && b->pairs && d_tab_b) {
-      n = launch_sweeps_pairs(d_tab, d_tab_b, b->d_ptiles, b->d_qoff, b->nstrips, nzz, h->kc, h->eqh, skew, b->pr,
+      n = launch_sweeps_pairs(d_tab, d_tab_b, b->d_ptiles, b->d_qoff, b->nstrips, nzz, b->kc, h->eqh, skew, b->pr,
                               solve, st);
     } else if (b->schedule == SEISFAT_SCHED_TILED) {
-      n = launch_sweeps_persist(d_tab, b->d_tiles, b->ntiles, b->nstrips, nzz, h->kc, h->eqh, skew, true, b->pr,
+      n = launch_sweeps_persist(d_tab, b->d_tiles, b->ntiles, b->nstrips, nzz, b->kc, h->eqh, skew, true, b->pr,
                                 solve, st);
     } else if (b->schedule == SEISFAT_SCHED_TILED_DIAG) {
-      n = launch_sweeps_tiled(d_tab, b->d_tiles, b->diag_off, nzz, h->kc, h->eqh, skew, true, st);
+      n = launch_sweeps_tiled(d_tab, b->d_tiles, b->diag_off, nzz, b->kc, h->eqh, skew, true, st);
     } else {
       for (int k = 0; k < b->nshots; ++k) n += launch_sweeps_plain(tab[k], nzz, h->kc, h->eqh, st);
     }
@@ -248,7 +248,7 @@ void build_tiles(seisfat_batch_s* b, std::vector<int4>& tiles) {
   std::vector<int> prio(b->nshots);
   for (int k = 0; k < b->nshots; ++k) {
     const int slack = Dmax - (2 * (NRS - 1) + NBs[k]);
-    prio[k] = slack <= 6 ? 3 : slack <= 16 ? 2 : slack <= 30 ? 1 : 0;
+    prio[k] = !b->opts.priority ? 0 : slack <= 6 ? 3 : slack <= 16 ? 2 : slack <= 30 ? 1 : 0;
   }
   b->diag_off.assign(1, 0);
   tiles.clear();
@@ -282,7 +282,7 @@ void build_pair_tiles(seisfat_batch_s* b, std::vector<int4>& pt, std::vector<int
   }
   for (int k = 0; k < n; ++k) {
     const int slack = Dmax - (2 * (NRS - 1) + NBs[k]);
-    prio[k] = slack <= 6 ? 3 : slack <= 16 ? 2 : slack <= 30 ? 1 : 0;
+    prio[k] = !b->opts.priority ? 0 : slack <= 6 ? 3 : slack <= 16 ? 2 : slack <= 30 ? 1 : 0;
     order[k] = k;
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return NBs[a] > NBs[c]; });
@@ -403,8 +403,13 @@ int validate_shots(const seisfat_eik_s* h, int type, int nshots, const seisfat_s
 }
 
 int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seisfat_shot* shots,
-                 int nrec_total, const float* rx, const float* rz, bool use_graph, seisfat_batch_s** out) {
+                 int nrec_total, const float* rx, const float* rz, bool use_graph, seisfat_batch_s** out,
+                 const seisfat_sched_opts* opts = nullptr) {
   *out = nullptr;
+  const seisfat_sched_opts o = opts ? *opts : seisfat_sched_opts{1, 0, 1, 1};
+  if ((o.pairs != 0 && o.pairs != 1) || (o.skip != 0 && o.skip != 1) || (o.priority != 0 && o.priority != 1) ||
+      o.wpcu < 0 || o.wpcu > 64)
+    return fail(SEISFAT_EINVAL, "schedule options: pairs, skip and priority take 0 or 1, wpcu 0..64");
   if (type != SEISFAT_ISO && type != SEISFAT_ANI) return fail(SEISFAT_EINVAL, "type must be ISO (0) or ANI (1)");
   if (schedule != SEISFAT_SCHED_PLAIN && schedule != SEISFAT_SCHED_TILED && schedule != SEISFAT_SCHED_TILED_DIAG)
     return fail(SEISFAT_EINVAL, "schedule must be PLAIN (0), TILED (1) or TILED_DIAG (2)");
@@ -422,6 +427,9 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   b->h = h;
   b->type = type;
   b->schedule = schedule;
+  b->opts = o;
+  b->kc = h->kc;
+  b->kc.tinf_fixed = h->kc.tinf_fixed && o.skip;
   b->nshots = nshots;
   b->nrec = nrec_total;
   b->shots.assign(shots, shots + nshots);
@@ -508,7 +516,7 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
   }
   if (schedule == SEISFAT_SCHED_TILED) {
     TRY(check_xcc(h));
-    b->pairs = h->xcc_ok == 1;
+    b->pairs = h->xcc_ok == 1 && b->opts.pairs;
     // exchange words per shot: bottom row of each strip + double-buffered
     // carries (a second set for the fused pairs' second sweeps), then the
     // pairs' done words (one per tile)
@@ -572,6 +580,7 @@ int create_batch(seisfat_eik_t h, int type, int schedule, int nshots, const seis
       }
     }
     b->pr.counters = b->d_ctl;
+    b->pr.wpcu = o.wpcu;
     b->pr.epoch = (unsigned*)(b->d_ctl + CTL_EPOCH);
     b->pr.err = b->d_ctl + CTL_ERR;
     // all-ones words carry a tag no run ever uses; epoch and flags start at 0
@@ -710,13 +719,13 @@ int run_converged(seisfat_batch_s* b, int max_passes, int chunk, int* passes, hi
       if (part == 0) {
         launch_conv_init(b->d_conv, max_passes, cs);
         launch_begin_run(b->pr, cs);
-        launch_sweeps_persist(b->d_iso, b->d_tiles, b->ntiles, b->nstrips, nzz, h->kc, h->eqh, h->skew, true, b->pr,
+        launch_sweeps_persist(b->d_iso, b->d_tiles, b->ntiles, b->nstrips, nzz, b->kc, h->eqh, h->skew, true, b->pr,
                               0, cs, b->d_conv + 2, b->d_conv);
       } else if (part == 1) {
         for (int q = 0; q < chunk; ++q) {  // passes 1 + q (+ a multiple of chunk)
           launch_conv_gate(b->d_conv, cs);
           launch_begin_run(b->pr, cs);
-          launch_sweeps_persist(b->d_iso, b->d_tiles, b->ntiles, b->nstrips, nzz, h->kc, h->eqh, h->skew, false,
+          launch_sweeps_persist(b->d_iso, b->d_tiles, b->ntiles, b->nstrips, nzz, b->kc, h->eqh, h->skew, false,
                                 b->pr, 0, cs, b->d_conv + 2 + ((1 + q) & 1), b->d_conv);
         }
       } else {
@@ -1225,6 +1234,22 @@ int seisfat_batch_create(seisfat_eik_t h, int type, int schedule, int nshots, co
   *out = nullptr;
   if (int rc = check_handle(h)) return rc;
   return create_batch(h, type, schedule, nshots, shots, nrec_total, rx, rz, /*use_graph=*/true, out);
+}
+
+int seisfat_batch_create_opts(seisfat_eik_t h, int type, int schedule, int nshots, const seisfat_shot* shots,
+                              int nrec_total, const float* rx, const float* rz, const seisfat_sched_opts* opts,
+                              seisfat_batch_t* out) {
+  if (!out) return fail(SEISFAT_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (int rc = check_handle(h)) return rc;
+  return create_batch(h, type, schedule, nshots, shots, nrec_total, rx, rz, /*use_graph=*/true, out, opts);
+}
+
+int seisfat_batch_get_opts(seisfat_batch_t b, seisfat_sched_opts* opts) {
+  if (!b || !opts) return fail(SEISFAT_EINVAL, "null batch or output");
+  *opts = b->opts;
+  opts->pairs = b->pairs ? 1 : 0;
+  return SEISFAT_OK;
 }
 
 int seisfat_batch_run(seisfat_batch_t b, void* stream) {

@@ -88,6 +88,8 @@ struct AdjState {
 struct seisfat_batch_s {
   seisfat_eik_s* h = nullptr;
   int type = SEISFAT_ISO, schedule = SEISFAT_SCHED_TILED;
+  seisfat_sched_opts opts{1, 0, 1, 1};
+  EikConsts kc{};  // the geometry's constants with the skip option applied
   int nshots = 0, nrec = 0;
   std::vector<seisfat_shot> shots;
   std::vector<ShotDev> iso, qp;  // host copies of the shot tables

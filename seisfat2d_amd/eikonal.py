@@ -95,8 +95,17 @@ class Eikonal:
             check(rc)
         return rc == 1
 
-    def batch(self, shots, rx=None, rz=None, type=ISO, schedule=SCHED_TILED):
-        return Batch(self, shots, rx, rz, type, schedule)
+    def batch(self, shots, rx=None, rz=None, type=ISO, schedule=SCHED_TILED, opts=None):
+        """A batch of shots.  opts: schedule options of the tiled schedule
+        (pairs, wpcu, skip, priority; bit-identical results), a dict or
+        "tuned" for the options seisfat2d_amd.tune found for this workload
+        (measured now and cached when the cache has none)."""
+        if isinstance(opts, str):
+            if opts != "tuned":
+                raise ValueError('opts must be a dict, None or "tuned"')
+            from . import tune
+            opts = tune.tune(self, shots, rx, rz, type=type)["opts"]
+        return Batch(self, shots, rx, rz, type, schedule, opts)
 
     def traveltime_tables(self, x, z, type=ISO, schedule=SCHED_TILED, chunk=64):
         """Padded traveltime grids (n, nx+6, nz+6) of whole-model solves from
@@ -138,7 +147,7 @@ class Batch:
     shots: sequence of (sx, sz, x0, width, rec_offset, nrec) tuples.
     """
 
-    def __init__(self, eik: Eikonal, shots, rx, rz, type=ISO, schedule=SCHED_TILED):
+    def __init__(self, eik: Eikonal, shots, rx, rz, type=ISO, schedule=SCHED_TILED, opts=None):
         self.eik = eik
         n = len(shots)
         arr = (Shot * n)(*[Shot(float(s[0]), float(s[1]), int(s[2]), int(s[3]), int(s[4]), int(s[5]))
@@ -148,8 +157,9 @@ class Batch:
         self.rz = np.ascontiguousarray(rz if rz is not None else np.zeros(0), np.float32)
         self.nrec = len(self.rx)
         b = C.c_void_p()
-        check(lib().seisfat_batch_create(eik._h, type, schedule, n, arr, self.nrec,
-                                         capi._ptr(self.rx), capi._ptr(self.rz), C.byref(b)))
+        so = capi.SchedOpts.of(opts)
+        check(lib().seisfat_batch_create_opts(eik._h, type, schedule, n, arr, self.nrec, capi._ptr(self.rx),
+                                              capi._ptr(self.rz), C.byref(so), C.byref(b)))
         self._b = b
         self.records = 0  # recorded runs so far; the adjoint linearises the latest one
 
@@ -197,6 +207,14 @@ class Batch:
         ms = C.c_float()
         check(lib().seisfat_batch_sweep_ms(self._b, C.byref(ms)))
         return ms.value
+
+    @property
+    def opts(self):
+        """The schedule options the batch runs with (pairs reads 0 where the
+        fused pairs are unavailable)."""
+        so = capi.SchedOpts()
+        check(lib().seisfat_batch_get_opts(self._b, C.byref(so)))
+        return so.as_dict()
 
     def tile_counts(self):
         """(total, skipped) tiles of the last completed run of the persistent
