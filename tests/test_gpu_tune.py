@@ -77,3 +77,41 @@ def test_tuning_run_caches_and_serves_the_options(oracle, tmp_path, monkeypatch)
         assert np.array_equal(b.T(k), Tr), k
     b.close()
     e.close()
+
+
+def test_every_option_combination_bit_exact_anisotropic(oracle):
+    """The qP solves (slowness staged per shot, both solves through the same
+    schedule) under every option combination, against the oracle's
+    Eikonal_ANI::time_propagation."""
+    from seisfat2d_amd import ANI
+    from seisfat2d_amd.stiffness import compute_stiffness
+    nz, nx, dh = 70, 110, 5.0
+    rng = np.random.default_rng(23)
+    vp = rng.uniform(1500.0, 4500.0, (nx, nz))
+    cs = compute_stiffness(vp, vp / np.sqrt(3), 1000.0, rng.uniform(0.0, 0.3, (nx, nz)),
+                           rng.uniform(-0.1, 0.2, (nx, nz)), rng.uniform(-30.0, 30.0, (nx, nz)))
+    s = np.sqrt(1000.0 / cs[3]).astype(np.float32)
+    cs = [c.astype(np.float32) for c in cs]
+    S = oracle.expand_boundary(s)
+    Cq, mn, mx = [], [], []
+    for c in cs:
+        u, a, b_ = oracle.compress(oracle.expand_boundary(c))
+        Cq.append(u)
+        mn.append(a)
+        mx.append(b_)
+    src = [(float(np.float32(f * (nx - 1) * dh)), float(np.float32(g * (nz - 1) * dh)))
+           for f, g in ((0.37, 0.2), (0.9, 0.0), (0.05, 0.6))]
+    Trefs = [oracle.solve_ani(S, dh, dh, x, z, Cq, mn, mx)[0] for x, z in src]
+    e = Eikonal(nz, nx, dh, dh)
+    e.set_slowness(s)
+    e.set_stiffness(cs)
+    for pairs, wpcu, skip, prio in itertools.product((1, 0), (0, 3), (1, 0), (1, 0)):
+        opts = {"pairs": pairs, "wpcu": wpcu, "skip": skip, "priority": prio}
+        b = e.batch([(x, z, 0, nx, 0, 0) for x, z in src], type=ANI, schedule=SCHED_TILED, opts=opts)
+        assert b.opts == opts
+        b.run()
+        b.status()
+        for k, Tr in enumerate(Trefs):
+            assert np.array_equal(b.T(k), Tr), (opts, k)
+        b.close()
+    e.close()
