@@ -39,6 +39,15 @@
 //                                  FP64 single-pass forward
 //   batch_misfit ................. FP64 central finite differences of its own
 //                                  J over window shots (pins the window fold)
+//   ricker ....................... closed form of the Ricker wavelet at its
+//                                  zeros and peak; the reference's sample count
+//   trace_convolve ............... numpy FFT product of the reference's
+//                                  method (unnormalised, nfft, shift)
+//   kdm_trace .................... adjointness <L m, d> = <m, L^T d> in both
+//                                  domains; isochron support of a spike
+//                                  trace; weights against an FP64 evaluation
+//                                  of the published factors; angle bins
+//                                  against the angle between the gradients
 //
 // Reference files restated (paths relative to the SeisFAT2D repository):
 //   src/modeling/modeling.cu, src/modeling/eikonal_iso.cu,
@@ -775,6 +784,130 @@ void smooth_model(const float* in, float* out, int nz, int nx, float stdv, int s
     for (int i = 0; i < nz; ++i) out[i + (size_t)j * nz] = sm[(i + samples) + (size_t)(j + samples) * anz];
 }
 
+// ---------------------------------------------------------------------------
+// Kirchhoff depth migration (src/migration/migration.cu): the Ricker wavelet of
+// Migration::set_wavelet (:54-73); the wavelet convolutions of
+// forward_convolution / adjoint_convolution (:197-261), whose FFT product
+// (unnormalised, size nfft) is the linear (cross-)correlation summed here
+// directly, times nfft; and one trace of the four Kirchhoff kernels
+// (:398-756), every node's weight from 9-point differences of both tables as
+// the kernels evaluate it, FP32, nodes in index order.
+constexpr float KDM_EPS = 1e-6f;  // migration.cuh:7
+
+void ricker(float dt, float fmax, std::vector<float>& w) {
+  const float t0 = 2.0f * std::sqrt((float)M_PI) / fmax;
+  const float fc = fmax / (3.0f * std::sqrt((float)M_PI));
+  const int nw = 2 * ((int)((t0 - 0.5f * dt) / dt));
+  w.assign(std::max(nw, 0), 0.0f);
+  for (int k = 0; k < nw; ++k) {
+    const float td = k * dt - t0;
+    const float arg = (float)(M_PI * M_PI * M_PI * fc * fc * td * td);  // a double expression in the reference
+    w[k] = (1.0f - 2.0f * arg) * std::exp(-arg);
+  }
+}
+
+long long nextpow2_ll(long long n) {
+  long long p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+void trace_convolve(int adjoint, int ntr, int nt, const float* w, int nw, const float* in, float* out) {
+  const double nfft = (double)nextpow2_ll((long long)nt + nw - 1);
+  const int shift = nw / 2 + nw / 10;
+  for (int t = 0; t < ntr; ++t) {
+    const float* d = in + (size_t)t * nt;
+    float* o = out + (size_t)t * nt;
+    for (int k = 0; k < nt; ++k) {
+      double acc = 0.0;
+      if (adjoint) {
+        if (k < shift) {
+          o[k] = 0.0f;
+          continue;
+        }
+        const int k0 = k - shift;
+        for (int m = 0; m <= std::min(nw - 1, nt - 1 - k0); ++m) acc += (double)w[m] * (double)d[k0 + m];
+      } else {
+        const int kk = k + shift;
+        for (int m = std::max(0, kk - nw + 1); m <= std::min(kk, nt - 1); ++m) acc += (double)d[m] * (double)w[kk - m];
+      }
+      o[k] = (float)(acc * nfft);
+    }
+  }
+}
+
+// float -> int rounding down as the GPU converts (NaN -> 0, saturating).
+int f2i_rd(float v) {
+  if (v != v) return 0;
+  if (v >= 2147483648.0f) return 2147483647;
+  if (v < -2147483648.0f) return (-2147483647 - 1);
+  return (int)std::floor(v);
+}
+
+void kdm_trace(int domain, int adjoint, const float* S, const float* Ts, const float* Tr, float* data, float* model,
+               float dt, int nt, float dx, float dz, int nxx, int nzz, float aperture, float cmp, int cmpId, float da,
+               int na) {
+  const int nz = nzz - 2 * NB;
+  const float tan_ap = std::tan((float)((double)aperture * M_PI / 180.0));
+  const float scale = (float)(1.0 / (2.0 * M_PI));
+  for (int j = NB; j < nxx - NB; ++j)
+    for (int i = NB; i < nzz - NB; ++i) {
+      const size_t c = i + (size_t)j * nzz;
+      const float z = (float)(i - NB) * dz, x = (float)(j - NB) * dx;
+      const float T = Ts[c] + Tr[c];
+      const int tId = f2i_rd(T / dt);
+      if (tId < 0 || tId >= nt) continue;
+      auto at = [&](const float* A, int ii, int jj) { return A[ii + (size_t)jj * nzz]; };
+      const float dTs_dz = 0.5f * (at(Ts, i + 1, j) - at(Ts, i - 1, j)) / dz;
+      const float dTs_dx = 0.5f * (at(Ts, i, j + 1) - at(Ts, i, j - 1)) / dx;
+      const float d2Ts_dz2 = (at(Ts, i + 1, j) - 2.0f * at(Ts, i, j) + at(Ts, i - 1, j)) / (dz * dz);
+      const float d2Ts_dx2 = (at(Ts, i, j + 1) - 2.0f * at(Ts, i, j) + at(Ts, i, j - 1)) / (dx * dx);
+      const float d2Ts_dxdz = (at(Ts, i + 1, j + 1) - at(Ts, i - 1, j + 1) - at(Ts, i + 1, j - 1) +
+                               at(Ts, i - 1, j - 1)) / (4.0f * dx * dz);
+      const float dTr_dz = 0.5f * (at(Tr, i + 1, j) - at(Tr, i - 1, j)) / dz;
+      const float dTr_dx = 0.5f * (at(Tr, i, j + 1) - at(Tr, i, j - 1)) / dx;
+      const float d2Tr_dz2 = (at(Tr, i + 1, j) - 2.0f * at(Tr, i, j) + at(Tr, i - 1, j)) / (dz * dz);
+      const float d2Tr_dx2 = (at(Tr, i, j + 1) - 2.0f * at(Tr, i, j) + at(Tr, i, j - 1)) / (dx * dx);
+      const float d2Tr_dxdz = (at(Tr, i + 1, j + 1) - at(Tr, i - 1, j + 1) - at(Tr, i + 1, j - 1) +
+                               at(Tr, i - 1, j - 1)) / (4.0f * dx * dz);
+      const float norm_Ts = std::sqrt(dTs_dx * dTs_dx + dTs_dz * dTs_dz) + KDM_EPS;
+      const float norm_Tr = std::sqrt(dTr_dx * dTr_dx + dTr_dz * dTr_dz) + KDM_EPS;
+      const float detHs = std::fabs(d2Ts_dx2 * d2Ts_dz2 - d2Ts_dxdz * d2Ts_dxdz) + KDM_EPS;
+      const float detHr = std::fabs(d2Tr_dx2 * d2Tr_dz2 - d2Tr_dxdz * d2Tr_dxdz) + KDM_EPS;
+      const float Jterm = std::sqrt(detHs * detHr) * S[c] / (norm_Ts * norm_Tr);
+      const float ux_s = dTs_dx / norm_Ts, uz_s = dTs_dz / norm_Ts;
+      const float ux_r = dTr_dx / norm_Tr, uz_r = dTr_dz / norm_Tr;
+      const float nx_n = 0.0f, nz_n = -1.0f;  // surface normal
+      const float cos_s = std::fabs(ux_s * nx_n + uz_s * nz_n);
+      const float cos_r = std::fabs(ux_r * nx_n + uz_r * nz_n);
+      const float obliquity = std::sqrt(std::fmax(0.0f, cos_s) * std::fmax(0.0f, cos_r));
+      const float R_s = std::fmax(Ts[c] / S[c], KDM_EPS);
+      const float R_r = std::fmax(Tr[c] / S[c], KDM_EPS);
+      const float G = 1.0f / std::sqrt(R_s * R_r);
+      const float theta_s = std::acos(std::fmin(1.0f, std::fmax(-1.0f, ux_s * nx_n + uz_s * nz_n)));
+      const float theta_r = std::acos(std::fmin(1.0f, std::fmax(-1.0f, ux_r * nx_n + uz_r * nz_n)));
+      const float theta = 0.5f * (theta_s + theta_r);
+      const float Rf = 1.0f + 0.2f * std::cos(theta);
+      const float sigma = tan_ap * z;
+      const float taper = std::exp(-0.5f * ((x - cmp) / (sigma + KDM_EPS)) * ((x - cmp) / (sigma + KDM_EPS)));
+      const float weights = scale * Jterm * obliquity * G * Rf * taper;
+      size_t mId;
+      if (domain == 0) {
+        mId = (size_t)(i - NB) + (size_t)(j - NB) * nz;
+      } else {
+        const float ra = 0.5f * std::acos((dTs_dx * dTr_dx + dTs_dz * dTr_dz) / (norm_Ts * norm_Tr));
+        const float ang = (float)((double)(180.0f * ra) / M_PI);
+        const int aId = f2i_rd(ang / da);
+        if (aId < 0 || aId >= na) continue;
+        mId = (size_t)(i - NB) + (size_t)aId * nz + (size_t)cmpId * na * nz;
+      }
+      if (adjoint)
+        model[mId] += weights * data[tId];
+      else
+        data[tId] += weights * model[mId];
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -914,4 +1047,23 @@ int orc_ray_paths(const float* T, int nz, int nx, float dx, float dz, float sx, 
   return ray_paths(T, nz, nx, dx, dz, sx, sz, rx, rz, nrec, max_steps, out_rec, out_cell, out_len, cap);
 }
 
+int orc_ricker(float dt, float fmax, float* out, int cap) {
+  std::vector<float> w;
+  ricker(dt, fmax, w);
+  if ((int)w.size() <= cap) std::copy(w.begin(), w.end(), out);
+  return (int)w.size();
+}
+void orc_trace_convolve(int adjoint, int ntr, int nt, const float* w, int nw, const float* in, float* out) {
+  trace_convolve(adjoint, ntr, nt, w, nw, in, out);
+}
+// Traces in order, each the reference's per-trace kernel launch: data is
+// ntr x nt, tables are consecutive padded grids.
+void orc_kdm(int domain, int adjoint, const float* S, const float* src_tables, const float* rec_tables, int ntr,
+             const int* isrc, const int* irec, const float* cmp, const int* cmp_id, float* data, float* model,
+             int nt, float dt, float dx, float dz, int nzz, int nxx, float aperture, float da, int na) {
+  const size_t m = (size_t)nzz * nxx;
+  for (int t = 0; t < ntr; ++t)
+    kdm_trace(domain, adjoint, S, src_tables + isrc[t] * m, rec_tables + irec[t] * m, data + (size_t)t * nt, model,
+              dt, nt, dx, dz, nxx, nzz, aperture, cmp[t], cmp_id ? cmp_id[t] : 0, da, na);
+}
 }  // extern "C"

@@ -67,6 +67,10 @@ def lib():
                                          f32p, C.c_void_p, C.c_void_p, i]),
             "orc_batch_misfit_f64": (d, [f64p, i, i, d, d, i, f64p, f64p, i32p, i32p, i32p, i32p, f64p, f64p,
                                          f64p, C.c_void_p, C.c_void_p, i]),
+            "orc_ricker": (i, [f, f, f32p, i]),
+            "orc_trace_convolve": (None, [i, i, i, f32p, i, f32p, f32p]),
+            "orc_kdm": (None, [i, i, f32p, f32p, f32p, i, i32p, i32p, f32p, C.c_void_p, f32p, f32p, i, f, f, f,
+                               i, i, f, f, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -283,3 +287,110 @@ def smooth_model(m, stdv, samples):
     out = np.empty_like(m)
     lib().orc_smooth_model(m, out, nz, nx, stdv, samples)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Kirchhoff depth migration (src/migration/)
+
+def ricker(dt, fmax):
+    """Migration::set_wavelet (src/migration/migration.cu:54-73)."""
+    n = lib().orc_ricker(dt, fmax, np.zeros(1, np.float32), 0)
+    out = np.zeros(max(n, 1), np.float32)
+    lib().orc_ricker(dt, fmax, out, n)
+    return out[:n]
+
+
+def trace_convolve(traces, wavelet, adjoint):
+    """forward_convolution (adjoint=False) / adjoint_convolution (adjoint=True)
+    of src/migration/migration.cu:197-261 on every row of traces (ntr, nt)."""
+    x = np.ascontiguousarray(traces, np.float32)
+    x2 = x.reshape(-1, x.shape[-1])
+    w = np.ascontiguousarray(wavelet, np.float32)
+    out = np.empty_like(x2)
+    lib().orc_trace_convolve(int(bool(adjoint)), x2.shape[0], x2.shape[1], w, len(w), x2, out)
+    return out.reshape(x.shape)
+
+
+def kdm(domain, adjoint, S, src_tables, rec_tables, isrc, irec, cmp, cmp_id, data, model, dt, dx, dz,
+        aperture, da=1.0, nang=1):
+    """The reference's per-trace Kirchhoff kernels (src/migration/migration.cu:398-756)
+    for every trace t in order: adjoint adds into model, forward into data
+    (both updated in place and returned).  S: padded slowness (nxx, nzz);
+    tables (n, nxx, nzz); data (ntr, nt); model (nx, nz) for domain 0,
+    (ncmp, nang, nz) for domain 1."""
+    S = np.ascontiguousarray(S, np.float32)
+    ts = np.ascontiguousarray(src_tables, np.float32)
+    tr = np.ascontiguousarray(rec_tables, np.float32)
+    nxx, nzz = S.shape
+    assert ts.shape[1:] == S.shape and tr.shape[1:] == S.shape
+    isrc = np.ascontiguousarray(isrc, np.int32)
+    irec = np.ascontiguousarray(irec, np.int32)
+    cmp = np.ascontiguousarray(cmp, np.float32)
+    ntr = len(isrc)
+    assert data.dtype == np.float32 and data.flags.c_contiguous and data.shape[0] == ntr
+    assert model.dtype == np.float32 and model.flags.c_contiguous
+    assert isrc.max(initial=0) < len(ts) and irec.max(initial=0) < len(tr)
+    cid = None
+    if domain == 1:
+        cid = np.ascontiguousarray(cmp_id, np.int32)
+        assert model.ndim == 3 and model.shape[1:] == (nang, nzz - 2 * NB) and model.shape[0] > cid.max(initial=0)
+        assert cid.min(initial=0) >= 0
+    else:
+        assert model.shape == (nxx - 2 * NB, nzz - 2 * NB)
+    lib().orc_kdm(domain, int(bool(adjoint)), S, ts, tr, ntr, isrc, irec, cmp,
+                  None if cid is None else cid.ctypes.data_as(C.c_void_p), data, model, data.shape[1], dt, dx, dz,
+                  nzz, nxx, aperture, da, nang)
+    return model if adjoint else data
+
+
+def _seqsum(a):
+    """float32 sum in index order (the reference's host loops)."""
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    return np.cumsum(a, dtype=np.float32)[-1] if a.size else np.float32(0)
+
+
+def lskdm(domain, S, src_tables, rec_tables, isrc, irec, cmp, cmp_id, dobs, wavelet, max_it, smooth, dt, dx, dz,
+          aperture, model_shape, da=1.0, nang=1):
+    """LSKDM::kirchhoff_depth_migration (src/migration/LSKDM.cu:3-226) with
+    ADLSKDM::regularization (src/migration/ADLSKDM.cu:13-41) for domain 1:
+    float32 vectors, float32 sums in the reference's order.  dobs: (ntr, nt)
+    observed traces in trace order.  Returns (model, residual norms)."""
+    f = np.float32
+    eps = f(1e-6)
+    dobs = np.ascontiguousarray(dobs, np.float32)
+    mask = np.abs(dobs) > eps
+    model = np.zeros(model_shape, np.float32)
+    direction = np.zeros_like(model)
+    grad_old = np.zeros_like(model)
+    op = dict(dt=dt, dx=dx, dz=dz, aperture=aperture, da=da, nang=nang)
+
+    def fwd(m):
+        d = kdm(domain, False, S, src_tables, rec_tables, isrc, irec, cmp, cmp_id, np.zeros_like(dobs), m, **op)
+        return trace_convolve(d, wavelet, False)
+
+    def adj(d):
+        d = np.ascontiguousarray(trace_convolve(d, wavelet, True))
+        return kdm(domain, True, S, src_tables, rec_tables, isrc, irec, cmp, cmp_id, d, np.zeros_like(model), **op)
+
+    residuo, it = [], 0
+    while True:
+        r = np.where(mask, dobs - fwd(model), f(0)).astype(np.float32)
+        residuals = _seqsum(r * r)
+        grad = adj(r)
+        it += 1
+        residuo.append(float(np.sqrt(residuals)))
+        if it > max_it:
+            break
+        beta = _seqsum(grad * grad) / (_seqsum(grad_old * grad_old) + eps)
+        direction = (grad + f(beta) * direction).astype(np.float32)
+        alpha_num = _seqsum(grad * direction)
+        grad_old = grad
+        dd = fwd(direction)
+        alpha = alpha_num / (_seqsum(dd * dd) + eps)
+        if domain == 1 and smooth:
+            m = model
+            fw = np.concatenate([m[:, 1:], m[:, -2:-1]], axis=1)
+            bw = np.concatenate([m[:, 1:2], m[:, :-1]], axis=1)
+            direction = direction + (f(smooth) * ((fw - f(2) * m) + bw)) / (f(da) * f(da))
+        model = (model + f(alpha) * direction).astype(np.float32)
+    return model, residuo

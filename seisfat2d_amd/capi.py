@@ -27,7 +27,11 @@ EXPORTS = [
     "seisfat_batch_misfit_gradient", "seisfat_eik_fast_division", "seisfat_eik_set_slowness_padded",
     "seisfat_eik_device_S", "seisfat_eik_solve_noalloc", "seisfat_batch_vjp", "seisfat_batch_run_converged",
     "seisfat_batch_tile_counts", "seisfat_batch_create_opts", "seisfat_batch_get_opts",
+    # include/seisfat2d_kdm.h
+    "seisfat_kdm_create", "seisfat_kdm_set_tables", "seisfat_kdm_set_traces", "seisfat_kdm_adjoint",
+    "seisfat_kdm_forward", "seisfat_kdm_destroy", "seisfat_trace_convolve", "seisfat_ricker",
 ]
+KDM_IMAGE, KDM_ANGLE = 0, 1
 
 
 class SeisfatError(RuntimeError):
@@ -53,6 +57,12 @@ class SchedOpts(C.Structure):
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.FIELDS}
+
+
+class KdmParams(C.Structure):
+    """seisfat_kdm_params (include/seisfat2d_kdm.h)."""
+    _fields_ = [("domain", C.c_int32), ("nt", C.c_int32), ("dt", C.c_float), ("aperture", C.c_float),
+                ("da", C.c_float), ("nang", C.c_int32), ("ncmp", C.c_int32)]
 
 
 class Shot(C.Structure):
@@ -109,6 +119,14 @@ def lib():
         L.seisfat_batch_misfit_gradient.argtypes = [vp, vp, vp, vp, vp]
         L.seisfat_batch_vjp.argtypes = [vp, vp, vp]
         L.seisfat_batch_run_converged.argtypes = [vp, i, i, C.POINTER(i), vp]
+        L.seisfat_kdm_create.argtypes = [vp, C.POINTER(KdmParams), i, i, C.POINTER(vp)]
+        L.seisfat_kdm_set_tables.argtypes = [vp, vp, vp, vp]
+        L.seisfat_kdm_set_traces.argtypes = [vp, i, vp, vp, vp, vp]
+        L.seisfat_kdm_adjoint.argtypes = [vp, vp, vp, vp]
+        L.seisfat_kdm_forward.argtypes = [vp, vp, vp, vp]
+        L.seisfat_kdm_destroy.argtypes = [vp]
+        L.seisfat_trace_convolve.argtypes = [i, i, i, vp, i, vp, vp, vp]
+        L.seisfat_ricker.argtypes = [f, f, vp, i]
         ptr_ret = {"seisfat_last_error", "seisfat_eik_device_T", "seisfat_batch_device_picks",
                    "seisfat_batch_device_T", "seisfat_eik_device_S"}
         for name in EXPORTS:

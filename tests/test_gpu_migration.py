@@ -1,0 +1,215 @@
+"""Kirchhoff depth migration on the GPU against the oracle.
+
+Bars (the weights use the GPU's acosf / cosf / expf, which differ from the
+host's by an ulp or two; traveltime sums, time indices, the convolutions and
+the trace order of the image-domain adjoint are exact):
+  - convolutions: bit-identical (the same double sums in the same order);
+  - image-domain adjoint: |m_gpu - m_oracle| <= 1e-5 * max |m_oracle|;
+  - forward (LDS accumulation order differs from the reference's atomics):
+    per sample |d_gpu - d_oracle| <= 1e-5 * (L |m|) + 1e-30, L |m| the
+    oracle's forward of |m|;
+  - angle domain: the same bars on all but 0.1 % of the entries (an angle
+    within an ulp of a bin edge may land in the neighbouring bin), and the
+    gather summed over angles within the image-domain bar;
+  - the migration loops: KDM as the adjoint; LSKDM residual norms within
+    1e-4 relative (float64 sums here, float32 sums in trace order in the
+    reference), the image within 1e-4 of its largest value, the angle
+    gather within 1e-3 in norm after three iterations.
+"""
+import numpy as np
+import pytest
+
+from seisfat2d_amd import Eikonal, migration as mig, models
+
+pytestmark = pytest.mark.gpu
+
+NB = 3
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    return torch
+
+
+def survey(nz=48, nx=84, dz=10.0, dx=10.0, nsrc=4, nrec=14, seed=1):
+    m = models.rough(nz, nx, dz, dx, seed=seed, vlo=1800.0, vhi=3500.0, sigma=3.0)
+    xs = np.linspace(40.0, (nx - 1) * dx - 40.0, nsrc).astype(np.float32)
+    xr = np.linspace(0.0, (nx - 1) * dx, nrec).astype(np.float32)
+    return m, xs, xr
+
+
+def make(e, m, xs, xr, domain, nt=420, dt=2e-3, max_offset=500.0, aperture=30.0, da=3.0, max_angle=60.0,
+         tables=None):
+    return mig.Kirchhoff(e, xs, np.zeros_like(xs), xr, np.zeros_like(xr), nt, dt, aperture, max_offset, 30.0,
+                         domain=domain, da=da, max_angle=max_angle, tables=tables)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    m, xs, xr = survey()
+    e = Eikonal(m.s.shape[1], m.s.shape[0], m.dz, m.dx)
+    e.set_slowness(m.s)
+    ts = e.traveltime_tables(xs, np.zeros_like(xs))
+    tr = e.traveltime_tables(xr, np.zeros_like(xr))
+    yield dict(m=m, xs=xs, xr=xr, e=e, ts=ts, tr=tr)
+    e.close()
+
+
+def orc_args(oracle, K, st):
+    S = oracle.expand_boundary(st["m"].s)
+    return S, dict(dt=float(K.dt), dx=st["m"].dx, dz=st["m"].dz, aperture=30.0, da=float(K.da), nang=K.nang)
+
+
+def test_tables_are_the_oracle_solves(oracle, setup):
+    st = setup
+    S = oracle.expand_boundary(st["m"].s)
+    for k in (0, 3):
+        T = oracle.solve_iso(S, st["m"].dx, st["m"].dz, float(st["xs"][k]), 0.0)
+        assert np.array_equal(st["ts"][k], T)
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+@pytest.mark.parametrize("nt,dt", [(1001, 1e-3), (64, 2e-3)])
+def test_convolution_bit_exact(oracle, setup, torch, adjoint, nt, dt):
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, nt=nt, dt=dt, tables=(st["ts"], st["tr"]))
+    rng = np.random.default_rng(nt)
+    x = rng.standard_normal((K.ntr, nt)).astype(np.float32)
+    got = K.convolve(torch.from_numpy(x).cuda(), adjoint).cpu().numpy()
+    assert np.array_equal(got, oracle.trace_convolve(x, K.wavelet, adjoint))
+    K.close()
+
+
+def test_image_adjoint_matches_oracle(oracle, setup, torch):
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, tables=(st["ts"], st["tr"]))
+    assert K.ntr > 20
+    S, op = orc_args(oracle, K, st)
+    rng = np.random.default_rng(5)
+    d = rng.standard_normal((K.ntr, K.nt)).astype(np.float32)
+    m0 = rng.standard_normal(K.model_shape).astype(np.float32) * 1e-3
+    got = K.adjoint(torch.from_numpy(d).cuda(), torch.from_numpy(m0.copy()).cuda()).cpu().numpy()
+    want = oracle.kdm(0, True, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, None, d, m0.copy(), **op)
+    assert np.abs(want).max() > 1e-3
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+    K.close()
+
+
+@pytest.mark.parametrize("domain", [mig.IMAGE, mig.ANGLE])
+def test_forward_matches_oracle(oracle, setup, torch, domain):
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], domain, tables=(st["ts"], st["tr"]))
+    S, op = orc_args(oracle, K, st)
+    rng = np.random.default_rng(7 + domain)
+    m = rng.standard_normal(K.model_shape).astype(np.float32)
+    d0 = rng.standard_normal((K.ntr, K.nt)).astype(np.float32) * 1e-4
+    cid = K.cmp_id if domain == mig.ANGLE else None
+    got = K.forward(torch.from_numpy(m).cuda(), torch.from_numpy(d0.copy()).cuda()).cpu().numpy()
+    want = oracle.kdm(domain, False, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, cid, d0.copy(), m, **op)
+    bound = oracle.kdm(domain, False, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, cid, np.abs(d0), np.abs(m), **op)
+    bad = np.abs(got - want) > 1e-5 * bound + 1e-30
+    assert np.abs(want).max() > 0
+    if domain == mig.IMAGE:
+        assert not bad.any(), np.argwhere(bad)[:5]
+    else:
+        assert bad.mean() <= 1e-3, bad.sum()
+    K.close()
+
+
+def test_angle_adjoint_matches_oracle(oracle, setup, torch):
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], mig.ANGLE, tables=(st["ts"], st["tr"]))
+    S, op = orc_args(oracle, K, st)
+    assert K.model_shape == (K.ncmp, K.nang, 48) and K.nang == 21
+    rng = np.random.default_rng(9)
+    d = rng.standard_normal((K.ntr, K.nt)).astype(np.float32)
+    got = K.adjoint(torch.from_numpy(d).cuda()).cpu().numpy()
+    want = oracle.kdm(1, True, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, K.cmp_id, d,
+                      np.zeros(K.model_shape, np.float32), **op)
+    bound = oracle.kdm(1, True, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, K.cmp_id, np.abs(d),
+                       np.zeros(K.model_shape, np.float32), **op)
+    assert np.abs(want).max() > 0
+    bad = np.abs(got - want) > 1e-5 * bound + 1e-30
+    assert bad.mean() <= 1e-3, bad.sum()
+    g, w = got.sum(axis=1, dtype=np.float64), want.sum(axis=1, dtype=np.float64)
+    assert np.abs(g - w).max() <= 1e-5 * bound.sum(axis=1).max()
+    K.close()
+
+
+@pytest.mark.parametrize("domain", [mig.IMAGE, mig.ANGLE])
+def test_dot_product(setup, torch, domain):
+    """<L m, d> = <m, L^T d>: the reference's adjointness test of its
+    operators (src/migration/migration.cu:282-396, without the convolutions)."""
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], domain, tables=(st["ts"], st["tr"]))
+    g = torch.Generator(device="cuda").manual_seed(3)
+    m = torch.randn(K.model_shape, device="cuda", generator=g)
+    d = torch.randn((K.ntr, K.nt), device="cuda", generator=g)
+    Lm = K.forward(m)
+    Ltd = K.adjoint(d)
+    a = torch.sum(Lm.double() * d.double()).item()
+    b = torch.sum(m.double() * Ltd.double()).item()
+    assert abs(a - b) <= 1e-5 * (abs(a) + abs(b)), (a, b)
+    K.close()
+
+
+@pytest.mark.parametrize("domain", [mig.IMAGE, mig.ANGLE])
+def test_migration_matches_oracle(oracle, setup, torch, domain):
+    """KDM::kirchhoff_depth_migration: adjoint convolution of every trace,
+    then the adjoint operator (IDKDM / ADKDM)."""
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], domain, tables=(st["ts"], st["tr"]))
+    S, op = orc_args(oracle, K, st)
+    rng = np.random.default_rng(13)
+    seismic = rng.standard_normal((K.nsrc, K.nrec, K.nt)).astype(np.float32)
+    got = K.migrate(seismic).cpu().numpy()
+    d = np.ascontiguousarray(oracle.trace_convolve(seismic[K.src, K.rec], K.wavelet, True))
+    cid = K.cmp_id if domain == mig.ANGLE else None
+    want = oracle.kdm(domain, True, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, cid, d,
+                      np.zeros(K.model_shape, np.float32), **op)
+    bound = oracle.kdm(domain, True, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, cid, np.abs(d),
+                       np.zeros(K.model_shape, np.float32), **op)
+    bad = np.abs(got - want) > 1e-5 * bound + 1e-30
+    assert bad.mean() <= (0 if domain == mig.IMAGE else 1e-3), bad.sum()
+    K.close()
+
+
+@pytest.mark.parametrize("domain,smooth", [(mig.IMAGE, 0.0), (mig.ANGLE, 1e-3)])
+def test_least_squares_migration_matches_oracle(oracle, setup, torch, domain, smooth):
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], domain, tables=(st["ts"], st["tr"]))
+    S, op = orc_args(oracle, K, st)
+    # consistent data: the forward model of a reflector
+    cid = K.cmp_id if domain == mig.ANGLE else None
+    m_true = np.zeros(K.model_shape, np.float32)
+    m_true[..., 30] = 1.0
+    d = oracle.kdm(domain, False, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, cid,
+                   np.zeros((K.ntr, K.nt), np.float32), m_true, **op)
+    dobs = oracle.trace_convolve(d, K.wavelet, False)
+    seismic = np.zeros((K.nsrc, K.nrec, K.nt), np.float32)
+    seismic[K.src, K.rec] = dobs
+    got, res = K.lsm(seismic, max_it=3, smooth=smooth)
+    want, wres = oracle.lskdm(domain, S, st["ts"], st["tr"], K.src, K.rec, K.cmp, cid, dobs, K.wavelet, 3, smooth,
+                              op["dt"], op["dx"], op["dz"], op["aperture"], K.model_shape, op["da"], op["nang"])
+    got = got.cpu().numpy()
+    assert len(res) == len(wres) == 4
+    assert np.allclose(res, wres, rtol=1e-4, atol=0), (res, wres)
+    assert res[-1] < res[0]
+    if domain == mig.IMAGE:
+        assert np.abs(got - want).max() <= 1e-4 * np.abs(want).max()
+    else:   # a contribution moved to the neighbouring angle bin spreads over the later iterations
+        assert np.linalg.norm(got - want) <= 1e-3 * np.linalg.norm(want)
+    K.close()
+
+
+def test_argument_errors(setup, torch):
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, tables=(st["ts"], st["tr"]))
+    with pytest.raises(ValueError):
+        K.adjoint(torch.zeros((K.ntr, K.nt + 1), device="cuda"))
+    with pytest.raises(ValueError):
+        K.forward(torch.zeros((3, 3), device="cuda"))
+    with pytest.raises(ValueError):
+        make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, tables=(st["ts"][:2], st["tr"]))
+    K.close()
