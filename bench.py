@@ -311,6 +311,139 @@ def bench_tables(ntables=64):
             "tables_per_s": ntables / t_dev, "tables_per_s_with_host_copy": ntables / t_host}
 
 
+def bench_migration(steps=3, cpu_traces=200):
+    """Kirchhoff depth migration with the reference's sample migration
+    parameters (models.migration: 1001 x 1001 at 10 m, 51 sources, 201
+    receivers, offsets below 2000 m, nt 1001 at 1 ms): traveltime tables of
+    every source and receiver position (one batch each, left in HBM), then
+    per domain the adjoint operator (the migration's kernel, HIP events on
+    torch's stream, median of `steps`), the whole migration (gather, adjoint
+    convolution, adjoint operator; wall clock) and the forward operator.
+    Reports node-trace pairs per second (the reference launches one thread
+    per padded node and trace) and the pairs that reach the trace, and the
+    oracle timed on `cpu_traces` traces of the image-domain adjoint on one
+    host core."""
+    import numpy as np
+    import torch
+    from seisfat2d_amd import Eikonal, models
+    from seisfat2d_amd import migration as mig
+    w = models.migration()
+    P = w.extra["params"]
+    xs, zs, xr, zr = w.extra["xs"], w.extra["zs"], w.rx, w.rz
+    e = Eikonal(w.nz, w.nx, w.dz, w.dx)
+    e.set_slowness(w.s)
+    t0 = time.perf_counter()
+    ts = e.traveltime_tables(xs, zs, chunk=len(xs))
+    tr = e.traveltime_tables(xr, zr, chunk=len(xr))
+    t_tables = time.perf_counter() - t0
+    out = {"workload": (f"Kirchhoff depth migration, {w.nz}x{w.nx} dh=10 m layered model, {len(xs)} sources every "
+                        f"200 m, {len(xr)} receivers every 50 m, offsets < {P['max_offset']:.0f} m, nt {P['nt']} "
+                        f"at {P['dt'] * 1e3:.0f} ms, fmax {P['fmax']:.0f} Hz, aperture {P['aperture']:.0f} deg, "
+                        f"angles to {P['max_angle']:.0f} deg every {P['da']:.0f} deg"),
+           "tables_s_with_host_copies": t_tables}
+
+    def events_ms(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ms = []
+        for _ in range(steps):
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.median(ms))
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    for domain, name in ((mig.IMAGE, "image"), (mig.ANGLE, "angle")):
+        K = mig.Kirchhoff(e, xs, zs, xr, zr, P["nt"], P["dt"], P["aperture"], P["max_offset"], P["fmax"],
+                          domain=domain, da=P["da"], max_angle=P["max_angle"], tables=(ts, tr))
+        seismic = torch.randn((K.nsrc, K.nrec, K.nt), device="cuda", generator=g)
+        d = K.convolve(K.gather(seismic), adjoint=True)
+        model = K.adjoint(d)
+        torch.cuda.synchronize()
+        adj_ms = events_ms(lambda: K.adjoint(d, model.zero_()))
+        fwd_ms = events_ms(lambda: K.forward(model, d.zero_()))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            K.migrate(seismic)
+        torch.cuda.synchronize()
+        mig_ms = (time.perf_counter() - t0) / steps * 1e3
+        pairs = K.ntr * w.nz * w.nx
+        r = {"traces": K.ntr, "model_shape": list(K.model_shape), "adjoint_ms": adj_ms, "forward_ms": fwd_ms,
+             "migration_ms": mig_ms, "traces_per_s": K.ntr / (mig_ms / 1e3),
+             "adjoint_pairs_per_s": pairs / (adj_ms / 1e3), "forward_pairs_per_s": pairs / (fwd_ms / 1e3)}
+        if domain == mig.IMAGE:
+            # node-trace pairs that reach the trace (time index < nt), counted on the device
+            dt32 = torch.tensor(np.float32(P["dt"]), device="cuda")
+            Ts = torch.from_numpy(ts[:, 3:-3, 3:-3].copy()).cuda()
+            Tr = torch.from_numpy(tr[:, 3:-3, 3:-3].copy()).cuda()
+            reach = 0
+            for t in range(K.ntr):
+                reach += int((torch.floor((Ts[K.src[t]] + Tr[K.rec[t]]) / dt32) < P["nt"]).sum())
+            del Ts, Tr
+            r["pairs_reaching"] = reach
+            r["reaching_fraction"] = reach / pairs
+            r["adjoint_reaching_pairs_per_s"] = reach / (adj_ms / 1e3)
+            out["cpu_baseline"] = _migration_cpu(K, ts, tr, w, P, cpu_traces, reach / pairs)
+        out[name] = r
+        K.close()
+    # the same survey with an 8 s record (nt 8001): most node-trace pairs
+    # reach the trace, so the adjoint is bound by the weights' evaluation
+    # and the table quantities' traffic, not by the passes over chunks
+    nt_deep = 8001
+    K = mig.Kirchhoff(e, xs, zs, xr, zr, nt_deep, P["dt"], P["aperture"], P["max_offset"], P["fmax"],
+                      domain=mig.IMAGE, tables=(ts, tr))
+    d = torch.randn((K.ntr, K.nt), device="cuda", generator=g)
+    model = K.adjoint(d)
+    torch.cuda.synchronize()
+    adj_ms = events_ms(lambda: K.adjoint(d, model.zero_()))
+    dt32 = torch.tensor(np.float32(P["dt"]), device="cuda")
+    Ts = torch.from_numpy(ts[:, 3:-3, 3:-3].copy()).cuda()
+    Tr = torch.from_numpy(tr[:, 3:-3, 3:-3].copy()).cuda()
+    reach = 0
+    for t in range(K.ntr):
+        reach += int((torch.floor((Ts[K.src[t]] + Tr[K.rec[t]]) / dt32) < nt_deep).sum())
+    del Ts, Tr
+    pairs = K.ntr * w.nz * w.nx
+    # algorithmic bytes of one adjoint launch: per reaching pair the receiver
+    # table's five image-domain quantities (T, |det H|, |grad T|, angle,
+    # |cos|) and one data sample (24 B); per image point and source run the
+    # source's five quantities (20 B); the image read and written (8 B)
+    runs = len(set(K.src.tolist()))
+    alg = 24 * reach + 20 * w.nz * w.nx * runs + 8 * w.nz * w.nx
+    achieved = alg / (adj_ms / 1e3) / 1e9
+    out["image_deep_record"] = {
+        "nt": nt_deep, "traces": K.ntr, "adjoint_ms": adj_ms, "pairs_reaching": reach,
+        "reaching_fraction": reach / pairs, "adjoint_pairs_per_s": pairs / (adj_ms / 1e3),
+        "adjoint_reaching_pairs_per_s": reach / (adj_ms / 1e3),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "scope": "k_kdm_adjoint<image> launch, HIP events on torch's stream",
+                     "algorithmic_bytes_per_launch": alg}}
+    K.close()
+    e.close()
+    return out
+
+
+def _migration_cpu(K, ts, tr, w, P, ntr, reach_frac):
+    """The oracle (the reference's per-trace kernel restated on the CPU, one
+    thread) on the first `ntr` traces of the image-domain adjoint."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as o
+    S = o.expand_boundary(w.s)
+    d = np.random.default_rng(0).standard_normal((ntr, K.nt)).astype(np.float32)
+    m = np.zeros(K.model_shape, np.float32)
+    t0 = time.perf_counter()
+    o.kdm(0, True, S, ts, tr, K.src[:ntr], K.rec[:ntr], K.cmp[:ntr], None, d, m, P["dt"], w.dx, w.dz,
+          P["aperture"], 1.0, 1)
+    dt = time.perf_counter() - t0
+    return {"value": ntr * w.nz * w.nx / dt, "unit": "node-trace pairs/s", "cores": 1, "kind": "port",
+            "sample": f"first {ntr} traces of the image-domain adjoint, 1 thread",
+            "traces_per_s": ntr / dt}
+
+
 def traveltime_errors():
     """Max relative traveltime error over interior nodes (T_ref > 0) of the GPU
     solve against the FP32 oracle (the reference's CPU path restated; the
@@ -408,7 +541,7 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="profiler run: the batched steps only, plus a 256 MiB device copy that calibrates "
                          "per-dispatch byte counters")
-    ap.add_argument("--only", choices=["ani", "adjoint"],
+    ap.add_argument("--only", choices=["ani", "adjoint", "migration"],
                     help="profiler run of one secondary workload: its steps only, printed as one JSON line")
     args = ap.parse_args()
     if args.profile:
@@ -425,7 +558,7 @@ def main():
 
     torch.cuda.init()
     if args.only:
-        run = bench_ani if args.only == "ani" else bench_adjoint
+        run = {"ani": bench_ani, "adjoint": bench_adjoint, "migration": bench_migration}[args.only]
         print(json.dumps(run(max(1, args.steps))))
         if args.profile:
             calibration_copy()
@@ -526,6 +659,10 @@ def main():
         extras["adjoint_c5"] = bench_adjoint(max(1, min(args.steps, 5)))
         a = extras["adjoint_c5"]
         log(f"adjoint: record {a['ms_record']:.1f} ms + vjp {a['ms_vjp']:.1f} ms per gradient of 128 shots")
+        extras["migration"] = bench_migration(max(1, min(args.steps, 5)))
+        m = extras["migration"]
+        log(f"migration: image adjoint {m['image']['adjoint_ms']:.2f} ms, migration {m['image']['migration_ms']:.1f} "
+            f"ms for {m['image']['traces']} traces; angle adjoint {m['angle']['adjoint_ms']:.2f} ms")
 
     cpu = None
     if args.cpu_seconds > 0:

@@ -10,6 +10,12 @@
 //     Hessian determinant, the gradient norm, the incidence angle and its
 //     cosine) is evaluated once per table (k_kdm_prep), with the reference's
 //     operations, and every trace reads it (7 words per node and table);
+//   - each table also keeps the minimum traveltime of every 64 consecutive
+//     image points (k_kdm_tmin); rounded addition and division being
+//     monotone, a wave whose chunk minima give a time index >= nt holds no
+//     node that reaches the trace and passes over it with two scalar loads
+//     (with the reference's recording lengths most node-trace pairs end
+//     there); the other quantities are loaded only for nodes that reach it;
 //   - image-domain adjoint: one thread per image point loops over the traces
 //     in the migration's order and keeps its sum in a register, so the image
 //     receives the same FP32 sequence of additions as under the reference's
@@ -24,6 +30,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -37,6 +46,11 @@ constexpr int KQ = 7;             // per-table quantities: T, dT/dx, dT/dz, |det
 
 struct KdmK {
   int nz, nx, nzz, npts, nt, nang;
+  int nchunk;     // 64-node chunks of the image points (one wave each)
+  float thr;      // least x with RN(x / dt) >= nt: a traveltime sum x >= thr has time index >= nt
+  float zh_dt, zl_dt;  // two-word reciprocal of dt (fdiv)
+  int fastdiv_dt;      // fdiv(x, zh_dt, zl_dt) == RN(x / dt) for every 2^-100 <= x < 2^100 (checked exhaustively)
+  int rec_nonneg; // every receiver-table value at an image point is >= 0 (no NaN)
   float dx, dz, dt, da;
   float tan_ap;  // tanf(aperture in radians), the reference's sigma / z
   float scale;   // 1 / (2 pi) as the reference rounds it
@@ -89,26 +103,84 @@ __global__ __launch_bounds__(256) void k_kdm_prep(const float* __restrict__ tabl
   }
 }
 
+// NaN-propagating minimum: a chunk holding a NaN traveltime is never skipped
+// (its node's time index would be 0).
+__device__ __forceinline__ float min_nan(float a, float b) { return (a != a || b != b) ? __builtin_nanf("") : fminf(a, b); }
+
+// Per table, the minimum traveltime over each chunk of 64 consecutive image
+// points (p = i + j*nz): one wave per chunk.  For every node of a chunk,
+// RN(RN(Ts + Tr) / dt) >= RN(RN(min_s + min_r) / dt) (rounded addition and
+// division are monotone), so a chunk whose minima give a time index >= nt
+// holds no node that reaches the trace.
+__global__ __launch_bounds__(256) void k_kdm_tmin(const float* __restrict__ q, int ntab, KdmK K,
+                                                  float* __restrict__ tmin) {
+  const int lane = threadIdx.x & 63;
+  const size_t nw = (size_t)ntab * K.nchunk;
+  const size_t wstride = (size_t)gridDim.x * (blockDim.x >> 6);
+  for (size_t w = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nw; w += wstride) {
+    const int tab = (int)(w / K.nchunk);
+    const int c = (int)(w - (size_t)tab * K.nchunk);
+    const int p = c * 64 + lane;
+    float v = p < K.npts ? q[(size_t)tab * KQ * K.npts + p] : __builtin_inff();
+    for (int off = 32; off >= 1; off >>= 1) v = min_nan(v, __shfl_xor(v, off, 64));
+    if (lane == 0) tmin[w] = v;
+  }
+}
+
 struct Q7 {
   float t, dx, dz, det, norm, theta, cosv;
 };
-__device__ __forceinline__ Q7 q_load(const float* __restrict__ q, int tab, int p, int npts) {
+// The quantities after T (loaded once the node is known to reach the trace).
+__device__ __forceinline__ void q_rest(const float* __restrict__ q, int tab, int p, int npts, Q7& o) {
   const float* __restrict__ b = q + (size_t)tab * KQ * npts + p;
-  return Q7{b[0], b[(size_t)npts], b[(size_t)2 * npts], b[(size_t)3 * npts], b[(size_t)4 * npts],
-            b[(size_t)5 * npts], b[(size_t)6 * npts]};
+  o.dx = b[(size_t)npts];
+  o.dz = b[(size_t)2 * npts];
+  o.det = b[(size_t)3 * npts];
+  o.norm = b[(size_t)4 * npts];
+  o.theta = b[(size_t)5 * npts];
+  o.cosv = b[(size_t)6 * npts];
 }
+
+// Time index of a node for a trace.
+__device__ __forceinline__ int t_index(float ts, float tr, const KdmK& K) { return f2i_rd((ts + tr) / K.dt); }
+
+// The same for a traveltime sum xs < thr: RN(xs / dt) by the two-instruction
+// known-divisor division where the check proved it exact, IEEE division
+// otherwise (xs below 2^-100, negative or NaN).
+__device__ __forceinline__ int t_index_below(float xs, const KdmK& K) {
+  return f2i_rd((K.fastdiv_dt && xs >= 0x1p-100f) ? fdiv(xs, K.zh_dt, K.zl_dt) : xs / K.dt);
+}
+
+// Refined reciprocal of b (the r of rdiv).
+__device__ __forceinline__ float recip_refined(float b) {
+  const float r0 = __builtin_amdgcn_rcpf(b);
+  return __builtin_fmaf(__builtin_fmaf(-b, r0, 1.0f), r0, r0);
+}
+
+// True when every node whose traveltime sum is at least RN(a + b) has a time
+// index >= nt: RN(a + b) >= thr.  Exactly f2i_rd(RN(RN(a + b) / dt)) >= nt
+// (NaN gives index 0 and fails the compare; +inf passes both).
+__device__ __forceinline__ bool beyond(float a, float b, const KdmK& K) { return a + b >= K.thr; }
 
 // Kirchhoff weight of one node for one trace (the product the reference
 // kernels form): Hessian amplitude, obliquity, spreading, reflection-angle
 // factor and the aperture taper about the trace's CMP.
-__device__ __forceinline__ float kdm_weight(const Q7& s, const Q7& r, float S, float x, float sigma, float cmp,
-                                            const KdmK& K) {
-  const float jterm = sqrtf(s.det * r.det) * S / (s.norm * r.norm);
+// rs = max(Ts / S, EPS), rr = max(Tr / S, EPS) and sig_e = sigma + EPS (with
+// its refined reciprocal r_sig) come in evaluated: they depend on the node and
+// one table only (the same roundings as per trace).  Every operation is the
+// reference's IEEE one; where an operand range allows, in fewer instructions:
+//   - sqrt of det_s det_r and of rs rr (both >= EPS^2, normal): the hardware
+//     estimate and its two one-ulp corrections (rsqrt_rn);
+//   - (x - cmp) / sig_e (divisor in [EPS, 2^40), dividend 0 or of magnitude
+//     >= 2^-60: no operand rescaling): rdiv with the refined reciprocal.
+__device__ __forceinline__ float kdm_weight(const Q7& s, const Q7& r, float S, float rs, float rr, float x,
+                                            float sig_e, float r_sig, float cmp, const KdmK& K) {
+  const float jterm = rsqrt_rn(s.det * r.det) * S / (s.norm * r.norm);
   const float obliq = sqrtf(fmaxf(0.0f, s.cosv) * fmaxf(0.0f, r.cosv));
-  const float rs = fmaxf(s.t / S, KDM_EPS), rr = fmaxf(r.t / S, KDM_EPS);
-  const float g = 1.0f / sqrtf(rs * rr);
+  const float g = 1.0f / rsqrt_rn(rs * rr);
   const float rf = 1.0f + 0.2f * cosf(0.5f * (s.theta + r.theta));
-  const float a = (x - cmp) / (sigma + KDM_EPS);
+  const float n = x - cmp;
+  const float a = (n == 0.0f || fabsf(n) >= 0x1p-60f) ? rdiv(n, sig_e, r_sig) : n / sig_e;
   const float taper = expf(-0.5f * a * a);
   return K.scale * jterm * obliq * g * rf * taper;
 }
@@ -120,75 +192,135 @@ __device__ __forceinline__ int kdm_angle_bin(const Q7& s, const Q7& r, const Kdm
   return f2i_rd(ang / K.da);
 }
 
-// Image-domain adjoint: thread per image point, traces in order.
-__global__ __launch_bounds__(256) void k_kdm_adjoint_image(const float* __restrict__ q, const float* __restrict__ S,
-                                                           const int* __restrict__ tsrc, const int* __restrict__ trec,
-                                                           const float* __restrict__ tcmp, int ntr, int nsrc,
-                                                           const float* __restrict__ data, float* __restrict__ model,
-                                                           KdmK K) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= K.npts) return;
-  const int i = p % K.nz, j = p / K.nz;
-  const float z = (float)i * K.dz, x = (float)j * K.dx;
-  const float s_node = S[(i + 3) + (size_t)(j + 3) * K.nzz];
-  const float sigma = K.tan_ap * z;
-  float acc = model[p];
-  int cur = -1;
-  Q7 qs{};
-  for (int t = 0; t < ntr; ++t) {
-    const int s = tsrc[t];
-    if (s != cur) {
-      qs = q_load(q, s, p, K.npts);
-      cur = s;
-    }
-    const Q7 qr = q_load(q, nsrc + trec[t], p, K.npts);
-    const int it = f2i_rd((qs.t + qr.t) / K.dt);
-    if ((unsigned)it < (unsigned)K.nt)
-      acc += kdm_weight(qs, qr, s_node, x, sigma, tcmp[t], K) * data[(size_t)t * K.nt + it];
-  }
-  model[p] = acc;
-}
+// Adjoint: thread per image point (a wave holds 64 consecutive points, one
+// chunk).  The traces come grouped in source blocks: up to SB runs of equal
+// sources (a run: consecutive traces with one source) per block, and per
+// block a list of entries sorted by receiver, each one receiver table and,
+// per source slot k, the trace pairing it with the block's k-th source (or
+// -1): E = {receiver, t_0, ..., t_(SB-1)}.  A wave keeps the block's source
+// quantities in registers and loads a receiver's quantities once per entry,
+// for up to SB node-trace pairs.  It passes over a pair when the chunk minima
+// keep every node beyond the last sample, over a source slot when the source
+// minimum alone does (receiver tables non-negative), and loads nothing for
+// either.  Image domain: the point's sum stays in a register (no atomics) and
+// takes the contributions block by block, by receiver, by slot.  Angle
+// domain: adds into the (depth, angle, CMP) bins with atomics, as the
+// reference.
+#ifndef SEISFAT_KDM_SB
+#define SEISFAT_KDM_SB 4
+#endif
+constexpr int KDM_SB = SEISFAT_KDM_SB;
 
-// Angle-domain adjoint: thread per image point, traces in order, bins by atomics.
-__global__ __launch_bounds__(256) void k_kdm_adjoint_angle(const float* __restrict__ q, const float* __restrict__ S,
-                                                           const int* __restrict__ tsrc, const int* __restrict__ trec,
-                                                           const float* __restrict__ tcmp,
-                                                           const int* __restrict__ tcid, int ntr, int nsrc,
-                                                           const float* __restrict__ data, float* __restrict__ model,
-                                                           KdmK K) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= K.npts) return;
+template <bool ANGLE, int SB>
+__global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q, const float* __restrict__ tmin,
+                                                     const float* __restrict__ S, const int* __restrict__ bsrc,
+                                                     const int* __restrict__ bent, const int* __restrict__ ent,
+                                                     const float* __restrict__ tcmp, const int* __restrict__ tcid,
+                                                     int nb, int nsrc, const float* __restrict__ data,
+                                                     float* __restrict__ model, KdmK K) {
+  const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = p0 < K.npts;
+  const int p = live ? p0 : K.npts - 1;  // lanes past the end load in bounds and store nothing
+  const int c = __builtin_amdgcn_readfirstlane(p0 >> 6);
+  if (c >= K.nchunk) return;  // whole wave past the end
+  const int lane = threadIdx.x & 63;
   const int i = p % K.nz, j = p / K.nz;
   const float z = (float)i * K.dz, x = (float)j * K.dx;
   const float s_node = S[(i + 3) + (size_t)(j + 3) * K.nzz];
-  const float sigma = K.tan_ap * z;
-  int cur = -1;
-  Q7 qs{};
-  for (int t = 0; t < ntr; ++t) {
-    const int s = tsrc[t];
-    if (s != cur) {
-      qs = q_load(q, s, p, K.npts);
-      cur = s;
+  const float sig_e = K.tan_ap * z + KDM_EPS;
+  const float r_sig = recip_refined(sig_e);
+  float acc = (!ANGLE && live) ? model[p] : 0.0f;
+  for (int b = 0; b < nb; ++b) {
+    float ms[SB];
+    bool use[SB];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < SB; ++k) {
+      const int sk = bsrc[b * SB + k];
+      ms[k] = sk >= 0 ? tmin[(size_t)sk * K.nchunk + c] : 0.0f;
+      use[k] = sk >= 0 && !(K.rec_nonneg && beyond(ms[k], 0.0f, K));
+      any |= use[k];
     }
-    const Q7 qr = q_load(q, nsrc + trec[t], p, K.npts);
-    const int it = f2i_rd((qs.t + qr.t) / K.dt);
-    if ((unsigned)it < (unsigned)K.nt) {
-      const float w = kdm_weight(qs, qr, s_node, x, sigma, tcmp[t], K);
-      const int a = kdm_angle_bin(qs, qr, K);
-      if (a >= 0 && a < K.nang)
-        atomicAdd(model + i + (size_t)a * K.nz + (size_t)tcid[t] * K.nang * K.nz, w * data[(size_t)t * K.nt + it]);
+    if (!any) continue;
+    Q7 qs[SB];
+    float rs[SB];
+#pragma unroll
+    for (int k = 0; k < SB; ++k) {
+      if (use[k]) {
+        const int sk = bsrc[b * SB + k];
+        qs[k].t = q[(size_t)sk * KQ * K.npts + p];
+        q_rest(q, sk, p, K.npts, qs[k]);
+        rs[k] = fmaxf(qs[k].t / s_node, KDM_EPS);
+      }
+    }
+    const int e1 = bent[b + 1];
+    for (int e0 = bent[b]; e0 < e1; e0 += 64) {
+      // lane l tests entry e0 + l: does any of its pairs pass the chunk test?
+      bool need = false;
+      if (e0 + lane < e1) {
+        const int* __restrict__ El = ent + (size_t)(e0 + lane) * (1 + SB);
+        const float mr_l = tmin[(size_t)(nsrc + El[0]) * K.nchunk + c];
+#pragma unroll
+        for (int k = 0; k < SB; ++k) need |= El[1 + k] >= 0 && use[k] && !beyond(ms[k], mr_l, K);
+      }
+      for (uint64_t m = __ballot(need); m; m &= m - 1) {
+        const int e = e0 + __builtin_ctzll(m);
+        const int* __restrict__ E = ent + (size_t)e * (1 + SB);
+        const int r = nsrc + E[0];
+        const float mr = tmin[(size_t)r * K.nchunk + c];
+        const float rt = q[(size_t)r * KQ * K.npts + p];
+        // first the time indices and the data samples of every pair, then the
+        // weights: the gathers of one entry are in flight together
+        int tk[SB];
+        float dv[SB];
+        bool ok[SB];
+        bool anyok = false;
+#pragma unroll
+        for (int k = 0; k < SB; ++k) {
+          tk[k] = E[1 + k];
+          ok[k] = false;
+          dv[k] = 0.0f;
+          if (tk[k] < 0 || !use[k] || beyond(ms[k], mr, K)) continue;
+          const float xs = qs[k].t + rt;
+          const int it = xs >= K.thr ? K.nt : t_index_below(xs, K);
+          ok[k] = live && (unsigned)it < (unsigned)K.nt;
+          if (ok[k]) dv[k] = data[(size_t)tk[k] * K.nt + it];
+          anyok |= ok[k];
+        }
+        if (!__any(anyok)) continue;
+        Q7 qr;
+        qr.t = rt;
+        q_rest(q, r, p, K.npts, qr);
+        const float rr = fmaxf(rt / s_node, KDM_EPS);
+#pragma unroll
+        for (int k = 0; k < SB; ++k) {
+          if (!ok[k]) continue;
+          const float w = kdm_weight(qs[k], qr, s_node, rs[k], rr, x, sig_e, r_sig, tcmp[tk[k]], K);
+          const float v = w * dv[k];
+          if (ANGLE) {
+            const int a = kdm_angle_bin(qs[k], qr, K);
+            if (a >= 0 && a < K.nang)
+              atomicAdd(model + i + (size_t)a * K.nz + (size_t)tcid[tk[k]] * K.nang * K.nz, v);
+          } else {
+            acc += v;
+          }
+        }
+      }
     }
   }
+  if (!ANGLE && live) model[p] = acc;
 }
 
 // Forward (image or angle domain): one workgroup per trace, the trace's nt
-// samples in LDS.
+// samples in LDS; each wave passes over a chunk the minima keep beyond the
+// last sample with two scalar loads.
 template <bool ANGLE>
-__global__ __launch_bounds__(256) void k_kdm_forward(const float* __restrict__ q, const float* __restrict__ S,
-                                                     const int* __restrict__ tsrc, const int* __restrict__ trec,
-                                                     const float* __restrict__ tcmp, const int* __restrict__ tcid,
-                                                     int nsrc, const float* __restrict__ model,
-                                                     float* __restrict__ data, KdmK K) {
+__global__ __launch_bounds__(256) void k_kdm_forward(const float* __restrict__ q, const float* __restrict__ tmin,
+                                                     const float* __restrict__ S, const int* __restrict__ tsrc,
+                                                     const int* __restrict__ trec, const float* __restrict__ tcmp,
+                                                     const int* __restrict__ tcid, int nsrc,
+                                                     const float* __restrict__ model, float* __restrict__ data,
+                                                     KdmK K) {
   extern __shared__ float hist[];
   const int t = blockIdx.x;
   for (int k = threadIdx.x; k < K.nt; k += blockDim.x) hist[k] = 0.0f;
@@ -196,19 +328,35 @@ __global__ __launch_bounds__(256) void k_kdm_forward(const float* __restrict__ q
   const int s = tsrc[t], r = nsrc + trec[t];
   const float cmp = tcmp[t];
   const size_t moff = ANGLE ? (size_t)tcid[t] * K.nang * K.nz : 0;
-  for (int p = threadIdx.x; p < K.npts; p += blockDim.x) {
-    const Q7 qs = q_load(q, s, p, K.npts);
-    const Q7 qr = q_load(q, r, p, K.npts);
-    const int it = f2i_rd((qs.t + qr.t) / K.dt);
-    if ((unsigned)it >= (unsigned)K.nt) continue;
-    const int i = p % K.nz, j = p / K.nz;
-    const float z = (float)i * K.dz, x = (float)j * K.dx;
-    const float w = kdm_weight(qs, qr, S[(i + 3) + (size_t)(j + 3) * K.nzz], x, K.tan_ap * z, cmp, K);
-    if (ANGLE) {
-      const int a = kdm_angle_bin(qs, qr, K);
-      if (a >= 0 && a < K.nang) atomicAdd(hist + it, w * model[moff + i + (size_t)a * K.nz]);
-    } else {
-      atomicAdd(hist + it, w * model[p]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  // each wave takes groups of 64 chunks, tests them one per lane and works
+  // through the chunks that pass
+  for (int c0 = wave * 64; c0 < K.nchunk; c0 += nwaves * 64) {
+    const int cl = c0 + lane;
+    const bool need = cl < K.nchunk && !beyond(tmin[(size_t)s * K.nchunk + cl], tmin[(size_t)r * K.nchunk + cl], K);
+    for (uint64_t m = __ballot(need); m; m &= m - 1) {
+      const int p = (c0 + __builtin_ctzll(m)) * 64 + lane;
+      if (p >= K.npts) continue;
+      Q7 qs, qr;
+      qs.t = q[(size_t)s * KQ * K.npts + p];
+      qr.t = q[(size_t)r * KQ * K.npts + p];
+      const int it = t_index(qs.t, qr.t, K);
+      if ((unsigned)it >= (unsigned)K.nt) continue;
+      q_rest(q, s, p, K.npts, qs);
+      q_rest(q, r, p, K.npts, qr);
+      const int i = p % K.nz, j = p / K.nz;
+      const float z = (float)i * K.dz, x = (float)j * K.dx;
+      const float s_node = S[(i + 3) + (size_t)(j + 3) * K.nzz];
+        const float rs = fmaxf(qs.t / s_node, KDM_EPS);
+      const float rr = fmaxf(qr.t / s_node, KDM_EPS);
+      const float sig_e = K.tan_ap * z + KDM_EPS;
+      const float w = kdm_weight(qs, qr, s_node, rs, rr, x, sig_e, recip_refined(sig_e), cmp, K);
+      if (ANGLE) {
+        const int a = kdm_angle_bin(qs, qr, K);
+        if (a >= 0 && a < K.nang) atomicAdd(hist + it, w * model[moff + i + (size_t)a * K.nz]);
+      } else {
+        atomicAdd(hist + it, w * model[p]);
+      }
     }
   }
   __syncthreads();
@@ -254,8 +402,13 @@ struct seisfat_kdm_s {
   int nsrc = 0, nrec = 0, ntr = 0;
   KdmK K{};
   float* d_q = nullptr;    // per-table quantities, (nsrc + nrec) x 7 x npts
+  float* d_tmin = nullptr; // per-table chunk minima of T, (nsrc + nrec) x nchunk
   int* d_src = nullptr;    // trace tables
   int* d_rec = nullptr;
+  int nb = 0;              // source blocks of the adjoint
+  int* d_bsrc = nullptr;   // per block and slot: source table (-1: none)
+  int* d_bent = nullptr;   // per block: first entry (nb + 1)
+  int* d_ent = nullptr;    // entries {receiver, trace per slot}
   float* d_cmp = nullptr;
   int* d_cid = nullptr;
   bool has_tables = false;
@@ -264,6 +417,29 @@ struct seisfat_kdm_s {
 namespace {
 
 int kfail(int code, const std::string& m) { return seisfat::set_error(code, m.c_str()); }
+
+// The least float x with RN(x / dt) >= nt (+inf when none): RN(x / dt) is
+// monotone in x, so a bisection over the bit patterns of x >= 0 finds it.
+float index_threshold(float dt, int nt) {
+  auto ge = [&](float x) {
+    volatile float v = x / dt;
+    return v >= (float)nt;
+  };
+  const float big = std::numeric_limits<float>::max();
+  if (!ge(big)) return std::numeric_limits<float>::infinity();
+  if (ge(0.0f)) return 0.0f;
+  uint32_t lo = 0, hi;
+  std::memcpy(&hi, &big, 4);
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    float x;
+    std::memcpy(&x, &mid, 4);
+    (ge(x) ? hi : lo) = mid;
+  }
+  float x;
+  std::memcpy(&x, &hi, 4);
+  return x;
+}
 
 #define KHIP(expr)                                                                                  \
   do {                                                                                              \
@@ -284,7 +460,10 @@ void free_traces(seisfat_kdm_s* k) {
   kfree(k->d_rec);
   kfree(k->d_cmp);
   kfree(k->d_cid);
-  k->ntr = 0;
+  kfree(k->d_bsrc);
+  kfree(k->d_bent);
+  kfree(k->d_ent);
+  k->ntr = k->nb = 0;
 }
 
 }  // namespace
@@ -317,18 +496,45 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
   K.nx = h->nx;
   K.nzz = h->nzz;
   K.npts = (int)npts;
+  K.nchunk = (int)((npts + 63) / 64);
+  K.rec_nonneg = 0;
   K.nt = p->nt;
   K.nang = p->domain == SEISFAT_KDM_ANGLE ? p->nang : 0;
   K.dx = h->dx;
   K.dz = h->dz;
   K.dt = p->dt;
+  K.thr = index_threshold(p->dt, p->nt);
+  // two-word reciprocal of dt: zh = RN(1/dt); 1 - dt*zh is exact (one fma)
+  K.zh_dt = 1.0f / p->dt;
+  K.zl_dt = std::fma(-p->dt, K.zh_dt, 1.0f) / p->dt;
+  K.fastdiv_dt = 0;
   K.da = p->domain == SEISFAT_KDM_ANGLE ? p->da : 1.0f;
   // sigma = tanf(aperture * pi / 180) * z, the angle rounded to float as the
   // reference's float argument takes it; 1 / (2 pi) rounded from double
   K.tan_ap = std::tan((float)((double)p->aperture * M_PI / 180.0));
   K.scale = (float)(1.0 / (2.0 * M_PI));
-  const hipError_t e = hipMalloc((void**)&k->d_q, (size_t)(nsrc + nrec) * KQ * npts * sizeof(float));
+  hipError_t e = hipMalloc((void**)&k->d_q, (size_t)(nsrc + nrec) * KQ * npts * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&k->d_tmin, (size_t)(nsrc + nrec) * K.nchunk * sizeof(float));
+  if (e == hipSuccess) {
+    // exhaustive check of the known-divisor division by dt (every dividend of
+    // magnitude in [2^-100, 2^100)) against IEEE division
+    float* d = nullptr;
+    const float w[3] = {K.dt, K.zh_dt, K.zl_dt};
+    int bad = 1;
+    e = hipMalloc((void**)&d, 4 * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(d, w, sizeof(w), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d + 3, 0, sizeof(int));
+    if (e == hipSuccess) {
+      launch_fastdiv_check(d, d + 1, d + 2, 1, (int*)(d + 3), nullptr);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&bad, d + 3, sizeof(int), hipMemcpyDeviceToHost);
+    if (d) (void)hipFree(d);
+    K.fastdiv_dt = (e == hipSuccess && bad == 0) ? 1 : 0;
+  }
   if (e != hipSuccess) {
+    kfree(k->d_q);
+    kfree(k->d_tmin);
     delete k;
     return kfail(SEISFAT_ENOMEM, std::string("hipMalloc of the table quantities: ") + hipGetErrorString(e));
   }
@@ -346,7 +552,18 @@ int seisfat_kdm_set_tables(seisfat_kdm_t k, const float* d_src, const float* d_r
   const int blocks_r = (int)std::min<size_t>((nr + 255) / 256, 16384);
   hipLaunchKernelGGL(k_kdm_prep, dim3(blocks_r), dim3(256), 0, st, d_rec, k->nrec, k->K,
                      k->d_q + (size_t)k->nsrc * KQ * k->K.npts);
+  const int ntab = k->nsrc + k->nrec;
+  const size_t nw = (size_t)ntab * k->K.nchunk;
+  const int blocks_m = (int)std::min<size_t>((nw + 3) / 4, 16384);
+  hipLaunchKernelGGL(k_kdm_tmin, dim3(blocks_m), dim3(256), 0, st, k->d_q, ntab, k->K, k->d_tmin);
   KHIP(hipGetLastError());
+  // a source's run of traces may be passed over on the source minimum alone
+  // when no receiver traveltime is negative (or NaN)
+  std::vector<float> rmin((size_t)k->nrec * k->K.nchunk);
+  KHIP(hipMemcpyAsync(rmin.data(), k->d_tmin + (size_t)k->nsrc * k->K.nchunk, rmin.size() * sizeof(float),
+                      hipMemcpyDeviceToHost, st));
+  KHIP(hipStreamSynchronize(st));
+  k->K.rec_nonneg = std::all_of(rmin.begin(), rmin.end(), [](float v) { return v >= 0.0f; }) ? 1 : 0;
   k->has_tables = true;
   return SEISFAT_OK;
 }
@@ -371,6 +588,46 @@ int seisfat_kdm_set_traces(seisfat_kdm_t k, int ntr, const int* src, const int* 
   KHIP(hipMalloc((void**)&k->d_rec, ntr * sizeof(int)));
   KHIP(hipMalloc((void**)&k->d_cmp, ntr * sizeof(float)));
   KHIP(hipMalloc((void**)&k->d_cid, ntr * sizeof(int)));
+  // source blocks for the adjoint: SB runs of equal sources per block, each
+  // block's pairs sorted by (receiver, slot), traces in order within equal keys
+  std::vector<int> run0;
+  for (int t = 0; t < ntr; ++t)
+    if (t == 0 || src[t] != src[t - 1]) run0.push_back(t);
+  run0.push_back(ntr);
+  const int nruns = (int)run0.size() - 1;
+  const int nb = (nruns + KDM_SB - 1) / KDM_SB;
+  std::vector<int> bsrc((size_t)nb * KDM_SB, -1), bent(nb + 1, 0), ent;
+  struct Pair {
+    int r, k, t;
+  };
+  std::vector<Pair> pairs;
+  for (int b = 0; b < nb; ++b) {
+    pairs.clear();
+    for (int k = 0; k < KDM_SB && b * KDM_SB + k < nruns; ++k) {
+      const int r0 = run0[b * KDM_SB + k], r1 = run0[b * KDM_SB + k + 1];
+      bsrc[(size_t)b * KDM_SB + k] = src[r0];
+      for (int t = r0; t < r1; ++t) pairs.push_back({rec[t], k, t});
+    }
+    std::stable_sort(pairs.begin(), pairs.end(),
+                     [](const Pair& a, const Pair& c) { return a.r != c.r ? a.r < c.r : a.k < c.k; });
+    size_t cur = (size_t)-1;
+    for (const Pair& pr : pairs) {
+      if (cur == (size_t)-1 || ent[cur] != pr.r || ent[cur + 1 + pr.k] != -1) {
+        cur = ent.size();
+        ent.push_back(pr.r);
+        ent.insert(ent.end(), KDM_SB, -1);
+      }
+      ent[cur + 1 + pr.k] = pr.t;
+    }
+    bent[b + 1] = (int)(ent.size() / (1 + KDM_SB));
+  }
+  KHIP(hipMalloc((void**)&k->d_bsrc, bsrc.size() * sizeof(int)));
+  KHIP(hipMalloc((void**)&k->d_bent, bent.size() * sizeof(int)));
+  KHIP(hipMalloc((void**)&k->d_ent, ent.size() * sizeof(int)));
+  KHIP(hipMemcpy(k->d_bsrc, bsrc.data(), bsrc.size() * sizeof(int), hipMemcpyHostToDevice));
+  KHIP(hipMemcpy(k->d_bent, bent.data(), bent.size() * sizeof(int), hipMemcpyHostToDevice));
+  KHIP(hipMemcpy(k->d_ent, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice));
+  k->nb = nb;
   std::vector<int> cid(ntr, 0);
   if (k->p.domain == SEISFAT_KDM_ANGLE) cid.assign(cmp_id, cmp_id + ntr);
   KHIP(hipMemcpy(k->d_src, src, ntr * sizeof(int), hipMemcpyHostToDevice));
@@ -388,11 +645,13 @@ int seisfat_kdm_adjoint(seisfat_kdm_t k, const float* d_data, float* d_model, vo
   hipStream_t st = (hipStream_t)stream;
   const int blocks = (k->K.npts + 255) / 256;
   if (k->p.domain == SEISFAT_KDM_IMAGE)
-    hipLaunchKernelGGL(k_kdm_adjoint_image, dim3(blocks), dim3(256), 0, st, k->d_q, k->h->d_spad, k->d_src,
-                       k->d_rec, k->d_cmp, k->ntr, k->nsrc, d_data, d_model, k->K);
+    hipLaunchKernelGGL((k_kdm_adjoint<false, KDM_SB>), dim3(blocks), dim3(256), 0, st, k->d_q, k->d_tmin,
+                       k->h->d_spad, k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data,
+                       d_model, k->K);
   else
-    hipLaunchKernelGGL(k_kdm_adjoint_angle, dim3(blocks), dim3(256), 0, st, k->d_q, k->h->d_spad, k->d_src,
-                       k->d_rec, k->d_cmp, k->d_cid, k->ntr, k->nsrc, d_data, d_model, k->K);
+    hipLaunchKernelGGL((k_kdm_adjoint<true, KDM_SB>), dim3(blocks), dim3(256), 0, st, k->d_q, k->d_tmin,
+                       k->h->d_spad, k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data,
+                       d_model, k->K);
   KHIP(hipGetLastError());
   return SEISFAT_OK;
 }
@@ -404,11 +663,11 @@ int seisfat_kdm_forward(seisfat_kdm_t k, const float* d_model, float* d_data, vo
   hipStream_t st = (hipStream_t)stream;
   const size_t lds = (size_t)k->K.nt * sizeof(float);
   if (k->p.domain == SEISFAT_KDM_IMAGE)
-    hipLaunchKernelGGL(k_kdm_forward<false>, dim3(k->ntr), dim3(256), lds, st, k->d_q, k->h->d_spad, k->d_src,
-                       k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
+    hipLaunchKernelGGL(k_kdm_forward<false>, dim3(k->ntr), dim3(256), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
+                       k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
   else
-    hipLaunchKernelGGL(k_kdm_forward<true>, dim3(k->ntr), dim3(256), lds, st, k->d_q, k->h->d_spad, k->d_src,
-                       k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
+    hipLaunchKernelGGL(k_kdm_forward<true>, dim3(k->ntr), dim3(256), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
+                       k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
   KHIP(hipGetLastError());
   return SEISFAT_OK;
 }
@@ -418,6 +677,7 @@ int seisfat_kdm_destroy(seisfat_kdm_t k) {
   (void)hipDeviceSynchronize();
   free_traces(k);
   kfree(k->d_q);
+  kfree(k->d_tmin);
   delete k;
   return SEISFAT_OK;
 }

@@ -172,6 +172,24 @@ def c5(nshots=128, seed=5):
                     extra={"s_true": (1.0 / v_true).astype(np.float32), "shot_spacing_m": spacing})
 
 
+def migration(seed=2, src_step=200.0, rec_step=50.0):
+    """Kirchhoff migration workload with the reference's sample parameters
+    (run/parameters.txt:5-12,72-90): a 1001 x 1001 model at 10 m (layered,
+    seed 2), time_samples 1001 at 1 ms, max_frequency 30 Hz, mig_aperture
+    30 deg, angles to 60 deg every 2 deg, offsets below 2000 m; surface
+    sources every 200 m and receivers every 50 m (the sample's geometry files
+    are not shipped)."""
+    nz = nx = 1001
+    dh = 10.0
+    v = layered_model(nz, nx, dh, seed)
+    xs = np.arange(0.0, (nx - 1) * dh + 1.0, src_step).astype(np.float32)
+    xr = np.arange(0.0, (nx - 1) * dh + 1.0, rec_step).astype(np.float32)
+    return Workload("MIG", nz, nx, dh, dh, (1.0 / v).astype(np.float32), [], xr, np.zeros(len(xr), np.float32),
+                    extra={"xs": xs, "zs": np.zeros(len(xs), np.float32),
+                           "params": dict(nt=1001, dt=1e-3, fmax=30.0, aperture=30.0, max_angle=60.0, da=2.0,
+                                          max_offset=2000.0)})
+
+
 def rough(nz, nx, dz, dx, seed=0, vlo=1500.0, vhi=6000.0, sigma=0.0):
     """Rough, high-contrast model for parity tests: white-noise velocity
     U(vlo, vhi) per node (optionally Gaussian-smoothed with `sigma` cells and
