@@ -58,6 +58,8 @@ def measured_traffic():
 
 ANI_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "ani_c4_128shots_traffic.txt")
 ADJ_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "adjoint_c5_128shots_traffic.txt")
+MIG_COUNTERS_FILE = os.path.join(ROOT, "profiles", "kdm_migration_counters.txt")
+VALU_PEAK_TIPS = 1024 * 2.4e9 / 4 / 1e12  # wave64 VALU instructions per second, 256 CUs x 4 SIMDs
 
 
 def profile_figures(path):
@@ -406,21 +408,25 @@ def bench_migration(steps=3, cpu_traces=200):
         reach += int((torch.floor((Ts[K.src[t]] + Tr[K.rec[t]]) / dt32) < nt_deep).sum())
     del Ts, Tr
     pairs = K.ntr * w.nz * w.nx
-    # algorithmic bytes of one adjoint launch: per reaching pair the receiver
-    # table's five image-domain quantities (T, |det H|, |grad T|, angle,
-    # |cos|) and one data sample (24 B); per image point and source run the
-    # source's five quantities (20 B); the image read and written (8 B)
-    runs = len(set(K.src.tolist()))
-    alg = 24 * reach + 20 * w.nz * w.nx * runs + 8 * w.nz * w.nx
-    achieved = alg / (adj_ms / 1e3) / 1e9
+    # the launch is VALU-bound (counters): its wave-level VALU instructions
+    # (the committed profile's per-pair count times the pairs that reach) over
+    # the launch time, against the issue rate of 1024 SIMDs at one wave64
+    # instruction per 4 cycles at 2.4 GHz
+    prof = profile_figures(MIG_COUNTERS_FILE)
+    per_pair = prof.get("valu_instructions_per_pair")
+    valu = per_pair * reach / 64 if per_pair else None
+    achieved = valu / (adj_ms / 1e3) / 1e12 if valu else None
+    fetch = prof.get("fetch_bytes_per_launch")
     out["image_deep_record"] = {
         "nt": nt_deep, "traces": K.ntr, "adjoint_ms": adj_ms, "pairs_reaching": reach,
         "reaching_fraction": reach / pairs, "adjoint_pairs_per_s": pairs / (adj_ms / 1e3),
         "adjoint_reaching_pairs_per_s": reach / (adj_ms / 1e3),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TIPS, "unit": "T wave-instr/s",
+                     "frac": achieved / VALU_PEAK_TIPS if achieved else None,
+                     "traffic": fetch, "hbm_GBps_at_traffic": fetch / (adj_ms / 1e3) / 1e9 if fetch else None,
+                     "valu_instructions_per_pair": per_pair,
                      "scope": "k_kdm_adjoint<image> launch, HIP events on torch's stream",
-                     "algorithmic_bytes_per_launch": alg}}
+                     "source": os.path.relpath(MIG_COUNTERS_FILE, ROOT)}}
     K.close()
     e.close()
     return out

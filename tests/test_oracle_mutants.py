@@ -7,6 +7,8 @@ A pin set that passes a broken oracle pins nothing, so this is what shows the
 pins bind each part of the local solver (src/modeling/modeling.cu:308-357),
 the initialisation, the padding, the picks and the qP slowness.  The mistakes
 are planted in the oracle, on the CPU; nothing here touches the GPU path.
+The migration restatement (weights, indices, convolutions, Ricker wavelet) is
+checked the same way against tests/test_migration_oracle.py.
 """
 import os
 import subprocess
@@ -72,6 +74,39 @@ MUTANTS = [
     ("compress_constant_code_0", "(mx == mn) ? uint16_t(1)", "(mx == mn) ? uint16_t(0)"),
 ]
 
+# Kirchhoff migration (src/migration/migration.cu:54-73,197-261,398-756), checked
+# by tests/test_migration_oracle.py
+_TAPER = "const float taper = std::exp(-0.5f * ((x - cmp) / (sigma + KDM_EPS)) * ((x - cmp) / (sigma + KDM_EPS)));"
+MIG_MUTANTS = [
+    ("kdm_dTs_dx_over_dz", "const float dTs_dx = 0.5f * (at(Ts, i, j + 1) - at(Ts, i, j - 1)) / dx;",
+     "const float dTs_dx = 0.5f * (at(Ts, i, j + 1) - at(Ts, i, j - 1)) / dz;"),
+    ("kdm_d2Tr_dz2_over_dxdz", "const float d2Tr_dz2 = (at(Tr, i + 1, j) - 2.0f * at(Tr, i, j) + at(Tr, i - 1, j)) / (dz * dz);",
+     "const float d2Tr_dz2 = (at(Tr, i + 1, j) - 2.0f * at(Tr, i, j) + at(Tr, i - 1, j)) / (dx * dz);"),
+    ("kdm_cross_derivative_half_stencil", "at(Ts, i - 1, j - 1)) / (4.0f * dx * dz);", "at(Ts, i - 1, j - 1)) / (2.0f * dx * dz);"),
+    ("kdm_jterm_norms_multiplied", "std::sqrt(detHs * detHr) * S[c] / (norm_Ts * norm_Tr);",
+     "std::sqrt(detHs * detHr) * S[c] * (norm_Ts * norm_Tr);"),
+    ("kdm_normal_sign_in_angle", "std::fmax(-1.0f, ux_s * nx_n + uz_s * nz_n)", "std::fmax(-1.0f, ux_s * nx_n - uz_s * nz_n)"),
+    ("kdm_obliquity_without_sqrt", "const float obliquity = std::sqrt(std::fmax(0.0f, cos_s) * std::fmax(0.0f, cos_r));",
+     "const float obliquity = std::fmax(0.0f, cos_s) * std::fmax(0.0f, cos_r);"),
+    ("kdm_spreading_source_only", "const float G = 1.0f / std::sqrt(R_s * R_r);", "const float G = 1.0f / std::sqrt(R_s * R_s);"),
+    ("kdm_reflection_factor_0_5", "const float Rf = 1.0f + 0.2f * std::cos(theta);", "const float Rf = 1.0f + 0.5f * std::cos(theta);"),
+    ("kdm_taper_in_z", _TAPER, _TAPER.replace("(x - cmp)", "(z - cmp)")),
+    ("kdm_aperture_not_in_radians", "const float tan_ap = std::tan((float)((double)aperture * M_PI / 180.0));",
+     "const float tan_ap = std::tan(aperture);"),
+    ("kdm_reflection_angle_not_halved", "const float ra = 0.5f * std::acos(", "const float ra = 1.0f * std::acos("),
+    ("kdm_time_index_rounded", "const int tId = f2i_rd(T / dt);", "const int tId = f2i_rd(T / dt + 0.5f);"),
+    ("kdm_angle_index_transposed", "mId = (size_t)(i - NB) + (size_t)aId * nz + (size_t)cmpId * na * nz;",
+     "mId = (size_t)aId + (size_t)(i - NB) * na + (size_t)cmpId * na * nz;"),
+    ("conv_shift_without_tenth", "const int shift = nw / 2 + nw / 10;\n  for (int t = 0;",
+     "const int shift = nw / 2;\n  for (int t = 0;"),
+    ("conv_adjoint_is_a_convolution", "acc += (double)w[m] * (double)d[k0 + m];",
+     "acc += (double)w[nw - 1 - m] * (double)d[k0 + m];"),
+    ("conv_no_nfft_scale", "o[k] = (float)(acc * nfft);", "o[k] = (float)acc;"),
+    ("ricker_fc_without_sqrt_pi", "const float fc = fmax / (3.0f * std::sqrt((float)M_PI));\n  const int nw",
+     "const float fc = fmax / 3.0f;\n  const int nw"),
+    ("ricker_without_factor_2", "w[k] = (1.0f - 2.0f * arg) * std::exp(-arg);", "w[k] = (1.0f - arg) * std::exp(-arg);"),
+]
+
 _CXX = ["g++", "-O1", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-shared"]
 
 
@@ -82,16 +117,16 @@ def _build(text, path):
     subprocess.run(_CXX + ["-o", path, src], check=True, capture_output=True)
 
 
-def _pins_pass(lib_path):
+def _pins_pass(lib_path, tests="test_oracle.py"):
     env = dict(os.environ, SEISFAT_ORACLE_LIB=lib_path)
-    r = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_oracle.py"), "-x", "-q",
+    r = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", tests), "-x", "-q",
                         "-p", "no:cacheprovider"], cwd=ROOT, env=env, capture_output=True, text=True)
     return r.returncode == 0, r.stdout[-600:]
 
 
-def test_every_planted_mistake_fails_a_pin():
+def _check(mutants, tests):
     base = open(SRC).read()
-    for name, old, new in MUTANTS:
+    for name, old, new in mutants:
         assert base.count(old) == 1, f"{name}: anchor text occurs {base.count(old)} times"
         assert old != new, name
     with tempfile.TemporaryDirectory() as tmp:
@@ -99,14 +134,22 @@ def test_every_planted_mistake_fails_a_pin():
             name, old, new = case
             path = os.path.join(tmp, f"{name}.so")
             _build(base.replace(old, new), path)
-            return name, _pins_pass(path)
+            return name, _pins_pass(path, tests)
 
         # the unmutated source built the same way must pass (else the harness proves nothing)
         path = os.path.join(tmp, "control.so")
         _build(base, path)
-        ok, out = _pins_pass(path)
+        ok, out = _pins_pass(path, tests)
         assert ok, "unmutated oracle fails its pins:\n" + out
         with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
-            results = list(ex.map(run, MUTANTS))
+            results = list(ex.map(run, mutants))
     survivors = [name for name, (passed, _) in results if passed]
     assert not survivors, f"planted mistakes that no pin catches: {survivors}"
+
+
+def test_every_planted_mistake_fails_a_pin():
+    _check(MUTANTS, "test_oracle.py")
+
+
+def test_every_planted_migration_mistake_fails_a_pin():
+    _check(MIG_MUTANTS, "test_migration_oracle.py")
