@@ -50,6 +50,11 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
  * quantities are evaluated here once (the reference re-evaluates them for
  * every trace, with the same operations). */
 int seisfat_kdm_set_tables(seisfat_kdm_t k, const float* d_src, const float* d_rec, void* stream);
+/* One table at a time: table `index` of the sources (which = 0) or receivers
+ * (which = 1) from a padded grid in device memory, e.g. seisfat_batch_device_T
+ * of a batched solve, so tables need not be gathered into one array.  The
+ * operators run once every table has been set (by either call). */
+int seisfat_kdm_set_table(seisfat_kdm_t k, int which, int index, const float* d_table, void* stream);
 /* The traces, in the order the migration loops visit them
  * (src/migration/KDM.cu:14-53): trace t pairs source table src[t] with
  * receiver table rec[t], at CMP position cmp[t] (metres) and CMP bin

@@ -6,6 +6,7 @@ Everything is built in-tree so the shared objects travel with the source tree:
   seisfat2d_amd/seisfat2d_modeling     the per-shot modeling executable
   seisfat2d_amd/seisfat2d_tables       traveltime tables for migration
   seisfat2d_amd/seisfat2d_inversion    first-arrival tomography
+  seisfat2d_amd/seisfat2d_migration    Kirchhoff depth migration (IDKDM, ADKDM, IDLSKDM, ADLSKDM)
 """
 from __future__ import annotations
 
@@ -24,6 +25,7 @@ HOST_LIB = os.path.join(PKG, "libseisfat2d_host.so")
 MODELING_EXE = os.path.join(PKG, "seisfat2d_modeling")
 TABLES_EXE = os.path.join(PKG, "seisfat2d_tables")
 INVERSION_EXE = os.path.join(PKG, "seisfat2d_inversion")
+MIGRATION_EXE = os.path.join(PKG, "seisfat2d_migration")
 
 # Device arithmetic contract: no FMA contraction, correctly rounded fp32
 # division and square root (bit parity with the oracle); no packed fp32 math.
@@ -38,6 +40,10 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+# the migration driver allocates its trace and model buffers with the HIP runtime
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIP_HOST = ["-D__HIP_PLATFORM_AMD__", "-isystem", os.path.join(ROCM, "include")]
+HIP_LINK = ["-L", os.path.join(ROCM, "lib"), "-lamdhip64"]
 
 
 def _hipcc():
@@ -70,16 +76,16 @@ def build_host(verbose=False):
     if not srcs:
         return None
     tmp = HOST_LIB + ".tmp"
-    _run(["g++", *HOST_FLAGS, "-fopenmp", "-shared", "-I", INCLUDE, "-o", tmp, *srcs,
-          "-L", PKG, "-lseisfat2d_eik", "-Wl,-rpath,$ORIGIN"])
+    _run(["g++", *HOST_FLAGS, *HIP_HOST, "-fopenmp", "-shared", "-I", INCLUDE, "-o", tmp, *srcs,
+          "-L", PKG, "-lseisfat2d_eik", *HIP_LINK, "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, HOST_LIB)
     # one executable per driver: <name>_main.cpp -> seisfat2d_<name>
     for f in sorted(os.listdir(HOST)):
         if f.endswith("_main.cpp"):
             exe = os.path.join(PKG, "seisfat2d_" + f[:-len("_main.cpp")])
             tmp = exe + ".tmp"
-            _run(["g++", *HOST_FLAGS, "-fopenmp", "-I", INCLUDE, "-o", tmp, os.path.join(HOST, f), "-L", PKG,
-                  "-lseisfat2d_host", "-lseisfat2d_eik", "-Wl,-rpath,$ORIGIN"])
+            _run(["g++", *HOST_FLAGS, *HIP_HOST, "-fopenmp", "-I", INCLUDE, "-o", tmp, os.path.join(HOST, f), "-L",
+                  PKG, "-lseisfat2d_host", "-lseisfat2d_eik", *HIP_LINK, "-Wl,-rpath,$ORIGIN"])
             os.replace(tmp, exe)
     return HOST_LIB
 

@@ -30,6 +30,7 @@ EXPORTS = [
     # include/seisfat2d_kdm.h
     "seisfat_kdm_create", "seisfat_kdm_set_tables", "seisfat_kdm_set_traces", "seisfat_kdm_adjoint",
     "seisfat_kdm_forward", "seisfat_kdm_destroy", "seisfat_trace_convolve", "seisfat_ricker",
+    "seisfat_kdm_set_table",
 ]
 KDM_IMAGE, KDM_ANGLE = 0, 1
 
@@ -122,6 +123,7 @@ def lib():
         L.seisfat_kdm_create.argtypes = [vp, C.POINTER(KdmParams), i, i, C.POINTER(vp)]
         L.seisfat_kdm_set_tables.argtypes = [vp, vp, vp, vp]
         L.seisfat_kdm_set_traces.argtypes = [vp, i, vp, vp, vp, vp]
+        L.seisfat_kdm_set_table.argtypes = [vp, i, i, vp, vp]
         L.seisfat_kdm_adjoint.argtypes = [vp, vp, vp, vp]
         L.seisfat_kdm_forward.argtypes = [vp, vp, vp, vp]
         L.seisfat_kdm_destroy.argtypes = [vp]

@@ -411,7 +411,9 @@ struct seisfat_kdm_s {
   int* d_ent = nullptr;    // entries {receiver, trace per slot}
   float* d_cmp = nullptr;
   int* d_cid = nullptr;
-  bool has_tables = false;
+  std::vector<char> tab_set;  // per table (sources, then receivers): quantities evaluated
+  int tabs_missing = 0;
+  bool rec_dirty = true;       // receiver minima changed since rec_nonneg was derived
 };
 
 namespace {
@@ -538,6 +540,8 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
     delete k;
     return kfail(SEISFAT_ENOMEM, std::string("hipMalloc of the table quantities: ") + hipGetErrorString(e));
   }
+  k->tab_set.assign(nsrc + nrec, 0);
+  k->tabs_missing = nsrc + nrec;
   *out = k;
   return SEISFAT_OK;
 }
@@ -557,16 +561,53 @@ int seisfat_kdm_set_tables(seisfat_kdm_t k, const float* d_src, const float* d_r
   const int blocks_m = (int)std::min<size_t>((nw + 3) / 4, 16384);
   hipLaunchKernelGGL(k_kdm_tmin, dim3(blocks_m), dim3(256), 0, st, k->d_q, ntab, k->K, k->d_tmin);
   KHIP(hipGetLastError());
-  // a source's run of traces may be passed over on the source minimum alone
-  // when no receiver traveltime is negative (or NaN)
-  std::vector<float> rmin((size_t)k->nrec * k->K.nchunk);
-  KHIP(hipMemcpyAsync(rmin.data(), k->d_tmin + (size_t)k->nsrc * k->K.nchunk, rmin.size() * sizeof(float),
-                      hipMemcpyDeviceToHost, st));
-  KHIP(hipStreamSynchronize(st));
-  k->K.rec_nonneg = std::all_of(rmin.begin(), rmin.end(), [](float v) { return v >= 0.0f; }) ? 1 : 0;
-  k->has_tables = true;
+  std::fill(k->tab_set.begin(), k->tab_set.end(), 1);
+  k->tabs_missing = 0;
+  k->rec_dirty = true;
   return SEISFAT_OK;
 }
+
+int seisfat_kdm_set_table(seisfat_kdm_t k, int which, int index, const float* d_table, void* stream) {
+  if (!k || !d_table) return kfail(SEISFAT_EINVAL, "null operator or table pointer");
+  if (which != 0 && which != 1) return kfail(SEISFAT_EINVAL, "which must be 0 (source) or 1 (receiver)");
+  const int n = which ? k->nrec : k->nsrc;
+  if (index < 0 || index >= n) return kfail(SEISFAT_EINVAL, "table index out of range");
+  const int tab = (which ? k->nsrc : 0) + index;
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = (int)std::min<size_t>(((size_t)k->K.npts + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_table, 1, k->K,
+                     k->d_q + (size_t)tab * KQ * k->K.npts);
+  const int blocks_m = (int)std::min<size_t>(((size_t)k->K.nchunk + 3) / 4, 16384);
+  hipLaunchKernelGGL(k_kdm_tmin, dim3(blocks_m), dim3(256), 0, st, k->d_q + (size_t)tab * KQ * k->K.npts, 1,
+                     k->K, k->d_tmin + (size_t)tab * k->K.nchunk);
+  KHIP(hipGetLastError());
+  if (!k->tab_set[tab]) {
+    k->tab_set[tab] = 1;
+    --k->tabs_missing;
+  }
+  if (which) k->rec_dirty = true;
+  return SEISFAT_OK;
+}
+
+namespace {
+
+// Before a launch: every table set, and rec_nonneg derived from the current
+// receiver minima (a source's run of traces may be passed over on the source
+// minimum alone when no receiver traveltime is negative or NaN).
+int ready(seisfat_kdm_s* k, hipStream_t st) {
+  if (k->tabs_missing) return kfail(SEISFAT_ESTATE, "tables not set (seisfat_kdm_set_tables / _set_table)");
+  if (k->rec_dirty) {
+    std::vector<float> rmin((size_t)k->nrec * k->K.nchunk);
+    KHIP(hipMemcpyAsync(rmin.data(), k->d_tmin + (size_t)k->nsrc * k->K.nchunk, rmin.size() * sizeof(float),
+                        hipMemcpyDeviceToHost, st));
+    KHIP(hipStreamSynchronize(st));
+    k->K.rec_nonneg = std::all_of(rmin.begin(), rmin.end(), [](float v) { return v >= 0.0f; }) ? 1 : 0;
+    k->rec_dirty = false;
+  }
+  return SEISFAT_OK;
+}
+
+}  // namespace
 
 int seisfat_kdm_set_traces(seisfat_kdm_t k, int ntr, const int* src, const int* rec, const float* cmp,
                            const int* cmp_id) {
@@ -640,9 +681,9 @@ int seisfat_kdm_set_traces(seisfat_kdm_t k, int ntr, const int* src, const int* 
 
 int seisfat_kdm_adjoint(seisfat_kdm_t k, const float* d_data, float* d_model, void* stream) {
   if (!k || !d_data || !d_model) return kfail(SEISFAT_EINVAL, "null operator or buffer");
-  if (!k->has_tables) return kfail(SEISFAT_ESTATE, "tables not set (seisfat_kdm_set_tables)");
-  if (k->ntr == 0) return SEISFAT_OK;
   hipStream_t st = (hipStream_t)stream;
+  if (const int rc = ready(k, st)) return rc;
+  if (k->ntr == 0) return SEISFAT_OK;
   const int blocks = (k->K.npts + 255) / 256;
   if (k->p.domain == SEISFAT_KDM_IMAGE)
     hipLaunchKernelGGL((k_kdm_adjoint<false, KDM_SB>), dim3(blocks), dim3(256), 0, st, k->d_q, k->d_tmin,
@@ -658,9 +699,9 @@ int seisfat_kdm_adjoint(seisfat_kdm_t k, const float* d_data, float* d_model, vo
 
 int seisfat_kdm_forward(seisfat_kdm_t k, const float* d_model, float* d_data, void* stream) {
   if (!k || !d_data || !d_model) return kfail(SEISFAT_EINVAL, "null operator or buffer");
-  if (!k->has_tables) return kfail(SEISFAT_ESTATE, "tables not set (seisfat_kdm_set_tables)");
-  if (k->ntr == 0) return SEISFAT_OK;
   hipStream_t st = (hipStream_t)stream;
+  if (const int rc = ready(k, st)) return rc;
+  if (k->ntr == 0) return SEISFAT_OK;
   const size_t lds = (size_t)k->K.nt * sizeof(float);
   if (k->p.domain == SEISFAT_KDM_IMAGE)
     hipLaunchKernelGGL(k_kdm_forward<false>, dim3(k->ntr), dim3(256), lds, st, k->d_q, k->d_tmin, k->h->d_spad,

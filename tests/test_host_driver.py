@@ -180,3 +180,105 @@ def test_tables_driver_matches_oracle(tmp_path, oracle, ani):
             assert got.size == (nz + 6) * (nx + 6)
             assert np.array_equal(got.reshape(nx + 6, nz + 6), solve(x, z)), (prefix, k)
     assert not os.path.exists(os.path.join(tmp, "tables", f"eikonal_rec_{len(recs) + 1}.bin"))
+
+
+# -- the migration driver (src/migration_main.cpp) ------------------------------------
+
+MIG_KEYS = """
+mig_type = {t}
+anisotropy = false
+time_samples = {nt}
+time_spacing = 2e-3
+max_frequency = 30.0
+mig_aperture = 30
+mig_max_angle = 60
+mig_angle_spacing = 3.0
+mig_max_offset = 300
+mig_max_iteration = 2
+mig_smooth_lambda = 0.001
+mig_data_folder = {tmp}/data/
+mig_data_prefix = mig_input_shot_
+mig_tables_folder = {tmp}/tables/
+mig_seismic_folder = {tmp}/seismic/
+mig_residuo_folder = {tmp}/residuo/
+"""
+
+
+def _mig_case(tmp, t, nt=300, drop=None):
+    p, s = write_case(tmp)
+    keys = MIG_KEYS.format(t=t, nt=nt, tmp=tmp)
+    if drop:
+        keys = "\n".join(l for l in keys.splitlines() if not l.startswith(drop))
+    with open(p, "a") as f:
+        f.write(keys)
+    for d in ("data", "tables", "seismic", "residuo"):
+        os.makedirs(os.path.join(tmp, d), exist_ok=True)
+    nrec = len(np.arange(0, 61, 3))
+    rng = np.random.default_rng(t)
+    for k in range(3):
+        rng.standard_normal((nrec, nt)).astype(np.float32).tofile(os.path.join(tmp, "data", f"mig_input_shot_{k + 1}.bin"))
+    return p, s
+
+
+@pytest.mark.parametrize("drop,msg", [("mig_data_folder", "mig_data_folder"), ("mig_type", "mig_type"),
+                                      ("mig_max_iteration", "mig_max_iteration")])
+def test_migration_driver_rejects_missing_keys(tmp_path, drop, msg):
+    """A missing key fails before any device work, naming the key (LS keys
+    only for the least-squares types)."""
+    tmp = str(tmp_path)
+    p, _ = _mig_case(tmp, 2, drop=drop)
+    r = subprocess.run([build.MIGRATION_EXE, p], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=60)
+    assert r.returncode != 0 and msg in r.stderr, r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("t", [0, 1, 2, 3])
+def test_migration_driver_matches_oracle(tmp_path, oracle, t):
+    """seisfat2d_migration runs the reference's four migrations from a
+    parameter file: tables solved on the device (and written as
+    eikonal_{src,rec}_<k>.bin), the seismic files read per source, the result
+    written under the reference's name; checked against the oracle's
+    restatement on the oracle's own tables."""
+    from seisfat2d_amd import migration as mig
+    tmp = str(tmp_path)
+    nt = 300
+    p, s = _mig_case(tmp, t, nt)
+    r = subprocess.run([build.MIGRATION_EXE, p], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    nz, nx, dh = 41, 61, 10.0
+    S = oracle.expand_boundary(s)
+    xs = np.array([100.0 + 150.0 * k for k in range(3)], np.float32)
+    xr = (np.arange(0, nx, 3) * dh).astype(np.float32)
+    ts = np.stack([oracle.solve_iso(S, dh, dh, float(x), 0.0) for x in xs])
+    tr = np.stack([oracle.solve_iso(S, dh, dh, float(x), 0.0) for x in xr])
+    for k in range(len(xr)):
+        got = np.fromfile(os.path.join(tmp, "tables", f"eikonal_rec_{k + 1}.bin"), np.float32)
+        assert np.array_equal(got.reshape(nx + 6, nz + 6), tr[k])
+    nang, dcmp, min_cmp, ncmp = mig.gathers(xs, xr, 3.0, 60.0)
+    src, rec, cmp, cid = mig.trace_list(xs, xr, 300.0, min_cmp, dcmp)
+    seismic = np.stack([np.fromfile(os.path.join(tmp, "data", f"mig_input_shot_{k + 1}.bin"), np.float32)
+                        .reshape(len(xr), nt) for k in range(3)])
+    dobs = np.ascontiguousarray(seismic[src, rec])
+    w = oracle.ricker(2e-3, 30.0)
+    angle = t in (1, 3)
+    domain = 1 if angle else 0
+    shape = (ncmp, nang, nz) if angle else (nx, nz)
+    name = ["IDKDM", "ADKDM", "IDLSKDM", "ADLSKDM"][t]
+    size = f"{nz}x{ncmp}x{nang}" if angle else f"{nz}x{nx}"
+    out = os.path.join(tmp, "seismic", f"{name}_result_{size}{'_iteration_2' if t >= 2 else ''}.bin")
+    got = np.fromfile(out, np.float32).reshape(shape)
+    op = dict(dt=2e-3, dx=dh, dz=dh, aperture=30.0, da=3.0, nang=nang)
+    if t < 2:
+        d = np.ascontiguousarray(oracle.trace_convolve(dobs, w, True))
+        want = oracle.kdm(domain, True, S, ts, tr, src, rec, cmp, cid, d, np.zeros(shape, np.float32), **op)
+        bound = oracle.kdm(domain, True, S, ts, tr, src, rec, cmp, cid, np.abs(d), np.zeros(shape, np.float32), **op)
+        bad = np.abs(got - want) > 1e-5 * bound + 1e-30
+        assert np.abs(want).max() > 0 and bad.mean() <= (0 if not angle else 1e-3), bad.sum()
+    else:
+        want, res = oracle.lskdm(domain, S, ts, tr, src, rec, cmp, cid, dobs, w, 2, 0.001, 2e-3, dh, dh, 30.0,
+                                 shape, 3.0, nang)
+        got_res = np.loadtxt(os.path.join(tmp, "residuo", f"{name}_convergence_2_iterations.txt"))
+        assert np.allclose(got_res, res, rtol=1e-4, atol=0), (got_res, res)
+        assert np.linalg.norm(got - want) <= 1e-3 * np.linalg.norm(want)
