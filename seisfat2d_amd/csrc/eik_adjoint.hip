@@ -44,7 +44,23 @@ __global__ void k_adj_seed(const size_t* __restrict__ node, const int* __restric
 // are taken from the fp64 trajectory; the adjoint recurrence runs in fp32.
 struct ConstsD {
   double dx, dz, dx2i, dz2i, sum, prod, hyp;
+  double r_dx, r_dz, r_sum, r_hyp;  // RN(1/divisor), for cdiv
 };
+
+// RN(a / b) for a model-constant divisor b, from y = RN(1/b) (computed on the
+// host by IEEE division): q = RN(a y) is within one ulp of a/b, the residual
+// a - b q is exact (one fma), and RN(q + (a - b q) y) is the correctly
+// rounded quotient (Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al.,
+// Handbook of Floating-Point Arithmetic, the FMA-based division theorems),
+// for every a whose quotient neither underflows nor overflows.  Three
+// instructions instead of the general division's scale / reciprocal /
+// iteration / fix-up sequence; the dividends here are traveltime differences
+// and slowness products of a validated model, far from both limits.
+__device__ __forceinline__ double cdiv(double a, double b, double y) {
+  const double q = a * y;
+  const double r = __builtin_fma(-b, q, a);
+  return __builtin_fma(r, y, q);
+}
 
 __device__ __forceinline__ double dpp_shr1_d(double old, double src) {
   const long long o = __double_as_longlong(old), s = __double_as_longlong(src);
@@ -70,11 +86,14 @@ __device__ __forceinline__ double local_d(double tv, double te, double tev, doub
   if ((tv <= te + k.dx * sref) && (te <= tv + k.dz * sref) && (te - tev >= 0.0) && (tv - tev >= 0.0)) {
     const double ta = tev + te - tv;
     const double tb = tev - te + tv;
-    t2D = ((tb * k.dz2i + ta * k.dx2i) + sqrt(4.0 * sref * sref * k.sum - k.prod * (ta - tb) * (ta - tb))) / k.sum;
-  } else if ((te - tev <= sref * k.dz * k.dz / k.hyp) && (te - tev > 0.0)) {
-    t2D = te + k.dx * sqrt(sref * sref - ((te - tev) / k.dz) * ((te - tev) / k.dz));
-  } else if ((tv - tev <= sref * k.dx * k.dx / k.hyp) && (tv - tev > 0.0)) {
-    t2D = tv + k.dz * sqrt(sref * sref - ((tv - tev) / k.dx) * ((tv - tev) / k.dx));
+    t2D = cdiv((tb * k.dz2i + ta * k.dx2i) + sqrt(4.0 * sref * sref * k.sum - k.prod * (ta - tb) * (ta - tb)), k.sum,
+               k.r_sum);
+  } else if ((te - tev <= cdiv(sref * k.dz * k.dz, k.hyp, k.r_hyp)) && (te - tev > 0.0)) {
+    const double q = cdiv(te - tev, k.dz, k.r_dz);
+    t2D = te + k.dx * sqrt(sref * sref - q * q);
+  } else if ((tv - tev <= cdiv(sref * k.dx * k.dx, k.hyp, k.r_hyp)) && (tv - tev > 0.0)) {
+    const double p = cdiv(tv - tev, k.dx, k.r_dx);
+    t2D = tv + k.dz * sqrt(sref * sref - p * p);
   }
   return fmin(t1D, fmin(t2D, 1e6));
 }
@@ -339,17 +358,17 @@ __device__ __forceinline__ AdjCoef adj_local(double tv, double te, double tev, d
     const double ta = tev + te - tv;
     const double tb = tev - te + tv;
     const double sq = sqrt(4.0 * sref * sref * k.sum - k.prod * (ta - tb) * (ta - tb));
-    tbr = ((tb * k.dz2i + ta * k.dx2i) + sq) / k.sum;
+    tbr = cdiv((tb * k.dz2i + ta * k.dx2i) + sq, k.sum, k.r_sum);
     taken = true;
     // t = ((tb dz2i + ta dx2i) + sqrt(4 S^2 D - dz2i dx2i (ta - tb)^2)) / D
     const double kk = 2.0 * k.prod * (ta - tb) / sq;
-    p_te = (k.dx2i - k.dz2i - kk) / k.sum;
-    p_tv = (k.dz2i - k.dx2i + kk) / k.sum;
+    p_te = cdiv(k.dx2i - k.dz2i - kk, k.sum, k.r_sum);
+    p_tv = cdiv(k.dz2i - k.dx2i + kk, k.sum, k.r_sum);
     p_tev = 1.0;
     p_s = 4.0 * sref / sq;
-  } else if ((te - tev <= sref * k.dz * k.dz / k.hyp) && (te - tev > 0.0)) {
+  } else if ((te - tev <= cdiv(sref * k.dz * k.dz, k.hyp, k.r_hyp)) && (te - tev > 0.0)) {
     // t = te + dx sqrt(S^2 - q^2), q = (te - tev) / dz
-    const double q = (te - tev) / k.dz;
+    const double q = cdiv(te - tev, k.dz, k.r_dz);
     const double Q = sqrt(sref * sref - q * q);
     tbr = te + k.dx * Q;
     taken = true;
@@ -357,8 +376,8 @@ __device__ __forceinline__ AdjCoef adj_local(double tv, double te, double tev, d
     p_te = 1.0 - r;
     p_tev = r;
     p_s = k.dx * sref / Q;
-  } else if ((tv - tev <= sref * k.dx * k.dx / k.hyp) && (tv - tev > 0.0)) {
-    const double p = (tv - tev) / k.dx;
+  } else if ((tv - tev <= cdiv(sref * k.dx * k.dx, k.hyp, k.r_hyp)) && (tv - tev > 0.0)) {
+    const double p = cdiv(tv - tev, k.dx, k.r_dx);
     const double P = sqrt(sref * sref - p * p);
     tbr = tv + k.dz * P;
     taken = true;
@@ -707,6 +726,10 @@ static ConstsD consts_d(double dx, double dz) {
   k.sum = k.dz2i + k.dx2i;
   k.prod = k.dz2i * k.dx2i;
   k.hyp = std::sqrt(dx * dx + dz * dz);
+  k.r_dx = 1.0 / k.dx;
+  k.r_dz = 1.0 / k.dz;
+  k.r_sum = 1.0 / k.sum;
+  k.r_hyp = 1.0 / k.hyp;
   return k;
 }
 
