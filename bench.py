@@ -400,6 +400,7 @@ def bench_migration(steps=3, cpu_traces=200):
     model = K.adjoint(d)
     torch.cuda.synchronize()
     adj_ms = events_ms(lambda: K.adjoint(d, model.zero_()))
+    fwd_ms = events_ms(lambda: K.forward(model, d.zero_()))
     dt32 = torch.tensor(np.float32(P["dt"]), device="cuda")
     Ts = torch.from_numpy(ts[:, 3:-3, 3:-3].copy()).cuda()
     Tr = torch.from_numpy(tr[:, 3:-3, 3:-3].copy()).cuda()
@@ -418,7 +419,7 @@ def bench_migration(steps=3, cpu_traces=200):
     achieved = valu / (adj_ms / 1e3) / 1e12 if valu else None
     fetch = prof.get("fetch_bytes_per_launch")
     out["image_deep_record"] = {
-        "nt": nt_deep, "traces": K.ntr, "adjoint_ms": adj_ms, "pairs_reaching": reach,
+        "nt": nt_deep, "traces": K.ntr, "adjoint_ms": adj_ms, "forward_ms": fwd_ms, "pairs_reaching": reach,
         "reaching_fraction": reach / pairs, "adjoint_pairs_per_s": pairs / (adj_ms / 1e3),
         "adjoint_reaching_pairs_per_s": reach / (adj_ms / 1e3),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TIPS, "unit": "T wave-instr/s",

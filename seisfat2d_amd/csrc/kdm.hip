@@ -340,7 +340,8 @@ __global__ __launch_bounds__(256) void k_kdm_forward(const float* __restrict__ q
       Q7 qs, qr;
       qs.t = q[(size_t)s * KQ * K.npts + p];
       qr.t = q[(size_t)r * KQ * K.npts + p];
-      const int it = t_index(qs.t, qr.t, K);
+      const float xs = qs.t + qr.t;
+      const int it = xs >= K.thr ? K.nt : t_index_below(xs, K);
       if ((unsigned)it >= (unsigned)K.nt) continue;
       q_rest(q, s, p, K.npts, qs);
       q_rest(q, r, p, K.npts, qr);

@@ -96,10 +96,11 @@ def test_image_adjoint_matches_oracle(oracle, setup, torch):
     K.close()
 
 
+@pytest.mark.parametrize("nt", [420, 5000, 9000])   # four, two and one trace(s) per forward workgroup
 @pytest.mark.parametrize("domain", [mig.IMAGE, mig.ANGLE])
-def test_forward_matches_oracle(oracle, setup, torch, domain):
+def test_forward_matches_oracle(oracle, setup, torch, domain, nt):
     st = setup
-    K = make(st["e"], st["m"], st["xs"], st["xr"], domain, tables=(st["ts"], st["tr"]))
+    K = make(st["e"], st["m"], st["xs"], st["xr"], domain, nt=nt, tables=(st["ts"], st["tr"]))
     S, op = orc_args(oracle, K, st)
     rng = np.random.default_rng(7 + domain)
     m = rng.standard_normal(K.model_shape).astype(np.float32)

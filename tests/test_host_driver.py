@@ -204,9 +204,11 @@ mig_residuo_folder = {tmp}/residuo/
 """
 
 
-def _mig_case(tmp, t, nt=300, drop=None):
-    p, s = write_case(tmp)
+def _mig_case(tmp, t, nt=300, drop=None, ani=False):
+    p, s = write_case(tmp, ani=ani)
     keys = MIG_KEYS.format(t=t, nt=nt, tmp=tmp)
+    if ani:
+        keys = keys.replace("anisotropy = false", "anisotropy = true")
     if drop:
         keys = "\n".join(l for l in keys.splitlines() if not l.startswith(drop))
     with open(p, "a") as f:
@@ -282,3 +284,44 @@ def test_migration_driver_matches_oracle(tmp_path, oracle, t):
         got_res = np.loadtxt(os.path.join(tmp, "residuo", f"{name}_convergence_2_iterations.txt"))
         assert np.allclose(got_res, res, rtol=1e-4, atol=0), (got_res, res)
         assert np.linalg.norm(got - want) <= 1e-3 * np.linalg.norm(want)
+
+
+@pytest.mark.gpu
+def test_migration_driver_anisotropic_tables(tmp_path, oracle):
+    """anisotropy = true: the tables are qP solves (Eikonal_ANI, as
+    src/migration/migration.cu:26-27 selects), each equal to the oracle's, and
+    the image the oracle's migration on those tables."""
+    from seisfat2d_amd import migration as mig
+    tmp = str(tmp_path)
+    nt = 300
+    p, s = _mig_case(tmp, 0, nt, ani=True)
+    r = subprocess.run([build.MIGRATION_EXE, p], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    nz, nx, dh = 41, 61, 10.0
+    S = oracle.expand_boundary(s)
+    Cq, mn, mx = [], [], []
+    for name in ("C11", "C13", "C15", "C33", "C35", "C55"):
+        c = np.fromfile(os.path.join(tmp, f"model_{name}.bin"), np.float32).reshape(nx, nz)
+        u, a, b = oracle.compress(oracle.expand_boundary(c))
+        Cq.append(u)
+        mn.append(a)
+        mx.append(b)
+    xs = np.array([100.0 + 150.0 * k for k in range(3)], np.float32)
+    xr = (np.arange(0, nx, 3) * dh).astype(np.float32)
+    ts = np.stack([oracle.solve_ani(S, dh, dh, float(x), 0.0, Cq, mn, mx)[0] for x in xs])
+    tr = np.stack([oracle.solve_ani(S, dh, dh, float(x), 0.0, Cq, mn, mx)[0] for x in xr])
+    for k in range(len(xs)):
+        got = np.fromfile(os.path.join(tmp, "tables", f"eikonal_src_{k + 1}.bin"), np.float32)
+        assert np.array_equal(got.reshape(nx + 6, nz + 6), ts[k])
+    nang, dcmp, min_cmp, ncmp = mig.gathers(xs, xr, 3.0, 60.0)
+    src, rec, cmp, cid = mig.trace_list(xs, xr, 300.0, min_cmp, dcmp)
+    seismic = np.stack([np.fromfile(os.path.join(tmp, "data", f"mig_input_shot_{k + 1}.bin"), np.float32)
+                        .reshape(len(xr), nt) for k in range(3)])
+    d = np.ascontiguousarray(oracle.trace_convolve(np.ascontiguousarray(seismic[src, rec]), oracle.ricker(2e-3, 30.0),
+                                                   True))
+    op = dict(dt=2e-3, dx=dh, dz=dh, aperture=30.0, da=3.0, nang=nang)
+    want = oracle.kdm(0, True, S, ts, tr, src, rec, cmp, None, d, np.zeros((nx, nz), np.float32), **op)
+    bound = oracle.kdm(0, True, S, ts, tr, src, rec, cmp, None, np.abs(d), np.zeros((nx, nz), np.float32), **op)
+    got = np.fromfile(os.path.join(tmp, "seismic", f"IDKDM_result_{nz}x{nx}.bin"), np.float32).reshape(nx, nz)
+    assert np.abs(want).max() > 0 and not (np.abs(got - want) > 1e-5 * bound + 1e-30).any()
