@@ -681,10 +681,11 @@ int seisfat_kdm_set_traces(seisfat_kdm_t k, int ntr, const int* src, const int* 
 }
 
 int seisfat_kdm_adjoint(seisfat_kdm_t k, const float* d_data, float* d_model, void* stream) {
-  if (!k || !d_data || !d_model) return kfail(SEISFAT_EINVAL, "null operator or buffer");
+  if (!k) return kfail(SEISFAT_EINVAL, "null operator");
   hipStream_t st = (hipStream_t)stream;
   if (const int rc = ready(k, st)) return rc;
-  if (k->ntr == 0) return SEISFAT_OK;
+  if (k->ntr == 0) return SEISFAT_OK;  // no traces: the model is unchanged
+  if (!d_data || !d_model) return kfail(SEISFAT_EINVAL, "null buffer");
   const int blocks = (k->K.npts + 255) / 256;
   if (k->p.domain == SEISFAT_KDM_IMAGE)
     hipLaunchKernelGGL((k_kdm_adjoint<false, KDM_SB>), dim3(blocks), dim3(256), 0, st, k->d_q, k->d_tmin,
@@ -699,10 +700,11 @@ int seisfat_kdm_adjoint(seisfat_kdm_t k, const float* d_data, float* d_model, vo
 }
 
 int seisfat_kdm_forward(seisfat_kdm_t k, const float* d_model, float* d_data, void* stream) {
-  if (!k || !d_data || !d_model) return kfail(SEISFAT_EINVAL, "null operator or buffer");
+  if (!k) return kfail(SEISFAT_EINVAL, "null operator");
   hipStream_t st = (hipStream_t)stream;
   if (const int rc = ready(k, st)) return rc;
-  if (k->ntr == 0) return SEISFAT_OK;
+  if (k->ntr == 0) return SEISFAT_OK;  // no traces to add into
+  if (!d_data || !d_model) return kfail(SEISFAT_EINVAL, "null buffer");
   const size_t lds = (size_t)k->K.nt * sizeof(float);
   if (k->p.domain == SEISFAT_KDM_IMAGE)
     hipLaunchKernelGGL(k_kdm_forward<false>, dim3(k->ntr), dim3(256), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
@@ -726,7 +728,7 @@ int seisfat_kdm_destroy(seisfat_kdm_t k) {
 
 int seisfat_trace_convolve(int adjoint, int ntr, int nt, const float* wavelet, int nw, const float* d_in,
                            float* d_out, void* stream) {
-  if (ntr < 0 || nt <= 0 || nw <= 0 || !wavelet || (ntr > 0 && (!d_in || !d_out)) || d_in == d_out)
+  if (ntr < 0 || nt <= 0 || nw <= 0 || !wavelet || (ntr > 0 && (!d_in || !d_out || d_in == d_out)))
     return kfail(SEISFAT_EINVAL, "bad convolution arguments (distinct device buffers, positive sizes)");
   if (ntr == 0) return SEISFAT_OK;
   // nfft = the power of two >= nt + nw - 1 (the reference's nextpow2)

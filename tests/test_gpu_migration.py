@@ -205,6 +205,8 @@ def test_least_squares_migration_matches_oracle(oracle, setup, torch, domain, sm
 
 
 def test_argument_errors(setup, torch):
+    import ctypes as C
+    from seisfat2d_amd import capi
     st = setup
     K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, tables=(st["ts"], st["tr"]))
     with pytest.raises(ValueError):
@@ -213,4 +215,38 @@ def test_argument_errors(setup, torch):
         K.forward(torch.zeros((3, 3), device="cuda"))
     with pytest.raises(ValueError):
         make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, tables=(st["ts"][:2], st["tr"]))
+    # trace indices out of range, a table index out of range, operators before any table
+    bad = np.array([0, 99], np.int32)
+    rc = capi.lib().seisfat_kdm_set_traces(K._k, 2, capi._ptr(bad), capi._ptr(np.zeros(2, np.int32)),
+                                           capi._ptr(np.zeros(2, np.float32)), None)
+    assert rc == capi.EINVAL and "out of range" in capi.lib().seisfat_last_error().decode()
+    assert capi.lib().seisfat_kdm_set_table(K._k, 1, K.nrec, C.c_void_p(1), None) == capi.EINVAL
+    K.close()
+    e = st["e"]
+    for params, msg in (((mig.IMAGE, 0, 1e-3, 30.0, 1.0, 1, 1), "nt"), ((7, 100, 1e-3, 30.0, 1.0, 1, 1), "domain"),
+                        ((mig.ANGLE, 100, 1e-3, 30.0, 1.0, 0, 4), "angle domain"),
+                        ((mig.IMAGE, 100, -1e-3, 30.0, 1.0, 1, 1), "dt")):
+        h = C.c_void_p()
+        p = capi.KdmParams(*params)
+        assert capi.lib().seisfat_kdm_create(e._h, C.byref(p), 2, 3, C.byref(h)) == capi.EINVAL
+        assert msg in capi.lib().seisfat_last_error().decode()
+    h = C.c_void_p()
+    p = capi.KdmParams(mig.IMAGE, 100, 1e-3, 30.0, 1.0, 1, 1)
+    capi.check(capi.lib().seisfat_kdm_create(e._h, C.byref(p), 2, 3, C.byref(h)))
+    d = torch.zeros((1, 100), device="cuda")
+    m = torch.zeros((e.nx, e.nz), device="cuda")
+    assert capi.lib().seisfat_kdm_adjoint(h, C.c_void_p(d.data_ptr()), C.c_void_p(m.data_ptr()), None) == capi.ESTATE
+    capi.lib().seisfat_kdm_destroy(h)
+
+
+def test_no_traces_within_the_offset(setup, torch):
+    """A maximum offset that keeps no pair: no traces, a zero image, a
+    zero-trace forward."""
+    st = setup
+    K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, max_offset=1.0, tables=(st["ts"], st["tr"]))
+    assert K.ntr == 0
+    seismic = np.ones((K.nsrc, K.nrec, K.nt), np.float32)
+    img = K.migrate(seismic)
+    assert img.shape == K.model_shape and not img.any()
+    assert tuple(K.forward(torch.ones(K.model_shape, device="cuda")).shape) == (0, K.nt)
     K.close()
