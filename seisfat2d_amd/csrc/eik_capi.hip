@@ -661,16 +661,35 @@ int batch_status(seisfat_batch_s* b) {
   if (b->d_ctl) HIPCHK(hipMemcpyAsync(&err[0], b->d_ctl + CTL_ERR, sizeof(int), hipMemcpyDeviceToHost, st));
   if (b->adj.d_walk_ctl)
     HIPCHK(hipMemcpyAsync(&err[1], b->adj.d_walk_ctl + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  // Fused pairs: every XCD drains its own tile queue, so each queue needs at
+  // least one of the launch's workgroups on its XCD (the round robin checked
+  // when the geometry was made; a kernel running at the same time on another
+  // stream could take an XCD's whole share).  A queue's ticket counter ends
+  // at its length plus the workgroups that drained it, or at 0 if none did.
+  int cnt[NUM_COUNTERS] = {};
+  const bool queues = b->pairs && b->last_run_plain && b->d_ctl && b->pair_qoff.size() == 9;
+  if (queues) HIPCHK(hipMemcpyAsync(cnt, b->d_ctl, sizeof(cnt), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (err[0]) return fail(SEISFAT_EHIP, "persistent sweep: a tile dependency did not arrive within the spin limit");
   if (err[1])
     return fail(SEISFAT_EHIP, "adjoint strip walk (fp64 trajectory or reverse sweep): a strip's upper "
                               "neighbour did not arrive within the spin limit");
+  if (queues) {
+    const int solves = b->type == SEISFAT_ANI ? 2 : 1;
+    for (int s = 0; s < solves; ++s)
+      for (int p = 0; p < 2; ++p)
+        for (int x = 0; x < 8; ++x)
+          if (cnt[PAIR_COUNTERS + 16 * s + 8 * p + x] < b->pair_qoff[x + 1] - b->pair_qoff[x])
+            return fail(SEISFAT_EHIP, "fused sweep pairs: XCD " + std::to_string(x) +
+                                          " received no workgroup, its tiles were not solved (another kernel "
+                                          "running at the same time?); use schedule option pairs = 0");
+  }
   return SEISFAT_OK;
 }
 
 int run_batch(seisfat_batch_s* b, hipStream_t st) {
   if (!b->h->has_s) return fail(SEISFAT_ESTATE, "slowness rejected or not set since the batch was created");
+  b->last_run_plain = true;
   if (b->gexec[0]) {
     HIPCHK(hipGraphLaunch(b->gexec[0], st));
     HIPCHK(hipEventRecord(b->ev_sweep[0], st));
@@ -694,6 +713,7 @@ int run_batch(seisfat_batch_s* b, hipStream_t st) {
 // skipped at once after convergence), and the picks; the host reads the skip
 // word after each chunk.
 int run_converged(seisfat_batch_s* b, int max_passes, int chunk, int* passes, hipStream_t st) {
+  b->last_run_plain = false;  // gated passes leave their queues untouched
   seisfat_eik_s* h = b->h;
   if (b->type != SEISFAT_ISO || b->schedule != SEISFAT_SCHED_TILED)
     return fail(SEISFAT_EINVAL, "converged mode covers isotropic batches of the tiled schedule");

@@ -123,6 +123,9 @@ struct seisfat_batch_s {
   int* d_qoff = nullptr;
   std::vector<int4> pair_tiles;  // host copies of the queues' contents
   std::vector<int> pair_qoff;
+  // the last run was a plain pass (run / recorded run), whose fused-pair
+  // queues batch_status checks were drained
+  bool last_run_plain = false;
   PersistRun pr{};
   int launches = 0, sweep_launches = 0;
   hipEvent_t ev_sweep[2] = {nullptr, nullptr};
