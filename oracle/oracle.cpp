@@ -844,12 +844,17 @@ int f2i_rd(float v) {
   return (int)std::floor(v);
 }
 
+// Image-domain adjoint columns may run on several threads (nthreads > 1):
+// each image point receives its own sum, so the result is the same; the angle
+// domain and the forward add into shared bins and stay serial.
 void kdm_trace(int domain, int adjoint, const float* S, const float* Ts, const float* Tr, float* data, float* model,
                float dt, int nt, float dx, float dz, int nxx, int nzz, float aperture, float cmp, int cmpId, float da,
-               int na) {
+               int na, int nthreads = 1) {
   const int nz = nzz - 2 * NB;
   const float tan_ap = std::tan((float)((double)aperture * M_PI / 180.0));
   const float scale = (float)(1.0 / (2.0 * M_PI));
+  const int nth = (domain == 0 && adjoint) ? std::max(nthreads, 1) : 1;
+#pragma omp parallel for schedule(static) num_threads(nth)
   for (int j = NB; j < nxx - NB; ++j)
     for (int i = NB; i < nzz - NB; ++i) {
       const size_t c = i + (size_t)j * nzz;
@@ -1060,10 +1065,10 @@ void orc_trace_convolve(int adjoint, int ntr, int nt, const float* w, int nw, co
 // ntr x nt, tables are consecutive padded grids.
 void orc_kdm(int domain, int adjoint, const float* S, const float* src_tables, const float* rec_tables, int ntr,
              const int* isrc, const int* irec, const float* cmp, const int* cmp_id, float* data, float* model,
-             int nt, float dt, float dx, float dz, int nzz, int nxx, float aperture, float da, int na) {
+             int nt, float dt, float dx, float dz, int nzz, int nxx, float aperture, float da, int na, int nthreads) {
   const size_t m = (size_t)nzz * nxx;
   for (int t = 0; t < ntr; ++t)
     kdm_trace(domain, adjoint, S, src_tables + isrc[t] * m, rec_tables + irec[t] * m, data + (size_t)t * nt, model,
-              dt, nt, dx, dz, nxx, nzz, aperture, cmp[t], cmp_id ? cmp_id[t] : 0, da, na);
+              dt, nt, dx, dz, nxx, nzz, aperture, cmp[t], cmp_id ? cmp_id[t] : 0, da, na, nthreads);
 }
 }  // extern "C"

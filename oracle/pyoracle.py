@@ -70,7 +70,7 @@ def lib():
             "orc_ricker": (i, [f, f, f32p, i]),
             "orc_trace_convolve": (None, [i, i, i, f32p, i, f32p, f32p]),
             "orc_kdm": (None, [i, i, f32p, f32p, f32p, i, i32p, i32p, f32p, C.c_void_p, f32p, f32p, i, f, f, f,
-                               i, i, f, f, i]),
+                               i, i, f, f, i, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -312,12 +312,13 @@ def trace_convolve(traces, wavelet, adjoint):
 
 
 def kdm(domain, adjoint, S, src_tables, rec_tables, isrc, irec, cmp, cmp_id, data, model, dt, dx, dz,
-        aperture, da=1.0, nang=1):
+        aperture, da=1.0, nang=1, nthreads=1):
     """The reference's per-trace Kirchhoff kernels (src/migration/migration.cu:398-756)
     for every trace t in order: adjoint adds into model, forward into data
     (both updated in place and returned).  S: padded slowness (nxx, nzz);
     tables (n, nxx, nzz); data (ntr, nt); model (nx, nz) for domain 0,
-    (ncmp, nang, nz) for domain 1."""
+    (ncmp, nang, nz) for domain 1.  nthreads: threads over image columns for
+    the image-domain adjoint (the same result)."""
     S = np.ascontiguousarray(S, np.float32)
     ts = np.ascontiguousarray(src_tables, np.float32)
     tr = np.ascontiguousarray(rec_tables, np.float32)
@@ -339,7 +340,7 @@ def kdm(domain, adjoint, S, src_tables, rec_tables, isrc, irec, cmp, cmp_id, dat
         assert model.shape == (nxx - 2 * NB, nzz - 2 * NB)
     lib().orc_kdm(domain, int(bool(adjoint)), S, ts, tr, ntr, isrc, irec, cmp,
                   None if cid is None else cid.ctypes.data_as(C.c_void_p), data, model, data.shape[1], dt, dx, dz,
-                  nzz, nxx, aperture, da, nang)
+                  nzz, nxx, aperture, da, nang, nthreads)
     return model if adjoint else data
 
 

@@ -322,3 +322,21 @@ def test_angle_smoothing_only_changes_the_update_along_angle(o):
     m1, r1 = run_ls(o, 1, P, 2, smooth=1e-2)
     assert r0[:2] == r1[:2]             # the first update sees a zero model: no smoothing term
     assert not np.array_equal(m0, m1)
+
+
+def test_threaded_image_adjoint_is_the_serial_one(o):
+    """Image columns on several threads: every image point still receives its
+    contributions in trace order, so the image is bit for bit the serial one."""
+    nz, nx, dz, dx = 30, 44, 10.0, 10.0
+    rng = np.random.default_rng(8)
+    ts = homogeneous_tables(nz, nx, dz, dx, 2100.0, [(50.0, 0.0), (250.0, 0.0)])
+    tr = homogeneous_tables(nz, nx, dz, dx, 2100.0, [(10.0, 0.0), (180.0, 0.0), (400.0, 0.0)])
+    S = rng.uniform(4e-4, 5e-4, ts.shape[1:]).astype(np.float32)
+    src = np.array([0, 0, 1, 1, 1], np.int32)
+    rec = np.array([0, 1, 0, 1, 2], np.int32)
+    cmp = rng.uniform(0, 400, 5).astype(np.float32)
+    d = rng.standard_normal((5, 400)).astype(np.float32)
+    args = (S, ts, tr, src, rec, cmp, None)
+    m1 = o.kdm(0, True, *args, d, np.zeros((nx, nz), np.float32), 1e-3, dx, dz, 30.0)
+    m8 = o.kdm(0, True, *args, d, np.zeros((nx, nz), np.float32), 1e-3, dx, dz, 30.0, nthreads=8)
+    assert np.abs(m1).max() > 0 and np.array_equal(m1, m8)
