@@ -393,6 +393,40 @@ __global__ __launch_bounds__(256) void k_trace_conv(const float* __restrict__ in
   out[g] = (float)(acc * nfft);
 }
 
+// The same sums with one workgroup per trace, the trace and the wavelet staged
+// in LDS (nt + nw <= 16384 floats): every tap is read from LDS instead of
+// global memory.  Same operands, same order: the same bits.
+template <bool ADJ>
+__global__ __launch_bounds__(256) void k_trace_conv_lds(const float* __restrict__ in, float* __restrict__ out,
+                                                        int nt, const float* __restrict__ w, int nw, int shift,
+                                                        double nfft) {
+  extern __shared__ float sm[];
+  float* tr = sm;
+  float* wl = sm + nt;
+  const size_t t = blockIdx.x;
+  const float* __restrict__ d = in + t * nt;
+  for (int k = threadIdx.x; k < nt; k += blockDim.x) tr[k] = d[k];
+  for (int k = threadIdx.x; k < nw; k += blockDim.x) wl[k] = w[k];
+  __syncthreads();
+  float* __restrict__ o = out + t * nt;
+  for (int k = threadIdx.x; k < nt; k += blockDim.x) {
+    double acc = 0.0;
+    if (ADJ) {
+      if (k < shift) {
+        o[k] = 0.0f;
+        continue;
+      }
+      const int k0 = k - shift;
+      const int mmax = min(nw - 1, nt - 1 - k0);
+      for (int m = 0; m <= mmax; ++m) acc += (double)wl[m] * (double)tr[k0 + m];
+    } else {
+      const int kk = k + shift;
+      for (int m = max(0, kk - nw + 1); m <= min(kk, nt - 1); ++m) acc += (double)tr[m] * (double)wl[kk - m];
+    }
+    o[k] = (float)(acc * nfft);
+  }
+}
+
 }  // namespace seisfat
 
 using namespace seisfat;
@@ -740,13 +774,23 @@ int seisfat_trace_convolve(int adjoint, int ntr, int nt, const float* wavelet, i
   KHIP(hipMallocAsync((void**)&d_w, nw * sizeof(float), st));
   KHIP(hipMemcpyAsync(d_w, wavelet, nw * sizeof(float), hipMemcpyHostToDevice, st));
   const size_t n = (size_t)ntr * nt;
-  const int blocks = (int)((n + 255) / 256);
-  if (adjoint)
-    hipLaunchKernelGGL(k_trace_conv<true>, dim3(blocks), dim3(256), 0, st, d_in, d_out, ntr, nt, d_w, nw, shift,
-                       (double)nfft);
-  else
-    hipLaunchKernelGGL(k_trace_conv<false>, dim3(blocks), dim3(256), 0, st, d_in, d_out, ntr, nt, d_w, nw, shift,
-                       (double)nfft);
+  if ((long long)nt + nw <= 16384) {
+    const size_t lds = (size_t)(nt + nw) * sizeof(float);
+    if (adjoint)
+      hipLaunchKernelGGL(k_trace_conv_lds<true>, dim3(ntr), dim3(256), lds, st, d_in, d_out, nt, d_w, nw, shift,
+                         (double)nfft);
+    else
+      hipLaunchKernelGGL(k_trace_conv_lds<false>, dim3(ntr), dim3(256), lds, st, d_in, d_out, nt, d_w, nw, shift,
+                         (double)nfft);
+  } else {
+    const int blocks = (int)((n + 255) / 256);
+    if (adjoint)
+      hipLaunchKernelGGL(k_trace_conv<true>, dim3(blocks), dim3(256), 0, st, d_in, d_out, ntr, nt, d_w, nw, shift,
+                         (double)nfft);
+    else
+      hipLaunchKernelGGL(k_trace_conv<false>, dim3(blocks), dim3(256), 0, st, d_in, d_out, ntr, nt, d_w, nw, shift,
+                         (double)nfft);
+  }
   KHIP(hipGetLastError());
   KHIP(hipFreeAsync(d_w, st));
   KHIP(hipStreamSynchronize(st));  // the host wavelet is the caller's

@@ -81,6 +81,24 @@ def test_convolution_bit_exact(oracle, setup, torch, adjoint, nt, dt):
     K.close()
 
 
+@pytest.mark.parametrize("adjoint", [False, True])
+@pytest.mark.parametrize("nt,dt,fmax", [(1001, 1e-3, 2.0),       # wavelet (3542) longer than the trace, in LDS
+                                        (16000, 1e-3, 30.0),     # trace and wavelet just fit LDS
+                                        (17000, 1e-3, 30.0)])    # neither: the global-memory kernel
+def test_convolution_bit_exact_every_kernel(oracle, torch, adjoint, nt, dt, fmax):
+    import ctypes as C
+    from seisfat2d_amd import capi
+    w = mig.ricker(dt, fmax)
+    rng = np.random.default_rng(nt)
+    x = rng.standard_normal((3, nt)).astype(np.float32)
+    xi = torch.from_numpy(x).cuda()
+    out = torch.empty_like(xi)
+    capi.check(capi.lib().seisfat_trace_convolve(int(adjoint), 3, nt, capi._ptr(w), len(w), C.c_void_p(xi.data_ptr()),
+                                                 C.c_void_p(out.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), oracle.trace_convolve(x, w, adjoint))
+
+
 def test_image_adjoint_matches_oracle(oracle, setup, torch):
     st = setup
     K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, tables=(st["ts"], st["tr"]))
