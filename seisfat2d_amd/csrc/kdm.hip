@@ -255,20 +255,41 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
     }
     const int e1 = bent[b + 1];
     for (int e0 = bent[b]; e0 < e1; e0 += 64) {
-      // lane l tests entry e0 + l: does any of its pairs pass the chunk test?
+      // lane l holds entry e0 + l (receiver, traces, receiver minimum) and
+      // tests it: does any of its pairs pass the chunk test?  The entries that
+      // pass are then taken one at a time, their words read from that lane.
       bool need = false;
+      int el_r = 0, el_t[SB];
+      float mr_l = 0.0f;
+#pragma unroll
+      for (int k = 0; k < SB; ++k) el_t[k] = -1;
       if (e0 + lane < e1) {
         const int* __restrict__ El = ent + (size_t)(e0 + lane) * (1 + SB);
-        const float mr_l = tmin[(size_t)(nsrc + El[0]) * K.nchunk + c];
+        el_r = nsrc + El[0];
 #pragma unroll
-        for (int k = 0; k < SB; ++k) need |= El[1 + k] >= 0 && use[k] && !beyond(ms[k], mr_l, K);
+        for (int k = 0; k < SB; ++k) el_t[k] = El[1 + k];
+        mr_l = tmin[(size_t)el_r * K.nchunk + c];
+#pragma unroll
+        for (int k = 0; k < SB; ++k) need |= el_t[k] >= 0 && use[k] && !beyond(ms[k], mr_l, K);
       }
-      for (uint64_t m = __ballot(need); m; m &= m - 1) {
-        const int e = e0 + __builtin_ctzll(m);
-        const int* __restrict__ E = ent + (size_t)e * (1 + SB);
-        const int r = nsrc + E[0];
-        const float mr = tmin[(size_t)r * K.nchunk + c];
-        const float rt = q[(size_t)r * KQ * K.npts + p];
+      uint64_t m = __ballot(need);
+      // the next entry's receiver traveltime is loaded while this one computes
+      int r_next = 0;
+      float rt_next = 0.0f;
+      if (m) {
+        r_next = __builtin_amdgcn_readlane(el_r, (int)__builtin_ctzll(m));
+        rt_next = q[(size_t)r_next * KQ * K.npts + p];
+      }
+      while (m) {
+        const int li = (int)__builtin_ctzll(m);
+        m &= m - 1;
+        const int r = r_next;
+        const float rt = rt_next;
+        const float mr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mr_l), li));
+        if (m) {
+          r_next = __builtin_amdgcn_readlane(el_r, (int)__builtin_ctzll(m));
+          rt_next = q[(size_t)r_next * KQ * K.npts + p];
+        }
         // first the time indices and the data samples of every pair, then the
         // weights: the gathers of one entry are in flight together
         int tk[SB];
@@ -277,7 +298,7 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
         bool anyok = false;
 #pragma unroll
         for (int k = 0; k < SB; ++k) {
-          tk[k] = E[1 + k];
+          tk[k] = __builtin_amdgcn_readlane(el_t[k], li);
           ok[k] = false;
           dv[k] = 0.0f;
           if (tk[k] < 0 || !use[k] || beyond(ms[k], mr, K)) continue;
@@ -287,7 +308,7 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
           if (ok[k]) dv[k] = data[(size_t)tk[k] * K.nt + it];
           anyok |= ok[k];
         }
-        if (!__any(anyok)) continue;
+        if (!__any(anyok)) continue;  // (the while condition reads the updated m)
         Q7 qr;
         qr.t = rt;
         q_rest(q, r, p, K.npts, qr);
