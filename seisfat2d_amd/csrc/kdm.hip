@@ -16,16 +16,17 @@
 //     node that reaches the trace and passes over it with two scalar loads
 //     (with the reference's recording lengths most node-trace pairs end
 //     there); the other quantities are loaded only for nodes that reach it;
-//   - image-domain adjoint: one thread per image point loops over the traces
-//     in the migration's order and keeps its sum in a register, so the image
-//     receives the same FP32 sequence of additions as under the reference's
-//     launch order, without atomics;
-//   - angle-domain adjoint: the same loop, adding into the (depth, angle,
-//     CMP) bins with atomics, as the reference (their order varies);
+//   - adjoint: one thread per image point walks the traces grouped in source
+//     blocks (four sources' quantities in registers, each receiver's loaded
+//     once for up to four traces); image domain: the point's sum stays in a
+//     register, deterministic, no atomics (contributions block by block, by
+//     receiver, by source); angle domain: adds into the (depth, angle, CMP)
+//     bins with atomics, as the reference (their order varies);
 //   - forward: one workgroup per trace accumulates the trace in an LDS
 //     histogram of its nt samples (LDS atomics) and adds it to the trace once;
 //   - convolution: the FFT product of the reference is a linear
-//     (cross-)correlation with the wavelet, summed directly in double.
+//     (cross-)correlation with the wavelet, summed directly in double, the
+//     trace and the wavelet staged in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -141,10 +142,7 @@ __device__ __forceinline__ void q_rest(const float* __restrict__ q, int tab, int
   o.cosv = b[(size_t)6 * npts];
 }
 
-// Time index of a node for a trace.
-__device__ __forceinline__ int t_index(float ts, float tr, const KdmK& K) { return f2i_rd((ts + tr) / K.dt); }
-
-// The same for a traveltime sum xs < thr: RN(xs / dt) by the two-instruction
+// Time index f2i_rd(RN(xs / dt)) of a node for a trace, for a traveltime sum xs < thr: RN(xs / dt) by the two-instruction
 // known-divisor division where the check proved it exact, IEEE division
 // otherwise (xs below 2^-100, negative or NaN).
 __device__ __forceinline__ int t_index_below(float xs, const KdmK& K) {
