@@ -142,9 +142,10 @@ __device__ __forceinline__ void q_rest(const float* __restrict__ q, int tab, int
   o.cosv = b[(size_t)6 * npts];
 }
 
-// Time index f2i_rd(RN(xs / dt)) of a node for a trace, for a traveltime sum xs < thr: RN(xs / dt) by the two-instruction
-// known-divisor division where the check proved it exact, IEEE division
-// otherwise (xs below 2^-100, negative or NaN).
+// Time index f2i_rd(RN(xs / dt)) of a node for a trace whose traveltime sum
+// is xs < thr: RN(xs / dt) by the two-instruction known-divisor division where
+// the check proved it exact, IEEE division otherwise (xs below 2^-100,
+// negative or NaN).
 __device__ __forceinline__ int t_index_below(float xs, const KdmK& K) {
   return f2i_rd((K.fastdiv_dt && xs >= 0x1p-100f) ? fdiv(xs, K.zh_dt, K.zl_dt) : xs / K.dt);
 }
