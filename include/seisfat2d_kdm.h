@@ -47,8 +47,10 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
 /* The traveltime tables: d_src (nsrc) and d_rec (nrec) consecutive padded
  * (nz+6) x (nx+6) device grids, as set_src_travel_times / set_rec_travel_times
  * write them (src/migration/migration.cu:116-171).  Each table's derivative
- * quantities are evaluated here once (the reference re-evaluates them for
- * every trace, with the same operations). */
+ * quantities are evaluated here once, with the handle's current slowness (the
+ * reference re-evaluates them for every trace, with the same operations); a
+ * later slowness upload to the handle makes the tables stale: the operators
+ * then fail with SEISFAT_ESTATE until every table is set again. */
 int seisfat_kdm_set_tables(seisfat_kdm_t k, const float* d_src, const float* d_rec, void* stream);
 /* One table at a time: table `index` of the sources (which = 0) or receivers
  * (which = 1) from a padded grid in device memory, e.g. seisfat_batch_device_T

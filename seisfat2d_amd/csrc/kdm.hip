@@ -6,10 +6,16 @@
 // thread per padded node, each evaluating the node's weight from 9-point
 // finite differences of both traveltime tables and adding into the image
 // (adjoint) or the trace (forward) with atomicAdd.  Here:
-//   - the per-table part of the weight (first and second derivatives, the
-//     Hessian determinant, the gradient norm, the incidence angle and its
-//     cosine) is evaluated once per table (k_kdm_prep), with the reference's
-//     operations, and every trace reads it (7 words per node and table);
+//   - the per-table part of the weight is evaluated once per table
+//     (k_kdm_prep): the reference's first and second derivatives, Hessian
+//     determinant, gradient norm, incidence angle and cosine, folded into
+//     one amplitude per node (the Hessian, obliquity and spreading factors of
+//     the weight are products of a source and a receiver factor) and the
+//     cosine and sine of the half angle (the reflection-angle factor's
+//     cosine of the mean angle by the addition formula); a trace's weight is
+//     then four products, the angle factor and the aperture taper (7 words
+//     per node and table: T, amplitude, half-angle cosine and sine, and the
+//     gradient for the angle domain's bin);
 //   - each table also keeps the minimum traveltime of every 64 consecutive
 //     image points (k_kdm_tmin); rounded addition and division being
 //     monotone, a wave whose chunk minima give a time index >= nt holds no
@@ -43,7 +49,7 @@
 namespace seisfat {
 
 constexpr float KDM_EPS = 1e-6f;  // EPS of src/migration/migration.cuh:7
-constexpr int KQ = 7;             // per-table quantities: T, dT/dx, dT/dz, |det H| + EPS, |grad T| + EPS, angle, |cos|
+constexpr int KQ = 7;             // per-table quantities: T, amplitude, cos and sin of half the angle, dT/dx, dT/dz, |grad T| + EPS
 
 struct KdmK {
   int nz, nx, nzz, npts, nt, nang;
@@ -51,6 +57,7 @@ struct KdmK {
   float thr;      // least x with RN(x / dt) >= nt: a traveltime sum x >= thr has time index >= nt
   float zh_dt, zl_dt;  // two-word reciprocal of dt (fdiv)
   int fastdiv_dt;      // fdiv(x, zh_dt, zl_dt) == RN(x / dt) for every 2^-100 <= x < 2^100 (checked exhaustively)
+  int taper_rdiv;      // (x - cmp) / sig_e needs no operand rescaling at any node and trace (kdm_weight)
   int rec_nonneg; // every receiver-table value at an image point is >= 0 (no NaN)
   float dx, dz, dt, da;
   float tan_ap;  // tanf(aperture in radians), the reference's sigma / z
@@ -67,9 +74,20 @@ __device__ __forceinline__ int f2i_rd(float v) {
 }
 
 // The per-table quantities of padded table T at interior node (i, j) (padded
-// coordinates), each the expression the reference kernels evaluate.
-__global__ __launch_bounds__(256) void k_kdm_prep(const float* __restrict__ tables, int ntab, KdmK K,
-                                                  float* __restrict__ q) {
+// coordinates).  The reference's per-table expressions (float, its order):
+// the derivatives, norm = |grad T| + EPS, det = |det H| + EPS, the angle
+// theta = acos(clamp(-uz)) and cos = |-uz|, rs = max(T / S, EPS).  Its
+// weight's Hessian, obliquity and spreading factors,
+//   sqrt(det_s det_r) S / (norm_s norm_r) * sqrt(max(0, cos_s) max(0, cos_r))
+//   / sqrt(rs_s rs_r),
+// are S times the product of one amplitude per table,
+//   amp = sqrt(det max(0, cos)) / (norm sqrt(rs))   (in double, rounded once),
+// and its angle factor's cos((theta_s + theta_r) / 2) is
+// ch_s ch_r - sh_s sh_r with ch, sh = cos, sin(theta / 2) (double, rounded
+// once).  The weight a trace forms from them differs from the reference's
+// order of roundings by a few ulp.
+__global__ __launch_bounds__(256) void k_kdm_prep(const float* __restrict__ tables, int ntab,
+                                                  const float* __restrict__ S, KdmK K, float* __restrict__ q) {
   const size_t n = (size_t)ntab * K.npts;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t tsize = (size_t)K.nzz * (K.nx + 6);
@@ -93,14 +111,17 @@ __global__ __launch_bounds__(256) void k_kdm_prep(const float* __restrict__ tabl
     const float det = fabsf(d2x * d2z - dxz * dxz) + KDM_EPS;
     const float ux = d_x / norm, uz = d_z / norm;
     const float dot = ux * 0.0f + uz * -1.0f;  // unit gradient . (0, -1), the surface normal
+    const float theta = acosf(fminf(1.0f, fmaxf(-1.0f, dot)));
+    const float rs = fmaxf(c / S[i + j * nzz], KDM_EPS);
+    const double amp = sqrt((double)det * (double)fmaxf(0.0f, fabsf(dot))) / ((double)norm * sqrt((double)rs));
     float* __restrict__ o = q + (size_t)tab * KQ * K.npts + p;
     o[0] = c;
-    o[(size_t)1 * K.npts] = d_x;
-    o[(size_t)2 * K.npts] = d_z;
-    o[(size_t)3 * K.npts] = det;
-    o[(size_t)4 * K.npts] = norm;
-    o[(size_t)5 * K.npts] = acosf(fminf(1.0f, fmaxf(-1.0f, dot)));
-    o[(size_t)6 * K.npts] = fabsf(dot);
+    o[(size_t)1 * K.npts] = (float)amp;
+    o[(size_t)2 * K.npts] = (float)cos(0.5 * (double)theta);
+    o[(size_t)3 * K.npts] = (float)sin(0.5 * (double)theta);
+    o[(size_t)4 * K.npts] = d_x;
+    o[(size_t)5 * K.npts] = d_z;
+    o[(size_t)6 * K.npts] = norm;
   }
 }
 
@@ -129,25 +150,39 @@ __global__ __launch_bounds__(256) void k_kdm_tmin(const float* __restrict__ q, i
 }
 
 struct Q7 {
-  float t, dx, dz, det, norm, theta, cosv;
+  float t, amp, ch, sh, dx, dz, norm;
 };
-// The quantities after T (loaded once the node is known to reach the trace).
-__device__ __forceinline__ void q_rest(const float* __restrict__ q, int tab, int p, int npts, Q7& o) {
+// The weight's quantities after T (loaded once the node is known to reach the
+// trace).
+__device__ __forceinline__ void q_amp(const float* __restrict__ q, int tab, int p, int npts, Q7& o) {
   const float* __restrict__ b = q + (size_t)tab * KQ * npts + p;
-  o.dx = b[(size_t)npts];
-  o.dz = b[(size_t)2 * npts];
-  o.det = b[(size_t)3 * npts];
-  o.norm = b[(size_t)4 * npts];
-  o.theta = b[(size_t)5 * npts];
-  o.cosv = b[(size_t)6 * npts];
+  o.amp = b[(size_t)npts];
+  o.ch = b[(size_t)2 * npts];
+  o.sh = b[(size_t)3 * npts];
+}
+// The gradient, for the angle domain's bin.
+__device__ __forceinline__ void q_dir(const float* __restrict__ q, int tab, int p, int npts, Q7& o) {
+  const float* __restrict__ b = q + (size_t)tab * KQ * npts + p;
+  o.dx = b[(size_t)4 * npts];
+  o.dz = b[(size_t)5 * npts];
+  o.norm = b[(size_t)6 * npts];
 }
 
-// Time index f2i_rd(RN(xs / dt)) of a node for a trace whose traveltime sum
-// is xs < thr: RN(xs / dt) by the two-instruction known-divisor division where
-// the check proved it exact, IEEE division otherwise (xs below 2^-100,
-// negative or NaN).
-__device__ __forceinline__ int t_index_below(float xs, const KdmK& K) {
-  return f2i_rd((K.fastdiv_dt && xs >= 0x1p-100f) ? fdiv(xs, K.zh_dt, K.zl_dt) : xs / K.dt);
+// Time index of a node for a trace with traveltime sum xs, as far as the
+// kernels use it: the reference's tId = f2i_rd(RN(xs / dt)) where that lies in
+// [0, nt) or xs is NaN (0), a value >= nt or negative otherwise.  When every
+// lane's sum lies in [2^-100, 2^100) (traveltimes are positive), RN(xs / dt)
+// comes from the two-instruction known-divisor division the exhaustive check
+// proved exact there and is floored after a clamp to nt: the index is then
+// >= nt exactly when xs >= thr.  Otherwise, for the whole wave, IEEE division
+// where that division is not proven, and thr, NaN and the sign decided
+// explicitly.
+__device__ __forceinline__ int t_index(float xs, const KdmK& K) {
+  const bool mid = __float_as_uint(xs) - 0x0d800000u < 0x64000000u;  // 2^-100 <= xs < 2^100
+  if (K.fastdiv_dt && __all(mid)) return (int)floorf(fminf(fdiv(xs, K.zh_dt, K.zl_dt), (float)K.nt));
+  const float v = (K.fastdiv_dt && mid) ? fdiv(xs, K.zh_dt, K.zl_dt) : xs / K.dt;
+  const int it = v != v ? 0 : (int)fminf(fmaxf(floorf(v), -1.0f), (float)K.nt);
+  return xs >= K.thr ? K.nt : it;
 }
 
 // Refined reciprocal of b (the r of rdiv).
@@ -162,26 +197,22 @@ __device__ __forceinline__ float recip_refined(float b) {
 __device__ __forceinline__ bool beyond(float a, float b, const KdmK& K) { return a + b >= K.thr; }
 
 // Kirchhoff weight of one node for one trace (the product the reference
-// kernels form): Hessian amplitude, obliquity, spreading, reflection-angle
-// factor and the aperture taper about the trace's CMP.
-// rs = max(Ts / S, EPS), rr = max(Tr / S, EPS) and sig_e = sigma + EPS (with
-// its refined reciprocal r_sig) come in evaluated: they depend on the node and
-// one table only (the same roundings as per trace).  Every operation is the
-// reference's IEEE one; where an operand range allows, in fewer instructions:
-//   - sqrt of det_s det_r and of rs rr (both >= EPS^2, normal): the hardware
-//     estimate and its two one-ulp corrections (rsqrt_rn);
-//   - (x - cmp) / sig_e (divisor in [EPS, 2^40), dividend 0 or of magnitude
-//     >= 2^-60: no operand rescaling): rdiv with the refined reciprocal.
-__device__ __forceinline__ float kdm_weight(const Q7& s, const Q7& r, float S, float rs, float rr, float x,
-                                            float sig_e, float r_sig, float cmp, const KdmK& K) {
-  const float jterm = rsqrt_rn(s.det * r.det) * S / (s.norm * r.norm);
-  const float obliq = sqrtf(fmaxf(0.0f, s.cosv) * fmaxf(0.0f, r.cosv));
-  const float g = 1.0f / rsqrt_rn(rs * rr);
-  const float rf = 1.0f + 0.2f * cosf(0.5f * (s.theta + r.theta));
+// kernels form): Hessian amplitude, obliquity and spreading (S times the two
+// tables' amplitudes), the reflection-angle factor 1 + 0.2 cos of the mean
+// angle (by the half angles' cosines and sines), and the aperture taper about
+// the trace's CMP.  sS = S / (2 pi) and sig_e = sigma + EPS (with its refined
+// reciprocal r_sig) come in evaluated per node.  The taper is the reference's
+// expression and roundings: (x - cmp) / sig_e, by rdiv with the refined
+// reciprocal when the host found that no operand needs rescaling (rdiv_ok:
+// every divisor in [EPS, 2^40), every dividend 0 or of magnitude in
+// [2^-59, 2^41)), then expf(-0.5 a a).
+__device__ __forceinline__ float kdm_weight(const Q7& s, const Q7& r, float sS, float x, float sig_e, float r_sig,
+                                            float cmp, bool rdiv_ok) {
+  const float rf = __builtin_fmaf(0.2f, __builtin_fmaf(s.ch, r.ch, -(s.sh * r.sh)), 1.0f);
   const float n = x - cmp;
-  const float a = (n == 0.0f || fabsf(n) >= 0x1p-60f) ? rdiv(n, sig_e, r_sig) : n / sig_e;
+  const float a = rdiv_ok ? rdiv(n, sig_e, r_sig) : n / sig_e;
   const float taper = expf(-0.5f * a * a);
-  return K.scale * jterm * obliq * g * rf * taper;
+  return sS * (s.amp * r.amp) * rf * taper;
 }
 
 // Reflection-angle bin of the node for the trace (the reference's aId).
@@ -225,31 +256,30 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
   const int lane = threadIdx.x & 63;
   const int i = p % K.nz, j = p / K.nz;
   const float z = (float)i * K.dz, x = (float)j * K.dx;
-  const float s_node = S[(i + 3) + (size_t)(j + 3) * K.nzz];
+  const float sS = K.scale * S[(i + 3) + (size_t)(j + 3) * K.nzz];
   const float sig_e = K.tan_ap * z + KDM_EPS;
   const float r_sig = recip_refined(sig_e);
   float acc = (!ANGLE && live) ? model[p] : 0.0f;
   for (int b = 0; b < nb; ++b) {
+    // slot k in use: bit k (one scalar register for the block)
     float ms[SB];
-    bool use[SB];
-    bool any = false;
+    unsigned use = 0;
 #pragma unroll
     for (int k = 0; k < SB; ++k) {
       const int sk = bsrc[b * SB + k];
       ms[k] = sk >= 0 ? tmin[(size_t)sk * K.nchunk + c] : 0.0f;
-      use[k] = sk >= 0 && !(K.rec_nonneg && beyond(ms[k], 0.0f, K));
-      any |= use[k];
+      const bool u = sk >= 0 && !(K.rec_nonneg && beyond(ms[k], 0.0f, K));
+      use |= (unsigned)__builtin_amdgcn_readfirstlane((int)u) << k;
     }
-    if (!any) continue;
+    if (!use) continue;
     Q7 qs[SB];
-    float rs[SB];
 #pragma unroll
     for (int k = 0; k < SB; ++k) {
-      if (use[k]) {
+      if (use >> k & 1) {
         const int sk = bsrc[b * SB + k];
         qs[k].t = q[(size_t)sk * KQ * K.npts + p];
-        q_rest(q, sk, p, K.npts, qs[k]);
-        rs[k] = fmaxf(qs[k].t / s_node, KDM_EPS);
+        q_amp(q, sk, p, K.npts, qs[k]);
+        if (ANGLE) q_dir(q, sk, p, K.npts, qs[k]);
       }
     }
     const int e1 = bent[b + 1];
@@ -258,10 +288,14 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
       // tests it: does any of its pairs pass the chunk test?  The entries that
       // pass are then taken one at a time, their words read from that lane.
       bool need = false;
-      int el_r = 0, el_t[SB];
-      float mr_l = 0.0f;
+      int el_r = 0, el_t[SB], el_a[SB];
+      float mr_l = 0.0f, el_c[SB];
 #pragma unroll
-      for (int k = 0; k < SB; ++k) el_t[k] = -1;
+      for (int k = 0; k < SB; ++k) {
+        el_t[k] = -1;
+        el_a[k] = 0;
+        el_c[k] = 0.0f;
+      }
       if (e0 + lane < e1) {
         const int* __restrict__ El = ent + (size_t)(e0 + lane) * (1 + SB);
         el_r = nsrc + El[0];
@@ -269,7 +303,17 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
         for (int k = 0; k < SB; ++k) el_t[k] = El[1 + k];
         mr_l = tmin[(size_t)el_r * K.nchunk + c];
 #pragma unroll
-        for (int k = 0; k < SB; ++k) need |= el_t[k] >= 0 && use[k] && !beyond(ms[k], mr_l, K);
+        for (int k = 0; k < SB; ++k) need |= el_t[k] >= 0 && (use >> k & 1) && !beyond(ms[k], mr_l, K);
+        // the traces' CMP (and bin), read from this lane when the entry is taken
+        if (need) {
+#pragma unroll
+          for (int k = 0; k < SB; ++k) {
+            if (el_t[k] >= 0) {
+              el_c[k] = tcmp[el_t[k]];
+              if (ANGLE) el_a[k] = tcid[el_t[k]];
+            }
+          }
+        }
       }
       uint64_t m = __ballot(need);
       // the next entry's receiver traveltime is loaded while this one computes
@@ -300,9 +344,9 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
           tk[k] = __builtin_amdgcn_readlane(el_t[k], li);
           ok[k] = false;
           dv[k] = 0.0f;
-          if (tk[k] < 0 || !use[k] || beyond(ms[k], mr, K)) continue;
+          if (tk[k] < 0 || !(use >> k & 1) || beyond(ms[k], mr, K)) continue;
           const float xs = qs[k].t + rt;
-          const int it = xs >= K.thr ? K.nt : t_index_below(xs, K);
+          const int it = t_index(xs, K);
           ok[k] = live && (unsigned)it < (unsigned)K.nt;
           if (ok[k]) dv[k] = data[(size_t)tk[k] * K.nt + it];
           anyok |= ok[k];
@@ -310,17 +354,18 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
         if (!__any(anyok)) continue;  // (the while condition reads the updated m)
         Q7 qr;
         qr.t = rt;
-        q_rest(q, r, p, K.npts, qr);
-        const float rr = fmaxf(rt / s_node, KDM_EPS);
+        q_amp(q, r, p, K.npts, qr);
+        if (ANGLE) q_dir(q, r, p, K.npts, qr);
 #pragma unroll
         for (int k = 0; k < SB; ++k) {
           if (!ok[k]) continue;
-          const float w = kdm_weight(qs[k], qr, s_node, rs[k], rr, x, sig_e, r_sig, tcmp[tk[k]], K);
+          const float cmp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(el_c[k]), li));
+          const float w = kdm_weight(qs[k], qr, sS, x, sig_e, r_sig, cmp, K.taper_rdiv);
           const float v = w * dv[k];
           if (ANGLE) {
             const int a = kdm_angle_bin(qs[k], qr, K);
-            if (a >= 0 && a < K.nang)
-              atomicAdd(model + i + (size_t)a * K.nz + (size_t)tcid[tk[k]] * K.nang * K.nz, v);
+            const int cb = __builtin_amdgcn_readlane(el_a[k], li);
+            if (a >= 0 && a < K.nang) atomicAdd(model + i + (size_t)a * K.nz + (size_t)cb * K.nang * K.nz, v);
           } else {
             acc += v;
           }
@@ -361,17 +406,19 @@ __global__ __launch_bounds__(256) void k_kdm_forward(const float* __restrict__ q
       qs.t = q[(size_t)s * KQ * K.npts + p];
       qr.t = q[(size_t)r * KQ * K.npts + p];
       const float xs = qs.t + qr.t;
-      const int it = xs >= K.thr ? K.nt : t_index_below(xs, K);
+      const int it = t_index(xs, K);
       if ((unsigned)it >= (unsigned)K.nt) continue;
-      q_rest(q, s, p, K.npts, qs);
-      q_rest(q, r, p, K.npts, qr);
+      q_amp(q, s, p, K.npts, qs);
+      q_amp(q, r, p, K.npts, qr);
+      if (ANGLE) {
+        q_dir(q, s, p, K.npts, qs);
+        q_dir(q, r, p, K.npts, qr);
+      }
       const int i = p % K.nz, j = p / K.nz;
       const float z = (float)i * K.dz, x = (float)j * K.dx;
-      const float s_node = S[(i + 3) + (size_t)(j + 3) * K.nzz];
-        const float rs = fmaxf(qs.t / s_node, KDM_EPS);
-      const float rr = fmaxf(qr.t / s_node, KDM_EPS);
+      const float sS = K.scale * S[(i + 3) + (size_t)(j + 3) * K.nzz];
       const float sig_e = K.tan_ap * z + KDM_EPS;
-      const float w = kdm_weight(qs, qr, s_node, rs, rr, x, sig_e, recip_refined(sig_e), cmp, K);
+      const float w = kdm_weight(qs, qr, sS, x, sig_e, recip_refined(sig_e), cmp, K.taper_rdiv);
       if (ANGLE) {
         const int a = kdm_angle_bin(qs, qr, K);
         if (a >= 0 && a < K.nang) atomicAdd(hist + it, w * model[moff + i + (size_t)a * K.nz]);
@@ -468,6 +515,8 @@ struct seisfat_kdm_s {
   int* d_cid = nullptr;
   std::vector<char> tab_set;  // per table (sources, then receivers): quantities evaluated
   int tabs_missing = 0;
+  unsigned s_version = 0;      // the handle's slowness the set tables were evaluated with
+  bool taper_geom = false;     // the divisors and x positions allow rdiv for the taper (taper_geometry_ok)
   bool rec_dirty = true;       // receiver minima changed since rec_nonneg was derived
 };
 
@@ -510,6 +559,32 @@ template <class T>
 void kfree(T*& p) {
   if (p) (void)hipFree((void*)p);
   p = nullptr;
+}
+
+// The aperture quotient (x - cmp) / sig_e needs no operand rescaling (rdiv is
+// then IEEE division) when every divisor sig_e = tan_ap z + EPS lies in
+// [EPS, 2^40) (tan_ap >= 0: it grows with z, largest at the last row) and
+// every x = j dx and every CMP is 0 or of magnitude in [2^-36, 2^40): the
+// difference of two such floats is 0 or of magnitude in [2^-59, 2^41) (both
+// are multiples of 2^-59), and the quotient normal.  Here the divisors and the
+// x positions; set_traces checks the CMPs.
+bool taper_geometry_ok(const KdmK& K) {
+  if (!(K.tan_ap >= 0.0f) || !std::isfinite(K.tan_ap)) return false;
+  const double zmax = (double)(K.nz - 1) * K.dz, xmax = (double)(K.nx - 1) * K.dx;
+  if (!((double)K.tan_ap * zmax + KDM_EPS <= 0x1p39)) return false;
+  return (K.nx == 1 || K.dx >= 0x1p-36f) && xmax <= 0x1p39;
+}
+
+bool cmp_ok(float c) { return c == 0.0f || (std::fabs(c) >= 0x1p-36f && std::fabs(c) < 0x1p40f); }
+
+// The table quantities hold the slowness they were evaluated with (the
+// amplitudes): a slowness upload to the handle makes every table set before it
+// stale, to be set again.
+void track_slowness(seisfat_kdm_s* k) {
+  if (k->s_version == k->h->version) return;
+  std::fill(k->tab_set.begin(), k->tab_set.end(), 0);
+  k->tabs_missing = k->nsrc + k->nrec;
+  k->s_version = k->h->version;
 }
 
 void free_traces(seisfat_kdm_s* k) {
@@ -597,6 +672,9 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
   }
   k->tab_set.assign(nsrc + nrec, 0);
   k->tabs_missing = nsrc + nrec;
+  k->s_version = h->version;
+  k->taper_geom = taper_geometry_ok(K);
+  K.taper_rdiv = 0;
   *out = k;
   return SEISFAT_OK;
 }
@@ -604,12 +682,13 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
 int seisfat_kdm_set_tables(seisfat_kdm_t k, const float* d_src, const float* d_rec, void* stream) {
   if (!k || !d_src || !d_rec) return kfail(SEISFAT_EINVAL, "null operator or table pointer");
   hipStream_t st = (hipStream_t)stream;
+  track_slowness(k);
   const size_t nq = (size_t)k->nsrc * k->K.npts;
   const int blocks = (int)std::min<size_t>((nq + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_src, k->nsrc, k->K, k->d_q);
+  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_src, k->nsrc, k->h->d_spad, k->K, k->d_q);
   const size_t nr = (size_t)k->nrec * k->K.npts;
   const int blocks_r = (int)std::min<size_t>((nr + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks_r), dim3(256), 0, st, d_rec, k->nrec, k->K,
+  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks_r), dim3(256), 0, st, d_rec, k->nrec, k->h->d_spad, k->K,
                      k->d_q + (size_t)k->nsrc * KQ * k->K.npts);
   const int ntab = k->nsrc + k->nrec;
   const size_t nw = (size_t)ntab * k->K.nchunk;
@@ -629,8 +708,9 @@ int seisfat_kdm_set_table(seisfat_kdm_t k, int which, int index, const float* d_
   if (index < 0 || index >= n) return kfail(SEISFAT_EINVAL, "table index out of range");
   const int tab = (which ? k->nsrc : 0) + index;
   hipStream_t st = (hipStream_t)stream;
+  track_slowness(k);
   const int blocks = (int)std::min<size_t>(((size_t)k->K.npts + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_table, 1, k->K,
+  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_table, 1, k->h->d_spad, k->K,
                      k->d_q + (size_t)tab * KQ * k->K.npts);
   const int blocks_m = (int)std::min<size_t>(((size_t)k->K.nchunk + 3) / 4, 16384);
   hipLaunchKernelGGL(k_kdm_tmin, dim3(blocks_m), dim3(256), 0, st, k->d_q + (size_t)tab * KQ * k->K.npts, 1,
@@ -650,6 +730,8 @@ namespace {
 // receiver minima (a source's run of traces may be passed over on the source
 // minimum alone when no receiver traveltime is negative or NaN).
 int ready(seisfat_kdm_s* k, hipStream_t st) {
+  if (k->s_version != k->h->version)
+    return kfail(SEISFAT_ESTATE, "the slowness changed since the tables were set (set them again)");
   if (k->tabs_missing) return kfail(SEISFAT_ESTATE, "tables not set (seisfat_kdm_set_tables / _set_table)");
   if (k->rec_dirty) {
     std::vector<float> rmin((size_t)k->nrec * k->K.nchunk);
@@ -679,6 +761,7 @@ int seisfat_kdm_set_traces(seisfat_kdm_t k, int ntr, const int* src, const int* 
   }
   KHIP(hipDeviceSynchronize());  // runs in flight may read the previous trace tables
   free_traces(k);
+  k->K.taper_rdiv = (k->taper_geom && std::all_of(cmp, cmp + ntr, cmp_ok)) ? 1 : 0;
   if (ntr == 0) return SEISFAT_OK;
   KHIP(hipMalloc((void**)&k->d_src, ntr * sizeof(int)));
   KHIP(hipMalloc((void**)&k->d_rec, ntr * sizeof(int)));

@@ -1,8 +1,10 @@
 """Kirchhoff depth migration on the GPU against the oracle.
 
-Bars (the weights use the GPU's acosf / cosf / expf, which differ from the
-host's by an ulp or two; traveltime sums, time indices, the convolutions and
-the trace order of the image-domain adjoint are exact):
+Bars (the GPU forms each weight from per-table amplitudes and half-angle
+cosines and sines, within a few ulp of the reference's order of roundings,
+and its acosf / expf differ from the host's by an ulp or two; traveltime sums,
+time indices, the convolutions and the trace order of the image-domain
+adjoint are exact):
   - convolutions: bit-identical (the same double sums in the same order);
   - image-domain adjoint: |m_gpu - m_oracle| <= 1e-5 * max |m_oracle|;
   - forward (LDS accumulation order differs from the reference's atomics):
@@ -99,11 +101,17 @@ def test_convolution_bit_exact_every_kernel(oracle, torch, adjoint, nt, dt, fmax
     assert np.array_equal(out.cpu().numpy(), oracle.trace_convolve(x, w, adjoint))
 
 
-def test_image_adjoint_matches_oracle(oracle, setup, torch):
+# (aperture, dt, nt): the sample case; a negative aperture (negative taper
+# divisors: IEEE division for (x - cmp) / sigma); a time step above 1 s (the
+# time index by IEEE division below 2^-100)
+@pytest.mark.parametrize("aperture,dt,nt", [(30.0, 2e-3, 420), (-30.0, 2e-3, 420), (30.0, 1.25, 3)])
+def test_image_adjoint_matches_oracle(oracle, setup, torch, aperture, dt, nt):
     st = setup
-    K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, tables=(st["ts"], st["tr"]))
+    K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, nt=nt, dt=dt, aperture=aperture,
+             tables=(st["ts"], st["tr"]))
     assert K.ntr > 20
     S, op = orc_args(oracle, K, st)
+    op["aperture"] = aperture
     rng = np.random.default_rng(5)
     d = rng.standard_normal((K.ntr, K.nt)).astype(np.float32)
     m0 = rng.standard_normal(K.model_shape).astype(np.float32) * 1e-3
@@ -255,6 +263,53 @@ def test_argument_errors(setup, torch):
     m = torch.zeros((e.nx, e.nz), device="cuda")
     assert capi.lib().seisfat_kdm_adjoint(h, C.c_void_p(d.data_ptr()), C.c_void_p(m.data_ptr()), None) == capi.ESTATE
     capi.lib().seisfat_kdm_destroy(h)
+
+
+def test_tables_follow_the_slowness(oracle, setup, torch):
+    """The table quantities hold the slowness they were evaluated with: after
+    a slowness upload the operators refuse to run until every table is set
+    again, and then weigh with the new slowness (the oracle on the new S)."""
+    import ctypes as C
+    from seisfat2d_amd import capi
+    from seisfat2d_amd.capi import SeisfatError
+    st = setup
+    m = st["m"]
+    e = Eikonal(m.s.shape[1], m.s.shape[0], m.dz, m.dx)
+    try:
+        e.set_slowness(m.s)
+        K = make(e, m, st["xs"], st["xr"], mig.IMAGE, tables=(st["ts"], st["tr"]))
+        d = torch.from_numpy(np.random.default_rng(21).standard_normal((K.ntr, K.nt)).astype(np.float32)).cuda()
+        m1 = K.adjoint(d).cpu().numpy()
+        s2 = (m.s * 1.25).astype(np.float32)
+        e.set_slowness(s2)
+        with pytest.raises(SeisfatError) as ex:
+            K.adjoint(d)
+        assert ex.value.code == capi.ESTATE and "slowness changed" in str(ex.value)
+        ts = torch.from_numpy(np.ascontiguousarray(st["ts"], np.float32)).cuda()
+        tr = torch.from_numpy(np.ascontiguousarray(st["tr"], np.float32)).cuda()
+        # all but the last receiver table: still refused
+        for which, tabs, n in ((0, ts, K.nsrc), (1, tr, K.nrec - 1)):
+            for k in range(n):
+                capi.check(capi.lib().seisfat_kdm_set_table(K._k, which, k, C.c_void_p(tabs[k].data_ptr()), None))
+        with pytest.raises(SeisfatError) as ex:
+            K.adjoint(d)
+        assert ex.value.code == capi.ESTATE and "tables not set" in str(ex.value)
+        capi.check(capi.lib().seisfat_kdm_set_table(K._k, 1, K.nrec - 1, C.c_void_p(tr[-1].data_ptr()), None))
+        got = K.adjoint(d).cpu().numpy()
+        _, op = orc_args(oracle, K, st)
+        S2 = oracle.expand_boundary(s2)
+        want = oracle.kdm(0, True, S2, st["ts"], st["tr"], K.src, K.rec, K.cmp, None, d.cpu().numpy(),
+                          np.zeros(K.model_shape, np.float32), **op)
+        assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+        assert np.abs(got - m1).max() > 1e-3 * np.abs(m1).max()   # the slowness entered the weights
+        # back to the first slowness, every table again: the first image, bit for bit
+        e.set_slowness(m.s)
+        capi.check(capi.lib().seisfat_kdm_set_tables(K._k, C.c_void_p(ts.data_ptr()), C.c_void_p(tr.data_ptr()),
+                                                     None))
+        assert np.array_equal(K.adjoint(d).cpu().numpy(), m1)
+        K.close()
+    finally:
+        e.close()
 
 
 def test_no_traces_within_the_offset(setup, torch):

@@ -340,3 +340,61 @@ def test_threaded_image_adjoint_is_the_serial_one(o):
     m1 = o.kdm(0, True, *args, d, np.zeros((nx, nz), np.float32), 1e-3, dx, dz, 30.0)
     m8 = o.kdm(0, True, *args, d, np.zeros((nx, nz), np.float32), 1e-3, dx, dz, 30.0, nthreads=8)
     assert np.abs(m1).max() > 0 and np.array_equal(m1, m8)
+
+
+def test_folded_weight_within_a_few_ulp(o):
+    """The GPU folds the weight's per-table factors into one amplitude per
+    table node, amp = sqrt(det cos) / (norm sqrt(max(T/S, EPS))) in double,
+    and the angle factor's cos((th_s + th_r) / 2) into the half angles'
+    cosines and sines.  Restated here in float32 next to the reference's order
+    of roundings (src/migration/migration.cu:417-475) on solved tables of a
+    rough model with unequal spacing: every weight above 1e-30 within 8 ulp.
+    (The GPU's weights themselves are checked against the oracle's on the
+    GPU.)"""
+    from seisfat2d_amd import models
+    f32 = np.float32
+    nz, nx, dz, dx = 40, 72, 7.0, 10.0
+    m = models.rough(nz, nx, dz, dx, seed=3, vlo=1500.0, vhi=4500.0, sigma=1.0)
+    S = o.expand_boundary(m.s)
+    sn = m.s.astype(f32)
+
+    def table(xpos):
+        T = o.solve_iso(S, dx, dz, float(xpos), 0.0)
+        c = T[3:-3, 3:-3]
+        up, dn, lf, rt = T[3:-3, 2:-4], T[3:-3, 4:-2], T[2:-4, 3:-3], T[4:-2, 3:-3]
+        d_z = f32(0.5) * (dn - up) / f32(dz)
+        d_x = f32(0.5) * (rt - lf) / f32(dx)
+        d2z = (dn - f32(2) * c + up) / f32(dz * dz)
+        d2x = (rt - f32(2) * c + lf) / f32(dx * dx)
+        dxz = (T[4:-2, 4:-2] - T[4:-2, 2:-4] - T[2:-4, 4:-2] + T[2:-4, 2:-4]) / (f32(4) * f32(dx) * f32(dz))
+        norm = np.sqrt(d_x * d_x + d_z * d_z) + f32(EPS)
+        det = np.abs(d2x * d2z - dxz * dxz) + f32(EPS)
+        dot = (d_z / norm) * f32(-1)
+        th = np.arccos(np.clip(dot, f32(-1), f32(1)))
+        rs = np.maximum(c / sn, f32(EPS))
+        amp = (np.sqrt(det.astype(np.float64) * np.abs(dot)) / (norm.astype(np.float64) * np.sqrt(rs))).astype(f32)
+        return dict(det=det, norm=norm, cos=np.abs(dot), th=th, rs=rs, amp=amp,
+                    ch=np.cos(0.5 * th.astype(np.float64)).astype(f32), sh=np.sin(0.5 * th.astype(np.float64)).astype(f32))
+
+    x = (np.arange(nx, dtype=f32) * f32(dx))[:, None] * np.ones((1, nz), f32)
+    z = np.ones((nx, 1), f32) * (np.arange(nz, dtype=f32) * f32(dz))[None, :]
+    tan_ap = f32(math.tan(math.radians(30.0)))
+    scale = f32(1.0 / (2.0 * math.pi))
+    rels = []
+    for xs in (60.0, 400.0):
+        s = table(xs)
+        for xr in (0.0, 250.0, 700.0):
+            r = table(xr)
+            cmp = f32(0.5) * (f32(xs) + f32(xr))
+            a = (x - cmp) / (tan_ap * z + f32(EPS))
+            taper = np.exp(f32(-0.5) * a * a)
+            ref = (scale * (np.sqrt(s["det"] * r["det"]) * sn / (s["norm"] * r["norm"]))
+                   * np.sqrt(s["cos"] * r["cos"]) * (f32(1) / np.sqrt(s["rs"] * r["rs"]))
+                   * (f32(1) + f32(0.2) * np.cos(f32(0.5) * (s["th"] + r["th"])))) * taper
+            rf = f32(1) + f32(0.2) * (s["ch"] * r["ch"] - s["sh"] * r["sh"])
+            fold = scale * sn * (s["amp"] * r["amp"]) * rf * taper
+            keep = np.abs(ref) > 1e-30
+            rels.append(np.abs(fold[keep].astype(np.float64) - ref[keep]) / np.abs(ref[keep]))
+    rel = np.concatenate(rels)
+    assert rel.size > 10000
+    assert rel.max() <= 8 * 2.0 ** -23, rel.max() / 2.0 ** -23
