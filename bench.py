@@ -409,10 +409,11 @@ def bench_migration(steps=3, cpu_traces=200):
         reach += int((torch.floor((Ts[K.src[t]] + Tr[K.rec[t]]) / dt32) < nt_deep).sum())
     del Ts, Tr
     pairs = K.ntr * w.nz * w.nx
-    # the launch is VALU-bound (counters): its wave-level VALU instructions
-    # (the committed profile's per-pair count times the pairs that reach) over
-    # the launch time, against the issue rate of 1024 SIMDs at one wave64
-    # instruction per 4 cycles at 2.4 GHz
+    # the launch's busiest resource is VALU issue (counters; its fetch past L2
+    # comes next): its wave-level VALU instructions (the committed profile's
+    # per-pair count times the pairs that reach) over the launch time, against
+    # the issue rate of 1024 SIMDs at one wave64 instruction per 4 cycles at
+    # 2.4 GHz
     prof = profile_figures(MIG_COUNTERS_FILE)
     per_pair = prof.get("valu_instructions_per_pair")
     valu = per_pair * reach / 64 if per_pair else None
