@@ -67,7 +67,9 @@ int seisfat_kdm_set_traces(seisfat_kdm_t k, int ntr, const int* src, const int* 
  * (image_domain_adjoint_kernel / angle_domain_adjoint_kernel,
  * src/migration/migration.cu:398-483,572-663).  d_data: ntr x nt device
  * array (each trace after the adjoint convolution); d_model: the domain's
- * model (device). */
+ * model (device).  The image domain keeps partial images in a buffer of the
+ * operator (sized by seisfat_kdm_set_traces, at most 1 GiB): calls of one
+ * operator run one at a time (one stream, or synchronised between streams). */
 int seisfat_kdm_adjoint(seisfat_kdm_t k, const float* d_data, float* d_model, void* stream);
 /* data += L model: perform_forward for every trace
  * (image_domain_forward_kernel / angle_domain_forward_kernel,

@@ -24,10 +24,11 @@
 //     there); the other quantities are loaded only for nodes that reach it;
 //   - adjoint: one thread per image point walks the traces grouped in source
 //     blocks (four sources' quantities in registers, each receiver's loaded
-//     once for up to four traces); image domain: the point's sum stays in a
-//     register, deterministic, no atomics (contributions block by block, by
-//     receiver, by source); angle domain: adds into the (depth, angle, CMP)
-//     bins with atomics, as the reference (their order varies);
+//     once for up to four traces), the blocks spread over up to 16 waves per
+//     64 points; image domain: each wave's sum stays in a register and the
+//     partial images are added in a fixed order, deterministic, no atomics;
+//     angle domain: adds into the (depth, angle, CMP) bins with atomics, as
+//     the reference (their order varies);
 //   - forward: one workgroup per trace accumulates the trace in an LDS
 //     histogram of its nt samples (LDS atomics) and adds it to the trace once;
 //   - convolution: the FFT product of the reference is a linear
@@ -232,14 +233,24 @@ __device__ __forceinline__ int kdm_angle_bin(const Q7& s, const Q7& r, const Kdm
 // for up to SB node-trace pairs.  It passes over a pair when the chunk minima
 // keep every node beyond the last sample, over a source slot when the source
 // minimum alone does (receiver tables non-negative), and loads nothing for
-// either.  Image domain: the point's sum stays in a register (no atomics) and
-// takes the contributions block by block, by receiver, by slot.  Angle
-// domain: adds into the (depth, angle, CMP) bins with atomics, as the
-// reference.
+// either.  The blocks are split into gridDim.y consecutive parts, one wave per
+// chunk and part, so that a chunk's work spreads over several waves.  Image
+// domain: a part's sum stays in a register (no atomics), taking the
+// contributions block by block, by receiver, by slot; with one part it is
+// added to the model, with several it is written to part[h] and
+// k_kdm_sum_parts adds the parts in order.  Angle domain: adds into the
+// (depth, angle, CMP) bins with atomics, as the reference.
 #ifndef SEISFAT_KDM_SB
 #define SEISFAT_KDM_SB 4
 #endif
 constexpr int KDM_SB = SEISFAT_KDM_SB;
+// Parts per chunk, at most (measured on a 1001 x 1001 image with 13 blocks:
+// more waves per chunk, up to one block each, keep shortening both the
+// 1 s and the 8 s record's adjoint).
+#ifndef SEISFAT_KDM_PARTS
+#define SEISFAT_KDM_PARTS 16
+#endif
+constexpr int KDM_PARTS = SEISFAT_KDM_PARTS;
 
 template <bool ANGLE, int SB>
 __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q, const float* __restrict__ tmin,
@@ -247,8 +258,10 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
                                                      const int* __restrict__ bent, const int* __restrict__ ent,
                                                      const float* __restrict__ tcmp, const int* __restrict__ tcid,
                                                      int nb, int nsrc, const float* __restrict__ data,
-                                                     float* __restrict__ model, KdmK K) {
+                                                     float* __restrict__ model, float* __restrict__ part, KdmK K) {
   const int p0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nparts = gridDim.y, h = blockIdx.y;
+  const int b_begin = (int)((long long)nb * h / nparts), b_end = (int)((long long)nb * (h + 1) / nparts);
   const bool live = p0 < K.npts;
   const int p = live ? p0 : K.npts - 1;  // lanes past the end load in bounds and store nothing
   const int c = __builtin_amdgcn_readfirstlane(p0 >> 6);
@@ -259,8 +272,8 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
   const float sS = K.scale * S[(i + 3) + (size_t)(j + 3) * K.nzz];
   const float sig_e = K.tan_ap * z + KDM_EPS;
   const float r_sig = recip_refined(sig_e);
-  float acc = (!ANGLE && live) ? model[p] : 0.0f;
-  for (int b = 0; b < nb; ++b) {
+  float acc = (!ANGLE && live && nparts == 1) ? model[p] : 0.0f;
+  for (int b = b_begin; b < b_end; ++b) {
     // slot k in use: bit k (one scalar register for the block)
     float ms[SB];
     unsigned use = 0;
@@ -373,7 +386,23 @@ __global__ __launch_bounds__(256) void k_kdm_adjoint(const float* __restrict__ q
       }
     }
   }
-  if (!ANGLE && live) model[p] = acc;
+  if (!ANGLE && live) {
+    if (nparts == 1)
+      model[p] = acc;
+    else
+      part[(size_t)h * K.npts + p] = acc;
+  }
+}
+
+// model[p] += part[0][p] + part[1][p] + ... in that order (the image adjoint's
+// partial sums).
+__global__ __launch_bounds__(256) void k_kdm_sum_parts(float* __restrict__ model, const float* __restrict__ part,
+                                                       int nparts, int npts) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  float v = model[p];
+  for (int h = 0; h < nparts; ++h) v += part[(size_t)h * npts + p];
+  model[p] = v;
 }
 
 // Forward (image or angle domain): one workgroup per trace, the trace's nt
@@ -513,6 +542,8 @@ struct seisfat_kdm_s {
   int* d_ent = nullptr;    // entries {receiver, trace per slot}
   float* d_cmp = nullptr;
   int* d_cid = nullptr;
+  int nparts = 1;            // adjoint waves per chunk (block ranges)
+  float* d_part = nullptr;   // image adjoint partial sums, nparts x npts (nparts > 1)
   std::vector<char> tab_set;  // per table (sources, then receivers): quantities evaluated
   int tabs_missing = 0;
   unsigned s_version = 0;      // the handle's slowness the set tables were evaluated with
@@ -595,7 +626,9 @@ void free_traces(seisfat_kdm_s* k) {
   kfree(k->d_bsrc);
   kfree(k->d_bent);
   kfree(k->d_ent);
+  kfree(k->d_part);
   k->ntr = k->nb = 0;
+  k->nparts = 1;
 }
 
 }  // namespace
@@ -807,6 +840,14 @@ int seisfat_kdm_set_traces(seisfat_kdm_t k, int ntr, const int* src, const int* 
   KHIP(hipMemcpy(k->d_bent, bent.data(), bent.size() * sizeof(int), hipMemcpyHostToDevice));
   KHIP(hipMemcpy(k->d_ent, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice));
   k->nb = nb;
+  // adjoint parts: up to KDM_PARTS, no more than the blocks; the image
+  // domain's partial sums within 1 GiB
+  int np = std::max(1, std::min(KDM_PARTS, nb));
+  if (k->p.domain == SEISFAT_KDM_IMAGE) {
+    while (np > 1 && (size_t)np * k->K.npts * sizeof(float) > ((size_t)1 << 30)) --np;
+    if (np > 1) KHIP(hipMalloc((void**)&k->d_part, (size_t)np * k->K.npts * sizeof(float)));
+  }
+  k->nparts = np;
   std::vector<int> cid(ntr, 0);
   if (k->p.domain == SEISFAT_KDM_ANGLE) cid.assign(cmp_id, cmp_id + ntr);
   KHIP(hipMemcpy(k->d_src, src, ntr * sizeof(int), hipMemcpyHostToDevice));
@@ -824,14 +865,19 @@ int seisfat_kdm_adjoint(seisfat_kdm_t k, const float* d_data, float* d_model, vo
   if (k->ntr == 0) return SEISFAT_OK;  // no traces: the model is unchanged
   if (!d_data || !d_model) return kfail(SEISFAT_EINVAL, "null buffer");
   const int blocks = (k->K.npts + 255) / 256;
-  if (k->p.domain == SEISFAT_KDM_IMAGE)
-    hipLaunchKernelGGL((k_kdm_adjoint<false, KDM_SB>), dim3(blocks), dim3(256), 0, st, k->d_q, k->d_tmin,
-                       k->h->d_spad, k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data,
-                       d_model, k->K);
-  else
-    hipLaunchKernelGGL((k_kdm_adjoint<true, KDM_SB>), dim3(blocks), dim3(256), 0, st, k->d_q, k->d_tmin,
-                       k->h->d_spad, k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data,
-                       d_model, k->K);
+  const dim3 grid(blocks, k->nparts);
+  if (k->p.domain == SEISFAT_KDM_IMAGE) {
+    hipLaunchKernelGGL((k_kdm_adjoint<false, KDM_SB>), grid, dim3(256), 0, st, k->d_q, k->d_tmin, k->h->d_spad,
+                       k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data, d_model,
+                       k->d_part, k->K);
+    if (k->nparts > 1)
+      hipLaunchKernelGGL(k_kdm_sum_parts, dim3(blocks), dim3(256), 0, st, d_model, k->d_part, k->nparts,
+                         k->K.npts);
+  } else {
+    hipLaunchKernelGGL((k_kdm_adjoint<true, KDM_SB>), grid, dim3(256), 0, st, k->d_q, k->d_tmin, k->h->d_spad,
+                       k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data, d_model,
+                       nullptr, k->K);
+  }
   KHIP(hipGetLastError());
   return SEISFAT_OK;
 }

@@ -269,6 +269,72 @@ def test_argument_errors(setup, torch):
     capi.lib().seisfat_kdm_destroy(h)
 
 
+@pytest.mark.parametrize("domain", [mig.IMAGE, mig.ANGLE])
+def test_adjoint_over_several_block_parts(oracle, setup, torch, domain):
+    """13 sources: four source blocks, which the adjoint spreads over several
+    waves per chunk (partial images added in order in the image domain)."""
+    st = setup
+    e = st["e"]
+    xs = np.linspace(20.0, 810.0, 13).astype(np.float32)
+    ts = e.traveltime_tables(xs, np.zeros_like(xs))
+    K = make(e, st["m"], xs, st["xr"], domain, tables=(ts, st["tr"]))
+    S, op = orc_args(oracle, K, st)
+    cid = K.cmp_id if domain == mig.ANGLE else None
+    rng = np.random.default_rng(41)
+    d = rng.standard_normal((K.ntr, K.nt)).astype(np.float32)
+    m0 = (rng.standard_normal(K.model_shape) * 1e-3).astype(np.float32) if domain == mig.IMAGE else \
+        np.zeros(K.model_shape, np.float32)
+    got = K.adjoint(torch.from_numpy(d).cuda(), torch.from_numpy(m0.copy()).cuda()).cpu().numpy()
+    want = oracle.kdm(domain, True, S, ts, st["tr"], K.src, K.rec, K.cmp, cid, d, m0.copy(), **op)
+    bound = oracle.kdm(domain, True, S, ts, st["tr"], K.src, K.rec, K.cmp, cid, np.abs(d), np.abs(m0), **op)
+    assert np.abs(want).max() > 0
+    bad = np.abs(got - want) > 1e-5 * bound + 1e-30
+    assert bad.mean() <= (0 if domain == mig.IMAGE else 1e-3), bad.sum()
+    # the same adjoint twice: the image domain is deterministic
+    again = K.adjoint(torch.from_numpy(d).cuda(), torch.from_numpy(m0.copy()).cuda()).cpu().numpy()
+    if domain == mig.IMAGE:
+        assert np.array_equal(again, got)
+    K.close()
+
+
+@pytest.mark.parametrize("domain", [mig.IMAGE, mig.ANGLE])
+def test_sums_outside_the_fast_division_range(oracle, setup, torch, domain):
+    """Traveltime sums the known-divisor division does not serve: the first
+    source table shifted down by 4 ms (negative sums near that source, nodes
+    the reference's negative index leaves out) and a receiver on every source
+    (zero-offset traces: sum 0 at the other sources' nodes, index 0).  The
+    waves holding such a sum take the IEEE path; the results match the
+    oracle."""
+    st = setup
+    xr = np.sort(np.concatenate([st["xr"], st["xs"]])).astype(np.float32)
+    e = st["e"]
+    ts = st["ts"].copy()
+    ts[0] -= np.float32(4e-3)
+    assert (ts[1:, 3:-3, 3:-3] == 0).sum() == len(ts) - 1      # sources on grid nodes
+    tr = e.traveltime_tables(xr, np.zeros_like(xr))
+    K = make(e, st["m"], st["xs"], xr, domain, tables=(ts, tr))
+    assert (K.src.size > 0) and np.any(st["xs"][K.src] == xr[K.rec])
+    S, op = orc_args(oracle, K, st)
+    cid = K.cmp_id if domain == mig.ANGLE else None
+    rng = np.random.default_rng(31)
+    d = rng.standard_normal((K.ntr, K.nt)).astype(np.float32)
+    zero = np.zeros(K.model_shape, np.float32)
+    got = K.adjoint(torch.from_numpy(d).cuda()).cpu().numpy()
+    want = oracle.kdm(domain, True, S, ts, tr, K.src, K.rec, K.cmp, cid, d, zero.copy(), **op)
+    bound = oracle.kdm(domain, True, S, ts, tr, K.src, K.rec, K.cmp, cid, np.abs(d), zero.copy(), **op)
+    assert np.abs(want).max() > 0
+    bad = np.abs(got - want) > 1e-5 * bound + 1e-30
+    assert bad.mean() <= (0 if domain == mig.IMAGE else 1e-3), bad.sum()
+    m = rng.standard_normal(K.model_shape).astype(np.float32)
+    d0 = np.zeros((K.ntr, K.nt), np.float32)
+    got = K.forward(torch.from_numpy(m).cuda()).cpu().numpy()
+    want = oracle.kdm(domain, False, S, ts, tr, K.src, K.rec, K.cmp, cid, d0.copy(), m, **op)
+    bound = oracle.kdm(domain, False, S, ts, tr, K.src, K.rec, K.cmp, cid, d0.copy(), np.abs(m), **op)
+    bad = np.abs(got - want) > 1e-5 * bound + 1e-30
+    assert bad.mean() <= (0 if domain == mig.IMAGE else 1e-3), bad.sum()
+    K.close()
+
+
 def test_tables_follow_the_slowness(oracle, setup, torch):
     """The table quantities hold the slowness they were evaluated with: after
     a slowness upload the operators refuse to run until every table is set
