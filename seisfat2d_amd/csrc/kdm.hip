@@ -29,8 +29,9 @@
 //     partial images are added in a fixed order, deterministic, no atomics;
 //     angle domain: adds into the (depth, angle, CMP) bins with atomics, as
 //     the reference (their order varies);
-//   - forward: one workgroup per trace accumulates the trace in an LDS
-//     histogram of its nt samples (LDS atomics) and adds it to the trace once;
+//   - forward: one workgroup (sixteen waves) per trace accumulates the trace
+//     in an LDS histogram of its nt samples (LDS atomics) and adds it to the
+//     trace once;
 //   - convolution: the FFT product of the reference is a linear
 //     (cross-)correlation with the wavelet, summed directly in double, the
 //     trace and the wavelet staged in LDS.
@@ -408,8 +409,17 @@ __global__ __launch_bounds__(256) void k_kdm_sum_parts(float* __restrict__ model
 // Forward (image or angle domain): one workgroup per trace, the trace's nt
 // samples in LDS; each wave passes over a chunk the minima keep beyond the
 // last sample with two scalar loads.
+// Threads per trace (one workgroup, sixteen waves sharing the trace's LDS
+// histogram).  Measured on the 1001 x 1001 survey against 256 and 512: the
+// 1 s record's forward 0.224 ms against 0.269 and 0.233 (image), 0.32 against
+// 0.44 and 0.37 (angle); the 8 s record's 19.5 ms against 19.9 and 19.7.
+#ifndef SEISFAT_KDM_FWD_THREADS
+#define SEISFAT_KDM_FWD_THREADS 1024
+#endif
+constexpr int KDM_FWD_THREADS = SEISFAT_KDM_FWD_THREADS;
+
 template <bool ANGLE>
-__global__ __launch_bounds__(256) void k_kdm_forward(const float* __restrict__ q, const float* __restrict__ tmin,
+__global__ __launch_bounds__(KDM_FWD_THREADS) void k_kdm_forward(const float* __restrict__ q, const float* __restrict__ tmin,
                                                      const float* __restrict__ S, const int* __restrict__ tsrc,
                                                      const int* __restrict__ trec, const float* __restrict__ tcmp,
                                                      const int* __restrict__ tcid, int nsrc,
@@ -890,10 +900,10 @@ int seisfat_kdm_forward(seisfat_kdm_t k, const float* d_model, float* d_data, vo
   if (!d_data || !d_model) return kfail(SEISFAT_EINVAL, "null buffer");
   const size_t lds = (size_t)k->K.nt * sizeof(float);
   if (k->p.domain == SEISFAT_KDM_IMAGE)
-    hipLaunchKernelGGL(k_kdm_forward<false>, dim3(k->ntr), dim3(256), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
+    hipLaunchKernelGGL(k_kdm_forward<false>, dim3(k->ntr), dim3(KDM_FWD_THREADS), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
                        k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
   else
-    hipLaunchKernelGGL(k_kdm_forward<true>, dim3(k->ntr), dim3(256), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
+    hipLaunchKernelGGL(k_kdm_forward<true>, dim3(k->ntr), dim3(KDM_FWD_THREADS), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
                        k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
   KHIP(hipGetLastError());
   return SEISFAT_OK;
