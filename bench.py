@@ -313,14 +313,15 @@ def bench_tables(ntables=64):
             "tables_per_s": ntables / t_dev, "tables_per_s_with_host_copy": ntables / t_host}
 
 
-def bench_migration(steps=3, cpu_traces=200):
+def bench_migration(steps=3, cpu_traces=200, least_squares=True):
     """Kirchhoff depth migration with the reference's sample migration
     parameters (models.migration: 1001 x 1001 at 10 m, 51 sources, 201
     receivers, offsets below 2000 m, nt 1001 at 1 ms): traveltime tables of
     every source and receiver position (one batch each, left in HBM), then
     per domain the adjoint operator (the migration's kernel, HIP events on
     torch's stream, median of `steps`), the whole migration (gather, adjoint
-    convolution, adjoint operator; wall clock) and the forward operator.
+    convolution, adjoint operator; wall clock), the forward operator and
+    (least_squares) three least-squares iterations (wall clock).
     Reports node-trace pairs per second (the reference launches one thread
     per padded node and trace) and the pairs that reach the trace, and the
     oracle timed on `cpu_traces` traces of the image-domain adjoint on one
@@ -371,9 +372,21 @@ def bench_migration(steps=3, cpu_traces=200):
             K.migrate(seismic)
         torch.cuda.synchronize()
         mig_ms = (time.perf_counter() - t0) / steps * 1e3
+        # least-squares migration (IDLSKDM / ADLSKDM): three iterations, each two
+        # forwards and an adjoint with their convolutions and the step's sums
+        lsm_ms = None
+        if least_squares:
+            smooth = 1e-3 if domain == mig.ANGLE else 0.0
+            K.lsm(seismic, max_it=3, smooth=smooth)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            K.lsm(seismic, max_it=3, smooth=smooth)
+            torch.cuda.synchronize()
+            lsm_ms = (time.perf_counter() - t0) * 1e3
         pairs = K.ntr * w.nz * w.nx
         r = {"traces": K.ntr, "model_shape": list(K.model_shape), "adjoint_ms": adj_ms, "forward_ms": fwd_ms,
              "migration_ms": mig_ms, "traces_per_s": K.ntr / (mig_ms / 1e3),
+             "least_squares_3_iterations_ms": lsm_ms,
              "adjoint_pairs_per_s": pairs / (adj_ms / 1e3), "forward_pairs_per_s": pairs / (fwd_ms / 1e3)}
         if domain == mig.IMAGE:
             # node-trace pairs that reach the trace (time index < nt), counted on the device
@@ -571,7 +584,8 @@ def main():
 
     torch.cuda.init()
     if args.only:
-        run = {"ani": bench_ani, "adjoint": bench_adjoint, "migration": bench_migration}[args.only]
+        run = {"ani": bench_ani, "adjoint": bench_adjoint,
+               "migration": lambda k: bench_migration(k, least_squares=not args.profile)}[args.only]
         print(json.dumps(run(max(1, args.steps))))
         if args.profile:
             calibration_copy()
@@ -675,7 +689,9 @@ def main():
         extras["migration"] = bench_migration(max(1, min(args.steps, 5)))
         m = extras["migration"]
         log(f"migration: image adjoint {m['image']['adjoint_ms']:.2f} ms, migration {m['image']['migration_ms']:.1f} "
-            f"ms for {m['image']['traces']} traces; angle adjoint {m['angle']['adjoint_ms']:.2f} ms")
+            f"ms for {m['image']['traces']} traces; angle adjoint {m['angle']['adjoint_ms']:.2f} ms; 3 least-squares "
+            f"iterations {m['image']['least_squares_3_iterations_ms']:.1f} ms (image), "
+            f"{m['angle']['least_squares_3_iterations_ms']:.1f} ms (angle)")
 
     cpu = None
     if args.cpu_seconds > 0:
