@@ -232,7 +232,9 @@ class Kirchhoff:
         (samples with |d_obs| <= 1e-6 do not count), Fletcher-Reeves
         direction, exact step length, the angle-domain smoothing; returns
         (model, residuals) with max_it + 1 residual norms.  Reductions are
-        taken in float64 (the reference adds in float32 in trace order)."""
+        taken in float64 and rounded to float32, the step's scalars formed in
+        float32, all on the device: no host round trip until the residual
+        norms are returned."""
         torch = _torch()
         if self.domain == ANGLE and smooth and self.nang < 2:
             raise ValueError("angle-domain smoothing needs at least two angle bins")
@@ -241,30 +243,31 @@ class Kirchhoff:
         model = torch.zeros(self.model_shape, dtype=torch.float32, device="cuda")
         direction = torch.zeros_like(model)
         grad_old = torch.zeros_like(model)
+
+        def sumsq(a, b):  # float64 sum of a*b, rounded to float32 (a 0-d device tensor)
+            return torch.sum(a.double() * b.double()).float()
+
         residuo = []
         it = 0
         while True:
             dcal = self.convolve(self.forward(model), adjoint=False)
             r = torch.where(mask, dobs - dcal, torch.zeros_like(dcal))
-            residuals = f32(torch.sum(r.double() * r.double()).item())
+            residuals = sumsq(r, r)
             grad = self.adjoint(self.convolve(r, adjoint=True))
             it += 1
-            residuo.append(float(np.sqrt(residuals)))
+            residuo.append(torch.sqrt(residuals))
             if it > max_it:
                 break
-            beta_num = f32(torch.sum(grad.double() * grad.double()).item())
-            beta_den = f32(torch.sum(grad_old.double() * grad_old.double()).item())
-            beta = beta_num / (beta_den + EPS)
-            direction = grad + float(beta) * direction
-            alpha_num = f32(torch.sum(grad.double() * direction.double()).item())
+            beta = sumsq(grad, grad) / (sumsq(grad_old, grad_old) + float(EPS))
+            direction = grad + beta * direction
+            alpha_num = sumsq(grad, direction)
             grad_old = grad
             dd = self.convolve(self.forward(direction), adjoint=False)
-            alpha_den = f32(torch.sum(dd.double() * dd.double()).item())
-            alpha = alpha_num / (alpha_den + EPS)
+            alpha = alpha_num / (sumsq(dd, dd) + float(EPS))
             if self.domain == ANGLE and smooth:
                 self._regularize(torch, direction, model, smooth)
-            model = model + float(alpha) * direction
-        return model, residuo
+            model = model + alpha * direction
+        return model, [float(v) for v in torch.stack(residuo).cpu().numpy()]
 
     def close(self):
         if getattr(self, "_k", None):
