@@ -361,6 +361,11 @@ def bench_migration(steps=3, cpu_traces=200, least_squares=True):
         K = mig.Kirchhoff(e, xs, zs, xr, zr, P["nt"], P["dt"], P["aperture"], P["max_offset"], P["fmax"],
                           domain=domain, da=P["da"], max_angle=P["max_angle"], tables=(ts, tr))
         seismic = torch.randn((K.nsrc, K.nrec, K.nt), device="cuda", generator=g)
+        if domain == mig.IMAGE:
+            # the table quantities and chunk minima of all the tables (seisfat_kdm_set_tables)
+            ts_d, tr_d = K.set_tables(ts, tr)
+            out["set_tables_ms"] = events_ms(lambda: K.set_tables(ts_d, tr_d))
+            del ts_d, tr_d
         d = K.convolve(K.gather(seismic), adjoint=True)
         model = K.adjoint(d)
         torch.cuda.synchronize()

@@ -16,8 +16,8 @@
 //     then four products, the angle factor and the aperture taper (7 words
 //     per node and table: T, amplitude, half-angle cosine and sine, and the
 //     gradient for the angle domain's bin);
-//   - each table also keeps the minimum traveltime of every 64 consecutive
-//     image points (k_kdm_tmin); rounded addition and division being
+//   - the same pass keeps each table's minimum traveltime over every 64
+//     consecutive image points; rounded addition and division being
 //     monotone, a wave whose chunk minima give a time index >= nt holds no
 //     node that reaches the trace and passes over it with two scalar loads
 //     (with the reference's recording lengths most node-trace pairs end
@@ -85,69 +85,66 @@ __device__ __forceinline__ int f2i_rd(float v) {
 // are S times the product of one amplitude per table,
 //   amp = sqrt(det max(0, cos)) / (norm sqrt(rs))   (in double, rounded once),
 // and its angle factor's cos((theta_s + theta_r) / 2) is
-// ch_s ch_r - sh_s sh_r with ch, sh = cos, sin(theta / 2) (double, rounded
-// once).  The weight a trace forms from them differs from the reference's
-// order of roundings by a few ulp.
-__global__ __launch_bounds__(256) void k_kdm_prep(const float* __restrict__ tables, int ntab,
-                                                  const float* __restrict__ S, KdmK K, float* __restrict__ q) {
-  const size_t n = (size_t)ntab * K.npts;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const size_t tsize = (size_t)K.nzz * (K.nx + 6);
-  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < n; g += stride) {
-    const int tab = (int)(g / K.npts);
-    const int p = (int)(g - (size_t)tab * K.npts);
-    const int i = p % K.nz + 3, j = p / K.nz + 3;
-    const float* __restrict__ T = tables + tab * tsize;
-    const int nzz = K.nzz;
-    const float c = T[i + j * nzz];
-    const float up = T[(i - 1) + j * nzz], dn = T[(i + 1) + j * nzz];
-    const float lf = T[i + (j - 1) * nzz], rt = T[i + (j + 1) * nzz];
-    const float d_z = 0.5f * (dn - up) / K.dz;
-    const float d_x = 0.5f * (rt - lf) / K.dx;
-    const float d2z = (dn - 2.0f * c + up) / (K.dz * K.dz);
-    const float d2x = (rt - 2.0f * c + lf) / (K.dx * K.dx);
-    const float dxz = (T[(i + 1) + (j + 1) * nzz] - T[(i - 1) + (j + 1) * nzz] - T[(i + 1) + (j - 1) * nzz] +
-                       T[(i - 1) + (j - 1) * nzz]) /
-                      (4.0f * K.dx * K.dz);
-    const float norm = sqrtf(d_x * d_x + d_z * d_z) + KDM_EPS;
-    const float det = fabsf(d2x * d2z - dxz * dxz) + KDM_EPS;
-    const float ux = d_x / norm, uz = d_z / norm;
-    const float dot = ux * 0.0f + uz * -1.0f;  // unit gradient . (0, -1), the surface normal
-    const float theta = acosf(fminf(1.0f, fmaxf(-1.0f, dot)));
-    const float rs = fmaxf(c / S[i + j * nzz], KDM_EPS);
-    const double amp = sqrt((double)det * (double)fmaxf(0.0f, fabsf(dot))) / ((double)norm * sqrt((double)rs));
-    float* __restrict__ o = q + (size_t)tab * KQ * K.npts + p;
-    o[0] = c;
-    o[(size_t)1 * K.npts] = (float)amp;
-    o[(size_t)2 * K.npts] = (float)cos(0.5 * (double)theta);
-    o[(size_t)3 * K.npts] = (float)sin(0.5 * (double)theta);
-    o[(size_t)4 * K.npts] = d_x;
-    o[(size_t)5 * K.npts] = d_z;
-    o[(size_t)6 * K.npts] = norm;
-  }
-}
-
-// NaN-propagating minimum: a chunk holding a NaN traveltime is never skipped
-// (its node's time index would be 0).
+// ch_s ch_r - sh_s sh_r with ch, sh = cos, sin(theta / 2).  The weight a
+// trace forms from them differs from the reference's order of roundings by a
+// few ulp.
+//
+// One wave per chunk of 64 consecutive image points (p = i + j*nz) of a table,
+// which also keeps the chunk's minimum traveltime (tmin).  For every node of a
+// chunk, RN(RN(Ts + Tr) / dt) >= RN(RN(min_s + min_r) / dt) (rounded addition
+// and division are monotone), so a chunk whose minima give a time index >= nt
+// holds no node that reaches the trace.  The minimum propagates NaN: a chunk
+// holding a NaN traveltime is never passed over (its node's time index would
+// be 0).
 __device__ __forceinline__ float min_nan(float a, float b) { return (a != a || b != b) ? __builtin_nanf("") : fminf(a, b); }
 
-// Per table, the minimum traveltime over each chunk of 64 consecutive image
-// points (p = i + j*nz): one wave per chunk.  For every node of a chunk,
-// RN(RN(Ts + Tr) / dt) >= RN(RN(min_s + min_r) / dt) (rounded addition and
-// division are monotone), so a chunk whose minima give a time index >= nt
-// holds no node that reaches the trace.
-__global__ __launch_bounds__(256) void k_kdm_tmin(const float* __restrict__ q, int ntab, KdmK K,
+__global__ __launch_bounds__(256) void k_kdm_prep(const float* __restrict__ tables, int ntab,
+                                                  const float* __restrict__ S, KdmK K, float* __restrict__ q,
                                                   float* __restrict__ tmin) {
   const int lane = threadIdx.x & 63;
   const size_t nw = (size_t)ntab * K.nchunk;
   const size_t wstride = (size_t)gridDim.x * (blockDim.x >> 6);
+  const size_t tsize = (size_t)K.nzz * (K.nx + 6);
   for (size_t w = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nw; w += wstride) {
     const int tab = (int)(w / K.nchunk);
-    const int c = (int)(w - (size_t)tab * K.nchunk);
-    const int p = c * 64 + lane;
-    float v = p < K.npts ? q[(size_t)tab * KQ * K.npts + p] : __builtin_inff();
-    for (int off = 32; off >= 1; off >>= 1) v = min_nan(v, __shfl_xor(v, off, 64));
-    if (lane == 0) tmin[w] = v;
+    const int p = (int)(w - (size_t)tab * K.nchunk) * 64 + lane;
+    float tv = __builtin_inff();
+    if (p < K.npts) {
+      const int i = p % K.nz + 3, j = p / K.nz + 3;
+      const float* __restrict__ T = tables + tab * tsize;
+      const int nzz = K.nzz;
+      const float c = T[i + j * nzz];
+      const float up = T[(i - 1) + j * nzz], dn = T[(i + 1) + j * nzz];
+      const float lf = T[i + (j - 1) * nzz], rt = T[i + (j + 1) * nzz];
+      const float d_z = 0.5f * (dn - up) / K.dz;
+      const float d_x = 0.5f * (rt - lf) / K.dx;
+      const float d2z = (dn - 2.0f * c + up) / (K.dz * K.dz);
+      const float d2x = (rt - 2.0f * c + lf) / (K.dx * K.dx);
+      const float dxz = (T[(i + 1) + (j + 1) * nzz] - T[(i - 1) + (j + 1) * nzz] - T[(i + 1) + (j - 1) * nzz] +
+                         T[(i - 1) + (j - 1) * nzz]) /
+                        (4.0f * K.dx * K.dz);
+      const float norm = sqrtf(d_x * d_x + d_z * d_z) + KDM_EPS;
+      const float det = fabsf(d2x * d2z - dxz * dxz) + KDM_EPS;
+      const float ux = d_x / norm, uz = d_z / norm;
+      const float dot = ux * 0.0f + uz * -1.0f;  // unit gradient . (0, -1), the surface normal
+      const float cth = fminf(1.0f, fmaxf(-1.0f, dot));  // cos(theta), theta = acos(cth) in the reference
+      const float rs = fmaxf(c / S[i + j * nzz], KDM_EPS);
+      const double amp = sqrt((double)det * (double)fmaxf(0.0f, fabsf(dot))) / ((double)norm * sqrt((double)rs));
+      float* __restrict__ o = q + (size_t)tab * KQ * K.npts + p;
+      o[0] = c;
+      o[(size_t)1 * K.npts] = (float)amp;
+      // cos and sin of theta / 2 by the half-angle identities (1 +- cth exact
+      // where it matters, by Sterbenz): within 5e-8 of those of acos(cth), closer
+      // than the cosine and sine of the float acosf(cth)
+      o[(size_t)2 * K.npts] = sqrtf(0.5f * (1.0f + cth));
+      o[(size_t)3 * K.npts] = sqrtf(0.5f * (1.0f - cth));
+      o[(size_t)4 * K.npts] = d_x;
+      o[(size_t)5 * K.npts] = d_z;
+      o[(size_t)6 * K.npts] = norm;
+      tv = c;
+    }
+    for (int off = 32; off >= 1; off >>= 1) tv = min_nan(tv, __shfl_xor(tv, off, 64));
+    if (lane == 0) tmin[w] = tv;
   }
 }
 
@@ -634,6 +631,15 @@ bool taper_geometry_ok(const KdmK& K) {
   return (K.nx == 1 || K.dx >= 0x1p-36f) && xmax <= 0x1p39;
 }
 
+// The quantities and chunk minima of ntab consecutive tables (padded grids at
+// d_tables) into table slots tab0, tab0 + 1, ...: one wave per chunk.
+void launch_prep(seisfat_kdm_s* k, const float* d_tables, int tab0, int ntab, hipStream_t st) {
+  const size_t nw = (size_t)ntab * k->K.nchunk;
+  const int blocks = (int)std::min<size_t>((nw + 3) / 4, 16384);
+  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_tables, ntab, k->h->d_spad, k->K,
+                     k->d_q + (size_t)tab0 * KQ * k->K.npts, k->d_tmin + (size_t)tab0 * k->K.nchunk);
+}
+
 bool cmp_ok(float c) { return c == 0.0f || (std::fabs(c) >= 0x1p-36f && std::fabs(c) < 0x1p40f); }
 
 // The table quantities hold the slowness they were evaluated with (the
@@ -744,17 +750,8 @@ int seisfat_kdm_set_tables(seisfat_kdm_t k, const float* d_src, const float* d_r
   if (!k || !d_src || !d_rec) return kfail(SEISFAT_EINVAL, "null operator or table pointer");
   hipStream_t st = (hipStream_t)stream;
   track_slowness(k);
-  const size_t nq = (size_t)k->nsrc * k->K.npts;
-  const int blocks = (int)std::min<size_t>((nq + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_src, k->nsrc, k->h->d_spad, k->K, k->d_q);
-  const size_t nr = (size_t)k->nrec * k->K.npts;
-  const int blocks_r = (int)std::min<size_t>((nr + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks_r), dim3(256), 0, st, d_rec, k->nrec, k->h->d_spad, k->K,
-                     k->d_q + (size_t)k->nsrc * KQ * k->K.npts);
-  const int ntab = k->nsrc + k->nrec;
-  const size_t nw = (size_t)ntab * k->K.nchunk;
-  const int blocks_m = (int)std::min<size_t>((nw + 3) / 4, 16384);
-  hipLaunchKernelGGL(k_kdm_tmin, dim3(blocks_m), dim3(256), 0, st, k->d_q, ntab, k->K, k->d_tmin);
+  launch_prep(k, d_src, 0, k->nsrc, st);
+  launch_prep(k, d_rec, k->nsrc, k->nrec, st);
   KHIP(hipGetLastError());
   std::fill(k->tab_set.begin(), k->tab_set.end(), 1);
   k->tabs_missing = 0;
@@ -770,12 +767,7 @@ int seisfat_kdm_set_table(seisfat_kdm_t k, int which, int index, const float* d_
   const int tab = (which ? k->nsrc : 0) + index;
   hipStream_t st = (hipStream_t)stream;
   track_slowness(k);
-  const int blocks = (int)std::min<size_t>(((size_t)k->K.npts + 255) / 256, 16384);
-  hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_table, 1, k->h->d_spad, k->K,
-                     k->d_q + (size_t)tab * KQ * k->K.npts);
-  const int blocks_m = (int)std::min<size_t>(((size_t)k->K.nchunk + 3) / 4, 16384);
-  hipLaunchKernelGGL(k_kdm_tmin, dim3(blocks_m), dim3(256), 0, st, k->d_q + (size_t)tab * KQ * k->K.npts, 1,
-                     k->K, k->d_tmin + (size_t)tab * k->K.nchunk);
+  launch_prep(k, d_table, tab, 1, st);
   KHIP(hipGetLastError());
   if (!k->tab_set[tab]) {
     k->tab_set[tab] = 1;

@@ -140,17 +140,28 @@ class Kirchhoff:
             tr = eik.traveltime_tables(self.xrec, zrec, type=type)
         else:
             ts, tr = tables
-        shape = (eik.nxx, eik.nzz)
+        self.set_tables(ts, tr)
+        torch.cuda.current_stream().synchronize()
+        cid = self.cmp_id if domain == ANGLE else None
+        check(lib().seisfat_kdm_set_traces(self._k, self.ntr, capi._ptr(self.src), capi._ptr(self.rec),
+                                           capi._ptr(self.cmp), capi._ptr(cid)))
+
+    def set_tables(self, ts, tr):
+        """The source and receiver traveltime tables, (nsrc, nxx, nzz) and
+        (nrec, nxx, nzz) padded grids (numpy or CUDA tensors): their weight
+        quantities are evaluated on the device with the eikonal handle's
+        current slowness (Migration::set_src/rec_travel_times,
+        src/migration/migration.cu:116-171).  After a slowness upload to the
+        handle the operators refuse to run until the tables are set again."""
+        torch = _torch()
+        shape = (self.eik.nxx, self.eik.nzz)
         ts_d = torch.as_tensor(np.asarray(ts, f32) if not torch.is_tensor(ts) else ts, device="cuda")
         tr_d = torch.as_tensor(np.asarray(tr, f32) if not torch.is_tensor(tr) else tr, device="cuda")
         if tuple(ts_d.shape) != (self.nsrc, *shape) or tuple(tr_d.shape) != (self.nrec, *shape):
             raise ValueError(f"tables must be (nsrc, {shape[0]}, {shape[1]}) and (nrec, ...) padded grids")
         ts_d, tr_d = ts_d.to(torch.float32).contiguous(), tr_d.to(torch.float32).contiguous()
         check(lib().seisfat_kdm_set_tables(self._k, _dptr(ts_d), _dptr(tr_d), _stream(torch)))
-        torch.cuda.current_stream().synchronize()
-        cid = self.cmp_id if domain == ANGLE else None
-        check(lib().seisfat_kdm_set_traces(self._k, self.ntr, capi._ptr(self.src), capi._ptr(self.rec),
-                                           capi._ptr(self.cmp), capi._ptr(cid)))
+        return ts_d, tr_d
 
     # -- the operators --------------------------------------------------------
     def _traces(self, torch, d):

@@ -372,10 +372,10 @@ def test_tables_follow_the_slowness(oracle, setup, torch):
                           np.zeros(K.model_shape, np.float32), **op)
         assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
         assert np.abs(got - m1).max() > 1e-3 * np.abs(m1).max()   # the slowness entered the weights
-        # back to the first slowness, every table again: the first image, bit for bit
+        # back to the first slowness, every table again (Kirchhoff.set_tables):
+        # the first image, bit for bit
         e.set_slowness(m.s)
-        capi.check(capi.lib().seisfat_kdm_set_tables(K._k, C.c_void_p(ts.data_ptr()), C.c_void_p(tr.data_ptr()),
-                                                     None))
+        K.set_tables(ts, tr)
         assert np.array_equal(K.adjoint(d).cpu().numpy(), m1)
         K.close()
     finally:

@@ -346,7 +346,7 @@ def test_folded_weight_within_a_few_ulp(o):
     """The GPU folds the weight's per-table factors into one amplitude per
     table node, amp = sqrt(det cos) / (norm sqrt(max(T/S, EPS))) in double,
     and the angle factor's cos((th_s + th_r) / 2) into the half angles'
-    cosines and sines.  Restated here in float32 next to the reference's order
+    cosines and sines, sqrt((1 +- cos th) / 2) in float32.  Restated here in float32 next to the reference's order
     of roundings (src/migration/migration.cu:417-475) on solved tables of a
     rough model with unequal spacing: every weight above 1e-30 within 8 ulp.
     (The GPU's weights themselves are checked against the oracle's on the
@@ -373,8 +373,9 @@ def test_folded_weight_within_a_few_ulp(o):
         th = np.arccos(np.clip(dot, f32(-1), f32(1)))
         rs = np.maximum(c / sn, f32(EPS))
         amp = (np.sqrt(det.astype(np.float64) * np.abs(dot)) / (norm.astype(np.float64) * np.sqrt(rs))).astype(f32)
+        cth = np.clip(dot, f32(-1), f32(1))
         return dict(det=det, norm=norm, cos=np.abs(dot), th=th, rs=rs, amp=amp,
-                    ch=np.cos(0.5 * th.astype(np.float64)).astype(f32), sh=np.sin(0.5 * th.astype(np.float64)).astype(f32))
+                    ch=np.sqrt(f32(0.5) * (f32(1) + cth)), sh=np.sqrt(f32(0.5) * (f32(1) - cth)))
 
     x = (np.arange(nx, dtype=f32) * f32(dx))[:, None] * np.ones((1, nz), f32)
     z = np.ones((nx, 1), f32) * (np.arange(nz, dtype=f32) * f32(dz))[None, :]
