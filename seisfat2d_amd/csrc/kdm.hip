@@ -62,6 +62,7 @@ struct KdmK {
   int taper_rdiv;      // (x - cmp) / sig_e needs no operand rescaling at any node and trace (kdm_weight)
   int rec_nonneg; // every receiver-table value at an image point is >= 0 (no NaN)
   float dx, dz, dt, da;
+  float ntf;     // (float)nt
   float tan_ap;  // tanf(aperture in radians), the reference's sigma / z
   float scale;   // 1 / (2 pi) as the reference rounds it
 };
@@ -178,7 +179,14 @@ __device__ __forceinline__ void q_dir(const float* __restrict__ q, int tab, int 
 // explicitly.
 __device__ __forceinline__ int t_index(float xs, const KdmK& K) {
   const bool mid = __float_as_uint(xs) - 0x0d800000u < 0x64000000u;  // 2^-100 <= xs < 2^100
-  if (K.fastdiv_dt && __all(mid)) return (int)floorf(fminf(fdiv(xs, K.zh_dt, K.zl_dt), (float)K.nt));
+  const uint64_t all_mid = __builtin_amdgcn_ballot_w64(mid);
+  if (K.fastdiv_dt && all_mid == __builtin_amdgcn_read_exec()) {
+    // min(RN(xs / dt), nt) of a positive finite quotient, nt from a scalar
+    // register (no NaN: the compiler's canonicalising fminf is not needed)
+    float v;
+    asm("v_min_f32 %0, %1, %2" : "=v"(v) : "s"(K.ntf), "v"(fdiv(xs, K.zh_dt, K.zl_dt)));
+    return (int)floorf(v);
+  }
   const float v = (K.fastdiv_dt && mid) ? fdiv(xs, K.zh_dt, K.zl_dt) : xs / K.dt;
   const int it = v != v ? 0 : (int)fminf(fmaxf(floorf(v), -1.0f), (float)K.nt);
   return xs >= K.thr ? K.nt : it;
@@ -202,14 +210,15 @@ __device__ __forceinline__ bool beyond(float a, float b, const KdmK& K) { return
 // the trace's CMP.  sS = S / (2 pi) and sig_e = sigma + EPS (with its refined
 // reciprocal r_sig) come in evaluated per node.  The taper is the reference's
 // expression and roundings: (x - cmp) / sig_e, by rdiv with the refined
-// reciprocal when the host found that no operand needs rescaling (rdiv_ok:
-// every divisor in [EPS, 2^40), every dividend 0 or of magnitude in
-// [2^-59, 2^41)), then expf(-0.5 a a).
+// reciprocal when the host found that no operand needs rescaling (RDIV, the
+// operator's taper_rdiv: every divisor in [EPS, 2^40), every dividend 0 or of
+// magnitude in [2^-59, 2^41)), then expf(-0.5 a a).
+template <bool RDIV>
 __device__ __forceinline__ float kdm_weight(const Q7& s, const Q7& r, float sS, float x, float sig_e, float r_sig,
-                                            float cmp, bool rdiv_ok) {
+                                            float cmp) {
   const float rf = __builtin_fmaf(0.2f, __builtin_fmaf(s.ch, r.ch, -(s.sh * r.sh)), 1.0f);
   const float n = x - cmp;
-  const float a = rdiv_ok ? rdiv(n, sig_e, r_sig) : n / sig_e;
+  const float a = RDIV ? rdiv(n, sig_e, r_sig) : n / sig_e;
   const float taper = expf(-0.5f * a * a);
   return sS * (s.amp * r.amp) * rf * taper;
 }
@@ -254,7 +263,7 @@ constexpr int KDM_PARTS = SEISFAT_KDM_PARTS;
 // vector registers instead of 87, no spills): 0.54 against 0.56 ms on the
 // 1001 x 1001 survey.  The image domain's scalar registers allow 7; asking for
 // 8 spills them into vector lanes (measured slower on long records).
-template <bool ANGLE, int SB>
+template <bool ANGLE, int SB, bool RDIV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANGLE ? 6 : 1))) void k_kdm_adjoint(const float* __restrict__ q, const float* __restrict__ tmin,
                                                      const float* __restrict__ S, const int* __restrict__ bsrc,
                                                      const int* __restrict__ bent, const int* __restrict__ ent,
@@ -364,20 +373,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANGLE ? 6 :
         }
         // first the time indices and the data samples of every pair, then the
         // weights: the gathers of one entry are in flight together
+        // (slots whose chunk test passes: bit k, uniform)
         int tk[SB];
         float dv[SB];
         bool ok[SB];
         bool anyok = false;
+        unsigned slots = 0;
 #pragma unroll
         for (int k = 0; k < SB; ++k) {
           tk[k] = __builtin_amdgcn_readlane(el_t[k], li);
           ok[k] = false;
           dv[k] = 0.0f;
           if (tk[k] < 0 || !(use >> k & 1) || beyond(ms[k], mr, K)) continue;
+          slots |= 1u << k;
           const float xs = qs[k].t + rt;
           const int it = t_index(xs, K);
           ok[k] = live && (unsigned)it < (unsigned)K.nt;
-          if (ok[k]) dv[k] = data[(size_t)tk[k] * K.nt + it];
+          // the trace's base (uniform) plus a 32-bit byte offset (it < nt <= 16384)
+          const char* __restrict__ dk = (const char*)(data + (size_t)tk[k] * K.nt);
+          if (ok[k]) dv[k] = *(const float*)(dk + (unsigned)it * 4u);
           anyok |= ok[k];
         }
         if (!__any(anyok)) continue;  // (the while condition reads the updated m)
@@ -385,18 +399,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANGLE ? 6 :
         qr.t = rt;
         q_amp(q, r, p, K.npts, qr);
         if (ANGLE) q_dir(q, r, p, K.npts, qr);
+        // every lane of a slot computes its weight; lanes that do not reach
+        // the trace add nothing (a select, not a product: their weight may be
+        // any value)
 #pragma unroll
         for (int k = 0; k < SB; ++k) {
-          if (!ok[k]) continue;
+          if (!(slots >> k & 1)) continue;
           const float cmp = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(el_c[k]), li));
-          const float w = kdm_weight(qs[k], qr, sS, x, sig_e, r_sig, cmp, K.taper_rdiv);
-          const float v = w * dv[k];
+          const float w = kdm_weight<RDIV>(qs[k], qr, sS, x, sig_e, r_sig, cmp);
           if (ANGLE) {
             const int a = kdm_angle_bin(qs[k], qr, K);
             const int cb = __builtin_amdgcn_readlane(el_a[k], li);
-            if (a >= 0 && a < K.nang) atomicAdd(model + i + (size_t)a * K.nz + (size_t)cb * K.nang * K.nz, v);
+            if (ok[k] && a >= 0 && a < K.nang)
+              atomicAdd(model + i + (size_t)a * K.nz + (size_t)cb * K.nang * K.nz, w * dv[k]);
           } else {
-            acc += v;
+            acc += ok[k] ? w * dv[k] : 0.0f;
           }
         }
       }
@@ -433,7 +450,7 @@ __global__ __launch_bounds__(256) void k_kdm_sum_parts(float* __restrict__ model
 #endif
 constexpr int KDM_FWD_THREADS = SEISFAT_KDM_FWD_THREADS;
 
-template <bool ANGLE>
+template <bool ANGLE, bool RDIV>
 __global__ __launch_bounds__(KDM_FWD_THREADS) void k_kdm_forward(const float* __restrict__ q, const float* __restrict__ tmin,
                                                      const float* __restrict__ S, const int* __restrict__ tsrc,
                                                      const int* __restrict__ trec, const float* __restrict__ tcmp,
@@ -472,7 +489,7 @@ __global__ __launch_bounds__(KDM_FWD_THREADS) void k_kdm_forward(const float* __
       const float z = (float)i * K.dz, x = (float)j * K.dx;
       const float sS = K.scale * S[(i + 3) + (size_t)(j + 3) * K.nzz];
       const float sig_e = K.tan_ap * z + KDM_EPS;
-      const float w = kdm_weight(qs, qr, sS, x, sig_e, recip_refined(sig_e), cmp, K.taper_rdiv);
+      const float w = kdm_weight<RDIV>(qs, qr, sS, x, sig_e, recip_refined(sig_e), cmp);
       if (ANGLE) {
         const int a = kdm_angle_bin(qs, qr, K);
         if (a >= 0 && a < K.nang) atomicAdd(hist + it, w * model[moff + i + (size_t)a * K.nz]);
@@ -640,6 +657,22 @@ void launch_prep(seisfat_kdm_s* k, const float* d_tables, int tab0, int ntab, hi
                      k->d_q + (size_t)tab0 * KQ * k->K.npts, k->d_tmin + (size_t)tab0 * k->K.nchunk);
 }
 
+// The operators' launches, one instantiation per domain and taper division.
+template <bool ANGLE, bool RDIV>
+void launch_adjoint(seisfat_kdm_s* k, const float* d_data, float* d_model, hipStream_t st) {
+  const dim3 grid((k->K.npts + 255) / 256, k->nparts);
+  hipLaunchKernelGGL((k_kdm_adjoint<ANGLE, KDM_SB, RDIV>), grid, dim3(256), 0, st, k->d_q, k->d_tmin, k->h->d_spad,
+                     k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data, d_model,
+                     ANGLE ? nullptr : k->d_part, k->K);
+}
+
+template <bool ANGLE, bool RDIV>
+void launch_forward(seisfat_kdm_s* k, const float* d_model, float* d_data, hipStream_t st) {
+  const size_t lds = (size_t)k->K.nt * sizeof(float);
+  hipLaunchKernelGGL((k_kdm_forward<ANGLE, RDIV>), dim3(k->ntr), dim3(KDM_FWD_THREADS), lds, st, k->d_q, k->d_tmin,
+                     k->h->d_spad, k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
+}
+
 bool cmp_ok(float c) { return c == 0.0f || (std::fabs(c) >= 0x1p-36f && std::fabs(c) < 0x1p40f); }
 
 // The table quantities hold the slowness they were evaluated with (the
@@ -698,6 +731,7 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
   K.nchunk = (int)((npts + 63) / 64);
   K.rec_nonneg = 0;
   K.nt = p->nt;
+  K.ntf = (float)p->nt;
   K.nang = p->domain == SEISFAT_KDM_ANGLE ? p->nang : 0;
   K.dx = h->dx;
   K.dz = h->dz;
@@ -884,19 +918,19 @@ int seisfat_kdm_adjoint(seisfat_kdm_t k, const float* d_data, float* d_model, vo
   if (const int rc = ready(k, st)) return rc;
   if (k->ntr == 0) return SEISFAT_OK;  // no traces: the model is unchanged
   if (!d_data || !d_model) return kfail(SEISFAT_EINVAL, "null buffer");
-  const int blocks = (k->K.npts + 255) / 256;
-  const dim3 grid(blocks, k->nparts);
+  const bool rd = k->K.taper_rdiv != 0;
   if (k->p.domain == SEISFAT_KDM_IMAGE) {
-    hipLaunchKernelGGL((k_kdm_adjoint<false, KDM_SB>), grid, dim3(256), 0, st, k->d_q, k->d_tmin, k->h->d_spad,
-                       k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data, d_model,
-                       k->d_part, k->K);
+    if (rd)
+      launch_adjoint<false, true>(k, d_data, d_model, st);
+    else
+      launch_adjoint<false, false>(k, d_data, d_model, st);
     if (k->nparts > 1)
-      hipLaunchKernelGGL(k_kdm_sum_parts, dim3(blocks), dim3(256), 0, st, d_model, k->d_part, k->nparts,
-                         k->K.npts);
+      hipLaunchKernelGGL(k_kdm_sum_parts, dim3((k->K.npts + 255) / 256), dim3(256), 0, st, d_model, k->d_part,
+                         k->nparts, k->K.npts);
+  } else if (rd) {
+    launch_adjoint<true, true>(k, d_data, d_model, st);
   } else {
-    hipLaunchKernelGGL((k_kdm_adjoint<true, KDM_SB>), grid, dim3(256), 0, st, k->d_q, k->d_tmin, k->h->d_spad,
-                       k->d_bsrc, k->d_bent, k->d_ent, k->d_cmp, k->d_cid, k->nb, k->nsrc, d_data, d_model,
-                       nullptr, k->K);
+    launch_adjoint<true, false>(k, d_data, d_model, st);
   }
   KHIP(hipGetLastError());
   return SEISFAT_OK;
@@ -908,13 +942,17 @@ int seisfat_kdm_forward(seisfat_kdm_t k, const float* d_model, float* d_data, vo
   if (const int rc = ready(k, st)) return rc;
   if (k->ntr == 0) return SEISFAT_OK;  // no traces to add into
   if (!d_data || !d_model) return kfail(SEISFAT_EINVAL, "null buffer");
-  const size_t lds = (size_t)k->K.nt * sizeof(float);
-  if (k->p.domain == SEISFAT_KDM_IMAGE)
-    hipLaunchKernelGGL(k_kdm_forward<false>, dim3(k->ntr), dim3(KDM_FWD_THREADS), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
-                       k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
-  else
-    hipLaunchKernelGGL(k_kdm_forward<true>, dim3(k->ntr), dim3(KDM_FWD_THREADS), lds, st, k->d_q, k->d_tmin, k->h->d_spad,
-                       k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
+  const bool rd = k->K.taper_rdiv != 0;
+  if (k->p.domain == SEISFAT_KDM_IMAGE) {
+    if (rd)
+      launch_forward<false, true>(k, d_model, d_data, st);
+    else
+      launch_forward<false, false>(k, d_model, d_data, st);
+  } else if (rd) {
+    launch_forward<true, true>(k, d_model, d_data, st);
+  } else {
+    launch_forward<true, false>(k, d_model, d_data, st);
+  }
   KHIP(hipGetLastError());
   return SEISFAT_OK;
 }
