@@ -152,7 +152,9 @@ class Kirchhoff:
         quantities are evaluated on the device with the eikonal handle's
         current slowness (Migration::set_src/rec_travel_times,
         src/migration/migration.cu:116-171).  After a slowness upload to the
-        handle the operators refuse to run until the tables are set again."""
+        handle the operators refuse to run until the tables are set again.
+        Returns the float32 device grids used (the inputs themselves when they
+        already are)."""
         torch = _torch()
         shape = (self.eik.nxx, self.eik.nzz)
         ts_d = torch.as_tensor(np.asarray(ts, f32) if not torch.is_tensor(ts) else ts, device="cuda")
