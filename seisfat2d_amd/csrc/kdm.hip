@@ -213,13 +213,30 @@ __device__ __forceinline__ bool beyond(float a, float b, const KdmK& K) { return
 // reciprocal when the host found that no operand needs rescaling (RDIV, the
 // operator's taper_rdiv: every divisor in [EPS, 2^40), every dividend 0 or of
 // magnitude in [2^-59, 2^41)), then expf(-0.5 a a).
+// e^x for a finite x <= 0: 2^(x log2 e), x log2 e split into its rounded
+// product hi and the residual fma(x, L, -hi) + x Llo (L + Llo = log2 e), the
+// nearest integer n of hi applied by ldexp to the hardware 2^f of the
+// remaining f in [-0.5, 0.5].  The same instructions as the library's expf
+// without its overflow and non-finite handling: equal results for x >= -103.97;
+// below, x is clamped to -104 and the result is 0 or the least subnormals
+// (e^x < 2^-149 there).
+__device__ __forceinline__ float exp_nonpos(float x) {
+#pragma clang fp contract(off)  // hi - n must use the rounded hi: lo holds its residual
+  x = fmaxf(x, -104.0f);
+  const float hi = x * 0x1.715476p+0f;
+  const float lo = __builtin_fmaf(x, 0x1.4ae0bep-26f, __builtin_fmaf(x, 0x1.715476p+0f, -hi));
+  const float n = __builtin_rintf(hi);
+  return __builtin_ldexpf(__builtin_amdgcn_exp2f((hi - n) + lo), (int)n);
+}
+
 template <bool RDIV>
 __device__ __forceinline__ float kdm_weight(const Q7& s, const Q7& r, float sS, float x, float sig_e, float r_sig,
                                             float cmp) {
   const float rf = __builtin_fmaf(0.2f, __builtin_fmaf(s.ch, r.ch, -(s.sh * r.sh)), 1.0f);
   const float n = x - cmp;
+  // RDIV: |a| < 2^61, so -0.5 a a is finite and <= 0
   const float a = RDIV ? rdiv(n, sig_e, r_sig) : n / sig_e;
-  const float taper = expf(-0.5f * a * a);
+  const float taper = RDIV ? exp_nonpos(-0.5f * a * a) : expf(-0.5f * a * a);
   return sS * (s.amp * r.amp) * rf * taper;
 }
 
