@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANGLE ? 6 :
       // lane l holds entry e0 + l (receiver, traces, receiver minimum) and
       // tests it: does any of its pairs pass the chunk test?  The entries that
       // pass are then taken one at a time, their words read from that lane.
-      bool need = false;
+      int el_pass = 0;  // bit k: slot k holds a trace that passes the chunk test
       int el_r = 0, el_t[SB], el_a[SB];
       float mr_l = 0.0f, el_c[SB];
 #pragma unroll
@@ -358,9 +358,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANGLE ? 6 :
         for (int k = 0; k < SB; ++k) el_t[k] = El[1 + k];
         mr_l = tmin[(size_t)el_r * K.nchunk + c];
 #pragma unroll
-        for (int k = 0; k < SB; ++k) need |= el_t[k] >= 0 && (use >> k & 1) && !beyond(ms[k], mr_l, K);
+        for (int k = 0; k < SB; ++k) el_pass |= (el_t[k] >= 0 && (use >> k & 1) && !beyond(ms[k], mr_l, K)) << k;
         // the traces' CMP (and bin), read from this lane when the entry is taken
-        if (need) {
+        if (el_pass) {
 #pragma unroll
           for (int k = 0; k < SB; ++k) {
             if (el_t[k] >= 0) {
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANGLE ? 6 :
           }
         }
       }
-      uint64_t m = __ballot(need);
+      uint64_t m = __ballot(el_pass != 0);
       // the next entry's receiver traveltime is loaded while this one computes
       int r_next = 0;
       float rt_next = 0.0f;
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANGLE ? 6 :
         m &= m - 1;
         const int r = r_next;
         const float rt = rt_next;
-        const float mr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mr_l), li));
+        const int pass = __builtin_amdgcn_readlane(el_pass, li);
         if (m) {
           r_next = __builtin_amdgcn_readlane(el_r, (int)__builtin_ctzll(m));
           rt_next = q[(size_t)r_next * KQ * K.npts + p];
@@ -398,10 +398,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ANGLE ? 6 :
         unsigned slots = 0;
 #pragma unroll
         for (int k = 0; k < SB; ++k) {
-          tk[k] = __builtin_amdgcn_readlane(el_t[k], li);
           ok[k] = false;
           dv[k] = 0.0f;
-          if (tk[k] < 0 || !(use >> k & 1) || beyond(ms[k], mr, K)) continue;
+          tk[k] = -1;
+          if (!(pass >> k & 1)) continue;
+          tk[k] = __builtin_amdgcn_readlane(el_t[k], li);
           slots |= 1u << k;
           const float xs = qs[k].t + rt;
           const int it = t_index(xs, K);
