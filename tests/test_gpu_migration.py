@@ -103,8 +103,9 @@ def test_convolution_bit_exact_every_kernel(oracle, torch, adjoint, nt, dt, fmax
 
 # (aperture, dt, nt): the sample case; a negative aperture (negative taper
 # divisors: IEEE division for (x - cmp) / sigma); a time step above 1 s (the
-# time index by IEEE division below 2^-100)
-@pytest.mark.parametrize("aperture,dt,nt", [(30.0, 2e-3, 420), (-30.0, 2e-3, 420), (30.0, 1.25, 3)])
+# time index by IEEE division below 2^-100); the longest record
+@pytest.mark.parametrize("aperture,dt,nt", [(30.0, 2e-3, 420), (-30.0, 2e-3, 420), (30.0, 1.25, 3),
+                                              (30.0, 5e-5, 16384)])
 def test_image_adjoint_matches_oracle(oracle, setup, torch, aperture, dt, nt):
     st = setup
     K = make(st["e"], st["m"], st["xs"], st["xr"], mig.IMAGE, nt=nt, dt=dt, aperture=aperture,
@@ -122,7 +123,7 @@ def test_image_adjoint_matches_oracle(oracle, setup, torch, aperture, dt, nt):
     K.close()
 
 
-@pytest.mark.parametrize("nt", [420, 5000, 9000])   # four, two and one trace(s) per forward workgroup
+@pytest.mark.parametrize("nt", [420, 5000, 9000, 16384])   # 16384: the largest record (a 64 KB LDS histogram)
 @pytest.mark.parametrize("domain", [mig.IMAGE, mig.ANGLE])
 def test_forward_matches_oracle(oracle, setup, torch, domain, nt):
     st = setup
