@@ -31,7 +31,8 @@
 //     the reference (their order varies);
 //   - forward: one workgroup (sixteen waves) per trace accumulates the trace
 //     in an LDS histogram of its nt samples (LDS atomics) and adds it to the
-//     trace once;
+//     trace once; the tables' minima over 64 chunks let it pass over 4096
+//     image points with one test;
 //   - convolution: the FFT product of the reference is a linear
 //     (cross-)correlation with the wavelet, summed directly in double, the
 //     trace and the wavelet staged in LDS.
@@ -56,6 +57,7 @@ constexpr int KQ = 7;             // per-table quantities: T, amplitude, cos and
 struct KdmK {
   int nz, nx, nzz, npts, nt, nang;
   int nchunk;     // 64-node chunks of the image points (one wave each)
+  int nsup;       // super-chunks of 64 chunks (4096 image points)
   float thr;      // least x with RN(x / dt) >= nt: a traveltime sum x >= thr has time index >= nt
   float zh_dt, zl_dt;  // two-word reciprocal of dt (fdiv)
   int fastdiv_dt;      // fdiv(x, zh_dt, zl_dt) == RN(x / dt) for every 2^-100 <= x < 2^100 (checked exhaustively)
@@ -146,6 +148,24 @@ __global__ __launch_bounds__(256) void k_kdm_prep(const float* __restrict__ tabl
     }
     for (int off = 32; off >= 1; off >>= 1) tv = min_nan(tv, __shfl_xor(tv, off, 64));
     if (lane == 0) tmin[w] = tv;
+  }
+}
+
+// Per table, the minimum of the chunk minima over each super-chunk of 64
+// chunks (NaN-propagating): one wave per super-chunk.  The super-chunk minima
+// bound every chunk's from below, so a super-chunk whose minima give a time
+// index >= nt holds no chunk that reaches the trace.
+__global__ __launch_bounds__(256) void k_kdm_tmin2(const float* __restrict__ tmin, int ntab, KdmK K,
+                                                   float* __restrict__ tmin2) {
+  const int lane = threadIdx.x & 63;
+  const size_t nw = (size_t)ntab * K.nsup;
+  const size_t wstride = (size_t)gridDim.x * (blockDim.x >> 6);
+  for (size_t w = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nw; w += wstride) {
+    const int tab = (int)(w / K.nsup);
+    const int c = (int)(w - (size_t)tab * K.nsup) * 64 + lane;
+    float v = c < K.nchunk ? tmin[(size_t)tab * K.nchunk + c] : __builtin_inff();
+    for (int off = 32; off >= 1; off >>= 1) v = min_nan(v, __shfl_xor(v, off, 64));
+    if (lane == 0) tmin2[w] = v;
   }
 }
 
@@ -457,8 +477,9 @@ __global__ __launch_bounds__(256) void k_kdm_sum_parts(float* __restrict__ model
 }
 
 // Forward (image or angle domain): one workgroup per trace, the trace's nt
-// samples in LDS; each wave passes over a chunk the minima keep beyond the
-// last sample with two scalar loads.
+// samples in LDS; a wave passes over 4096 image points whose super-chunk
+// minima keep them beyond the last sample with one test, then over 64 when
+// the chunk minima do.
 // Threads per trace (one workgroup, sixteen waves sharing the trace's LDS
 // histogram).  Measured on the 1001 x 1001 survey against 256 and 512: the
 // 1 s record's forward 0.224 ms against 0.269 and 0.233 (image), 0.32 against
@@ -472,7 +493,8 @@ template <bool ANGLE, bool RDIV>
 __global__ __launch_bounds__(KDM_FWD_THREADS) void k_kdm_forward(const float* __restrict__ q, const float* __restrict__ tmin,
                                                      const float* __restrict__ S, const int* __restrict__ tsrc,
                                                      const int* __restrict__ trec, const float* __restrict__ tcmp,
-                                                     const int* __restrict__ tcid, int nsrc,
+                                                     const int* __restrict__ tcid, const float* __restrict__ tmin2,
+                                                     int nsrc,
                                                      const float* __restrict__ model, float* __restrict__ data,
                                                      KdmK K) {
   extern __shared__ float hist[];
@@ -483,36 +505,42 @@ __global__ __launch_bounds__(KDM_FWD_THREADS) void k_kdm_forward(const float* __
   const float cmp = tcmp[t];
   const size_t moff = ANGLE ? (size_t)tcid[t] * K.nang * K.nz : 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  // each wave takes groups of 64 chunks, tests them one per lane and works
-  // through the chunks that pass
-  for (int c0 = wave * 64; c0 < K.nchunk; c0 += nwaves * 64) {
-    const int cl = c0 + lane;
-    const bool need = cl < K.nchunk && !beyond(tmin[(size_t)s * K.nchunk + cl], tmin[(size_t)r * K.nchunk + cl], K);
-    for (uint64_t m = __ballot(need); m; m &= m - 1) {
-      const int p = (c0 + __builtin_ctzll(m)) * 64 + lane;
-      if (p >= K.npts) continue;
-      Q7 qs, qr;
-      qs.t = q[(size_t)s * KQ * K.npts + p];
-      qr.t = q[(size_t)r * KQ * K.npts + p];
-      const float xs = qs.t + qr.t;
-      const int it = t_index(xs, K);
-      if ((unsigned)it >= (unsigned)K.nt) continue;
-      q_amp(q, s, p, K.npts, qs);
-      q_amp(q, r, p, K.npts, qr);
-      if (ANGLE) {
-        q_dir(q, s, p, K.npts, qs);
-        q_dir(q, r, p, K.npts, qr);
-      }
-      const int i = p % K.nz, j = p / K.nz;
-      const float z = (float)i * K.dz, x = (float)j * K.dx;
-      const float sS = K.scale * S[(i + 3) + (size_t)(j + 3) * K.nzz];
-      const float sig_e = K.tan_ap * z + KDM_EPS;
-      const float w = kdm_weight<RDIV>(qs, qr, sS, x, sig_e, recip_refined(sig_e), cmp);
-      if (ANGLE) {
-        const int a = kdm_angle_bin(qs, qr, K);
-        if (a >= 0 && a < K.nang) atomicAdd(hist + it, w * model[moff + i + (size_t)a * K.nz]);
-      } else {
-        atomicAdd(hist + it, w * model[p]);
+  // each wave takes super-chunks wave, wave + nwaves, ..., tests them 64 at a
+  // time (one per lane), and in each that passes tests its 64 chunks (one per
+  // lane) and works through the chunks that pass
+  for (int k0 = 0; wave + k0 * nwaves < K.nsup; k0 += 64) {
+    const int sl = wave + (k0 + lane) * nwaves;
+    const bool need2 = sl < K.nsup && !beyond(tmin2[(size_t)s * K.nsup + sl], tmin2[(size_t)r * K.nsup + sl], K);
+    for (uint64_t m2 = __ballot(need2); m2; m2 &= m2 - 1) {
+      const int c0 = (wave + (k0 + (int)__builtin_ctzll(m2)) * nwaves) * 64;
+      const int cl = c0 + lane;
+      const bool need = cl < K.nchunk && !beyond(tmin[(size_t)s * K.nchunk + cl], tmin[(size_t)r * K.nchunk + cl], K);
+      for (uint64_t m = __ballot(need); m; m &= m - 1) {
+        const int p = (c0 + __builtin_ctzll(m)) * 64 + lane;
+        if (p >= K.npts) continue;
+        Q7 qs, qr;
+        qs.t = q[(size_t)s * KQ * K.npts + p];
+        qr.t = q[(size_t)r * KQ * K.npts + p];
+        const float xs = qs.t + qr.t;
+        const int it = t_index(xs, K);
+        if ((unsigned)it >= (unsigned)K.nt) continue;
+        q_amp(q, s, p, K.npts, qs);
+        q_amp(q, r, p, K.npts, qr);
+        if (ANGLE) {
+          q_dir(q, s, p, K.npts, qs);
+          q_dir(q, r, p, K.npts, qr);
+        }
+        const int i = p % K.nz, j = p / K.nz;
+        const float z = (float)i * K.dz, x = (float)j * K.dx;
+        const float sS = K.scale * S[(i + 3) + (size_t)(j + 3) * K.nzz];
+        const float sig_e = K.tan_ap * z + KDM_EPS;
+        const float w = kdm_weight<RDIV>(qs, qr, sS, x, sig_e, recip_refined(sig_e), cmp);
+        if (ANGLE) {
+          const int a = kdm_angle_bin(qs, qr, K);
+          if (a >= 0 && a < K.nang) atomicAdd(hist + it, w * model[moff + i + (size_t)a * K.nz]);
+        } else {
+          atomicAdd(hist + it, w * model[p]);
+        }
       }
     }
   }
@@ -594,6 +622,7 @@ struct seisfat_kdm_s {
   KdmK K{};
   float* d_q = nullptr;    // per-table quantities, (nsrc + nrec) x 7 x npts
   float* d_tmin = nullptr; // per-table chunk minima of T, (nsrc + nrec) x nchunk
+  float* d_tmin2 = nullptr; // per-table minima over 64 chunks, (nsrc + nrec) x nsup
   int* d_src = nullptr;    // trace tables
   int* d_rec = nullptr;
   int nb = 0;              // source blocks of the adjoint
@@ -673,6 +702,10 @@ void launch_prep(seisfat_kdm_s* k, const float* d_tables, int tab0, int ntab, hi
   const int blocks = (int)std::min<size_t>((nw + 3) / 4, 16384);
   hipLaunchKernelGGL(k_kdm_prep, dim3(blocks), dim3(256), 0, st, d_tables, ntab, k->h->d_spad, k->K,
                      k->d_q + (size_t)tab0 * KQ * k->K.npts, k->d_tmin + (size_t)tab0 * k->K.nchunk);
+  const size_t ns = (size_t)ntab * k->K.nsup;
+  const int blocks2 = (int)std::min<size_t>((ns + 3) / 4, 16384);
+  hipLaunchKernelGGL(k_kdm_tmin2, dim3(blocks2), dim3(256), 0, st, k->d_tmin + (size_t)tab0 * k->K.nchunk, ntab,
+                     k->K, k->d_tmin2 + (size_t)tab0 * k->K.nsup);
 }
 
 // The operators' launches, one instantiation per domain and taper division.
@@ -688,7 +721,8 @@ template <bool ANGLE, bool RDIV>
 void launch_forward(seisfat_kdm_s* k, const float* d_model, float* d_data, hipStream_t st) {
   const size_t lds = (size_t)k->K.nt * sizeof(float);
   hipLaunchKernelGGL((k_kdm_forward<ANGLE, RDIV>), dim3(k->ntr), dim3(KDM_FWD_THREADS), lds, st, k->d_q, k->d_tmin,
-                     k->h->d_spad, k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->nsrc, d_model, d_data, k->K);
+                     k->h->d_spad, k->d_src, k->d_rec, k->d_cmp, k->d_cid, k->d_tmin2, k->nsrc, d_model, d_data,
+                     k->K);
 }
 
 bool cmp_ok(float c) { return c == 0.0f || (std::fabs(c) >= 0x1p-36f && std::fabs(c) < 0x1p40f); }
@@ -747,6 +781,7 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
   K.nzz = h->nzz;
   K.npts = (int)npts;
   K.nchunk = (int)((npts + 63) / 64);
+  K.nsup = (K.nchunk + 63) / 64;
   K.rec_nonneg = 0;
   K.nt = p->nt;
   K.ntf = (float)p->nt;
@@ -766,6 +801,7 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
   K.scale = (float)(1.0 / (2.0 * M_PI));
   hipError_t e = hipMalloc((void**)&k->d_q, (size_t)(nsrc + nrec) * KQ * npts * sizeof(float));
   if (e == hipSuccess) e = hipMalloc((void**)&k->d_tmin, (size_t)(nsrc + nrec) * K.nchunk * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&k->d_tmin2, (size_t)(nsrc + nrec) * K.nsup * sizeof(float));
   if (e == hipSuccess) {
     // exhaustive check of the known-divisor division by dt (every dividend of
     // magnitude in [2^-100, 2^100)) against IEEE division
@@ -786,6 +822,7 @@ int seisfat_kdm_create(seisfat_eik_t h, const seisfat_kdm_params* p, int nsrc, i
   if (e != hipSuccess) {
     kfree(k->d_q);
     kfree(k->d_tmin);
+    kfree(k->d_tmin2);
     delete k;
     return kfail(SEISFAT_ENOMEM, std::string("hipMalloc of the table quantities: ") + hipGetErrorString(e));
   }
@@ -981,6 +1018,7 @@ int seisfat_kdm_destroy(seisfat_kdm_t k) {
   free_traces(k);
   kfree(k->d_q);
   kfree(k->d_tmin);
+  kfree(k->d_tmin2);
   delete k;
   return SEISFAT_OK;
 }
